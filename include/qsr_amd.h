@@ -1,0 +1,168 @@
+/*
+ * qsr_amd.h -- C ABI of the MI355X-native QSR Landau-de Gennes relaxation /
+ * quench engine (libqsr_amd.so).
+ *
+ * The reference has no library API: its hot path is the chain of __global__
+ * kernels declared at QSR.cuh:98-133 and launched per iteration from the host
+ * loops in QSR.cu (sim_mode 1: QSR.cu:3220-3339, sim_mode 3: QSR.cu:2708-2964,
+ * periodic bulk: KZM_bulk_ldg.cu:918-971).  This header replaces that chain with
+ * a context object: the caller describes one box (or a batch of boxes), uploads
+ * a field in the reference's interchange format (AoS double {Qxx,Qxy,Qxz,Qyy,Qyz},
+ * index k + Nz*(j + Ny*i), QSR.cuh:22-44 / QSR.cu:286), advances it, and reads
+ * observables and fields back.
+ *
+ * Conventions: every function returns 0 on success and a negative QSR_E* code
+ * on failure (qsr_last_error() gives the message for the calling thread).  The
+ * library never calls exit().  The context owns device memory; the caller owns
+ * every host buffer.  One host thread per context; all work is ordered on the
+ * context's HIP stream.  Plain pointers and sizes only.
+ */
+#ifndef QSR_AMD_H
+#define QSR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QSR_OK 0
+#define QSR_E_ARG (-1)       /* invalid argument / size mismatch */
+#define QSR_E_HIP (-2)       /* HIP runtime failure (message names the call) */
+#define QSR_E_GEOMETRY (-3)  /* droplet mask not representable by row intervals */
+#define QSR_E_STATE (-4)     /* call out of order (e.g. step before upload) */
+#define QSR_E_NODEVICE (-5)  /* no HIP device */
+
+/* Temperature protocol of one box (QSR.cu:2437-2449, 2679-2693). */
+#define QSR_PROTOCOL_CONSTANT 0 /* sim_mode=1 single temperature: T = T_high */
+#define QSR_PROTOCOL_STEP 1     /* T_high for pre_equil_iters, then T_low */
+#define QSR_PROTOCOL_RAMP 2     /* T_high, then linear to T_low over ramp_iters */
+
+typedef struct qsr_params {
+    /* grid (QSR.cu:1970-1975) */
+    int32_t Nx, Ny, Nz;
+    double dx, dy, dz;
+    /* bulk LdG (QSR.cu:1978-1983); b > 0 is the reference's "-B/3 trQ^3" convention */
+    double a, b, c, T_star;
+    int32_t modechoice; /* 1 = std, 2 = ravnik (QSR.cu:1363-1378) */
+    /* elastic (QSR.cu:2010, 2079-2082): kappa used only when L1 = L2 = L3 = 0 */
+    double kappa, L1, L2, L3;
+    /* weak anchoring W (J/m^2; 0 = strong, QSR.cu:2113) and viscosity gamma */
+    double W, gamma;
+    double dt;          /* time step (s); the caller applies the dt_max clamp */
+    double q_norm_cap;  /* Frobenius cap (QSR.cu:2611); 0 = off */
+    /* geometry: 0 = confined droplet (QSR.cu:144-162), 1 = periodic bulk box
+     * without droplet (KZM_bulk_ldg.cu:351-469) */
+    int32_t periodic;
+    double radius_index;     /* droplet radius in voxels; <= 0 -> 0.5*(min N - 1) */
+    double center_index[3];  /* droplet centre in voxel units; NaN -> (N-1)/2 */
+    /* temperature protocol and shell order policy (QSR.cu:2233-2283) */
+    int32_t protocol;        /* QSR_PROTOCOL_* */
+    double T_high, T_low;    /* CONSTANT uses T_high */
+    int32_t pre_equil_iters, ramp_iters;
+    int32_t boundary_custom; /* 0 = equilibrium S_shell = max(S_eq(T), 0); 1 = custom */
+    double boundary_S;       /* custom shell order */
+    /* additive Gaussian noise per component per step, Philox4x32-10 keyed by
+     * (seed, voxel, iteration); sigma = 0 disables (KZM_bulk_ldg.cu:441-454) */
+    double noise_sigma;
+    uint64_t seed;
+} qsr_params;
+
+/* Log-point observables of one box (the quench_log / free_energy_vs_iteration
+ * columns, QSR.cu:2677, 2787-2882, 3218).  FP64, reduced on the GPU. */
+typedef struct qsr_obs {
+    int64_t iteration;      /* index of the last completed step (-1 before any) */
+    double T, S_shell;      /* temperature / shell order of that step */
+    double bulk, elastic, anchoring, total; /* J (droplet) or J/m^3 (periodic) */
+    double radiality;       /* <|n . r_hat|> (QSR.cu:994-1015) */
+    double avg_S, max_S;    /* over sd < -2*min_spacing (droplet) or all sites */
+    double checker_rel;     /* NaN when not evaluated (QSR.cu:2846-2873) */
+    uint64_t defects;       /* plaquettes with |s| > cutoff */
+    uint64_t plaquettes;    /* plaquettes used */
+    double defect_density;  /* defects / plaquettes (NaN if none) */
+    double xi_grad_proxy;   /* QSR.cu:2372-2410 (droplet) / KZM_bulk_ldg.cu:709-711 */
+    uint64_t xi_edges;
+    double defect_line_density; /* periodic box only (KZM_bulk_ldg.cu:706-708) */
+} qsr_obs;
+
+typedef struct qsr_obs_opts {
+    int32_t z_slice;          /* defects_z_slice; < 0 -> Nz/2 (QSR.cu:2646) */
+    double S_threshold;       /* defects_S_threshold (0.1) */
+    double charge_cutoff;     /* defects_charge_cutoff (0.25) */
+    int32_t xi_z_slice;       /* < 0 -> z_slice */
+    double xi_S_threshold;    /* < 0 -> S_threshold */
+    int32_t want_defects, want_xi; /* 0 skips that pass (columns -> NaN / 0) */
+} qsr_obs_opts;
+
+typedef struct qsr_box_info {
+    uint64_t voxels;          /* Nx*Ny*Nz */
+    uint64_t shell_voxels;    /* |sd| <= shell_half_thickness (QSR.cu:2160-2172) */
+    uint64_t active_voxels;   /* voxels that can change value in a step */
+    double shell_volume;      /* m^3 */
+    double surface_area;      /* Thomsen estimate, m^2 (QSR.cu:164-172) */
+    double nominal_thickness; /* 2*shell_half_thickness */
+    double effective_thickness; /* delta_eff = V_shell / A (QSR.cu:174-185) */
+    double S_eq;              /* at T_high */
+    int64_t iteration;        /* steps taken so far */
+} qsr_box_info;
+
+/* Context lifetime.  qsr_create_batch packs n boxes (each its own geometry,
+ * physics, protocol and seed) into one device field advanced by one launch per
+ * step -- the batched quench-rate ladder.  Replaces the per-run cudaMalloc x18
+ * block (QSR.cu:2202-2219) and __constant__ geometry upload (QSR.cu:2154). */
+int qsr_create(const qsr_params* p, int device, void** ctx);
+int qsr_create_batch(const qsr_params* p, int n_boxes, int device, void** ctx);
+int qsr_destroy(void* ctx);
+const char* qsr_last_error(void);
+int qsr_device_count(int* n);
+
+/* Field transfer in the interchange format (replaces the H2D/D2H cudaMemcpy of
+ * QSR.cu:2516, 3174, 2974, 3468).  n_voxels must equal the box's Nx*Ny*Nz. */
+int qsr_upload_q(void* ctx, int box, const double* aos5, size_t n_voxels);
+int qsr_download_q(void* ctx, int box, double* aos5, size_t n_voxels);
+/* Nematic field {S, nx, ny, nz} per voxel, FP64 (computeNematicFieldKernel,
+ * QSR.cu:1017-1025) */
+int qsr_download_nematic(void* ctx, int box, double* out4, size_t n_voxels);
+
+/* Advance every box by nsteps explicit-Euler iterations (one fused launch per
+ * step; replaces QSR.cu:3223-3238 / 2713-2753 / KZM_bulk_ldg.cu:958-970). */
+int qsr_step(void* ctx, int nsteps);
+/* Same, but captured once into a hipGraph of `chunk` steps and replayed. */
+int qsr_step_graph(void* ctx, int nsteps, int chunk);
+/* Semi-implicit IMEX iteration: explicit step with -L_stab*lap moved to the
+ * left, then jacobi_iters Jacobi sweeps (QSR.cu:734-849, 2721-2749). */
+int qsr_step_imex(void* ctx, int nsteps, double L_stab, int jacobi_iters);
+
+/* FP64 log-point observables (replaces reduce_energy, radiality, defect and xi
+ * passes with their full-array D2H copies, QSR.cu:2323-2410, 2787-2874). */
+int qsr_observables(void* ctx, int box, const qsr_obs_opts* opts, qsr_obs* out);
+int qsr_box_info_get(void* ctx, int box, qsr_box_info* out);
+/* Iteration counter of a box (T(iter) follows the protocol). */
+int qsr_set_iteration(void* ctx, int box, int64_t iteration);
+
+/* Instrumentation for measurement: average duration (ms) of the last `qsr_step`
+ * call's per-step launches, measured with hipEvents on the context stream. */
+int qsr_last_step_ms(void* ctx, double* ms_per_step);
+/* Work description of one step: bytes moved per step if each active voxel is
+ * read and written once (FP32 SoA, 40 B/voxel), voxels covered by launched
+ * tiles, and box voxels. */
+int qsr_step_work(void* ctx, uint64_t* active_voxels, uint64_t* tile_voxels, uint64_t* box_voxels);
+
+/* Host physics shared by the CLI and bindings (QSR.cu:1363-1604). */
+double qsr_S_eq(double a, double b, double c, double T, double T_star, int modechoice);
+/* out: min_dx, S_scale, L_eff_total, D_total, dt_diff_total, L_eff_explicit,
+ *      dt_diff, bulk_rate, dt_bulk, dt_max (estimate_stable_dt_window) */
+int qsr_dt_estimate(double dx, double dy, double dz, double gamma, double a, double b, double c,
+                    double T_star, int modechoice, double kappa, double L1, double L2, double L3,
+                    double S_ref_used, double S0, double T_a, double T_b, int use_semi_implicit,
+                    double L_stab, int jacobi_iters, double out[10]);
+/* Seeded initial states in the reference draw order (QSR.cu:1841-1904):
+ * mode 0 random director, 1 radial, 2 isotropic + N(0, noise_amplitude). */
+int qsr_init_field(const qsr_params* p, int mode, double S0, double noise_amplitude,
+                   int64_t random_seed, double* aos5_out, size_t n_voxels);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QSR_AMD_H */

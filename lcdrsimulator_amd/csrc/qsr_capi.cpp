@@ -1,0 +1,750 @@
+// qsr_capi.cpp -- the C ABI (include/qsr_amd.h): context lifetime, field
+// transfer, the step loop and the log-point observables.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/qsr_amd.h"
+#include "qsr_host.hpp"
+#include "qsr_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) return fail(QSR_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Release-path calls whose status cannot be acted on (destructors, cleanup
+// after an earlier error) go through this to make the discard explicit.
+inline void discard(hipError_t) {}
+
+constexpr double kNaN = std::numeric_limits<double>::quiet_NaN();
+constexpr int kDynChunk = 512;  // steps per dynamic-table upload
+
+struct Box {
+    qsr_params p{};
+    qsrh::Geometry g;
+    qsrh::RowMasks masks;
+    double thickness = 0, area = 0;
+    long long off = 0, nvox = 0, rows_off = 0, nface = 0;
+    int64_t iter = 0;  // steps completed
+    double last_T = 0, last_S = 0;
+    bool uploaded = false;
+    double m = 0;  // dt / gamma
+
+    double T_at(int64_t it) const {
+        // compute_T_current (QSR.cu:2679-2687)
+        if (p.protocol == QSR_PROTOCOL_CONSTANT) return p.T_high;
+        if (it < p.pre_equil_iters) return p.T_high;
+        if (p.protocol == QSR_PROTOCOL_STEP) return p.T_low;
+        const int64_t t = it - p.pre_equil_iters;
+        if (t <= 0) return p.T_high;
+        if (p.ramp_iters <= 0 || t >= p.ramp_iters) return p.T_low;
+        return p.T_high + (p.T_low - p.T_high) * (double(t) / double(p.ramp_iters));
+    }
+    double S_at(double T) const {
+        if (p.periodic) return 0.0;
+        return qsrh::shell_order(p.boundary_custom != 0, p.boundary_S, p.a, p.b, p.c, T, p.T_star, p.modechoice);
+    }
+    qsrk::StepDyn dyn_at(int64_t it) const {
+        const double T = T_at(it);
+        const qsrh::BulkABC k = qsrh::bulk_abc(p.a, p.b, p.c, T, p.T_star, p.modechoice);
+        qsrk::StepDyn d{};
+        d.cA = float(m * k.A);
+        d.S_shell = float(S_at(T));
+        d.iter = unsigned(it);
+        return d;
+    }
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    std::vector<Box> boxes;
+    long long cs = 0;  // floats per component plane
+    float* q[2] = {nullptr, nullptr};
+    int cur = 0;
+    bool full = false, noise = false;
+    int ch = 32;
+    std::vector<qsrk::BoxDev> h_boxes;
+    qsrk::BoxDev* d_boxes = nullptr;
+    std::vector<int4> h_tiles;
+    int4* d_tiles = nullptr;
+    int ntiles = 0;
+    long long tile_voxels = 0, active_voxels = 0, box_voxels = 0;
+    qsrk::RowMask* d_step_rows = nullptr;
+    qsrk::RowMask* d_obs_rows = nullptr;
+    long long* d_face_prefix = nullptr;
+    long long nface = 0;
+    qsrk::StepDyn* d_dyn = nullptr;
+    qsrk::StepDyn* h_dyn = nullptr;  // pinned
+    double* d_part = nullptr;
+    long long part_cap = 0;
+    double* d_out = nullptr;
+    double* d_nf_slice[2] = {nullptr, nullptr};
+    long long nf_slice_cap = 0;
+    double* d_nf3 = nullptr;
+    long long nf3_cap = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double last_ms = 0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_chunk = 0, graph_cur = -1;
+
+    ~Ctx() {
+        if (gexec) discard(hipGraphExecDestroy(gexec));
+        if (graph) discard(hipGraphDestroy(graph));
+        for (float* p : q) discard(hipFree(p));
+        for (void* p : {(void*)d_boxes, (void*)d_tiles, (void*)d_step_rows, (void*)d_obs_rows, (void*)d_face_prefix,
+                        (void*)d_dyn, (void*)d_part, (void*)d_out, (void*)d_nf_slice[0], (void*)d_nf_slice[1],
+                        (void*)d_nf3})
+            discard(hipFree(p));
+        if (h_dyn) discard(hipHostFree(h_dyn));
+        for (hipEvent_t e : ev)
+            if (e) discard(hipEventDestroy(e));
+        if (st) discard(hipStreamDestroy(st));
+    }
+};
+
+int check_params(const qsr_params& p, int idx) {
+    auto bad = [&](const char* what) {
+        return fail(QSR_E_ARG, "box " + std::to_string(idx) + ": " + what);
+    };
+    if (p.Nx < 3 || p.Ny < 3 || p.Nz < 3) return bad("Nx, Ny, Nz must be >= 3");
+    if (p.Nz > 32767) return bad("Nz > 32767 not supported");
+    if (!(p.dx > 0 && p.dy > 0 && p.dz > 0)) return bad("spacings must be positive");
+    if (!(p.gamma > 0)) return bad("gamma must be positive");
+    if (!(p.dt > 0) || !std::isfinite(p.dt)) return bad("dt must be positive");
+    if (p.protocol < 0 || p.protocol > 2) return bad("unknown protocol");
+    if (p.noise_sigma < 0) return bad("noise_sigma must be >= 0");
+    return QSR_OK;
+}
+
+qsrk::BoxDev make_boxdev(const Box& b) {
+    const qsr_params& p = b.p;
+    qsrk::BoxDev d{};
+    d.Nx = p.Nx; d.Ny = p.Ny; d.Nz = p.Nz;
+    d.off = b.off;
+    d.rows_off = b.rows_off;
+    const double m = b.m;
+    const qsrh::BulkABC k = qsrh::bulk_abc(p.a, p.b, p.c, p.T_high, p.T_star, p.modechoice);
+    d.cB = float(m * k.B);
+    d.cC = float(m * k.C);
+    const bool ldg = (p.L1 != 0.0 || p.L2 != 0.0 || p.L3 != 0.0);
+    const double Lk = ldg ? p.L1 : p.kappa;
+    d.lx = float(m * Lk / (p.dx * p.dx));
+    d.ly = float(m * Lk / (p.dy * p.dy));
+    d.lz = float(m * Lk / (p.dz * p.dz));
+    int fl = 0;
+    if (p.periodic) {
+        fl |= qsrk::F_PERIODIC;
+    } else {
+        if (p.W != 0.0 && b.thickness > 0.0) {
+            fl |= qsrk::F_ANCHOR;
+            d.cW = float(m * p.W / b.thickness);
+        }
+        if (!(p.W > 0.0)) fl |= qsrk::F_BC;
+        if (p.W == 0.0) fl |= qsrk::F_SKIPSHELL;
+    }
+    if (p.noise_sigma > 0.0) fl |= qsrk::F_NOISE;
+    if (p.q_norm_cap > 0.0) {
+        fl |= qsrk::F_CAP;
+        d.cap2 = float(p.q_norm_cap * p.q_norm_cap);
+    }
+    if (p.L2 != 0.0) fl |= qsrk::F_L2;
+    if (p.L3 != 0.0) fl |= qsrk::F_L3;
+    d.flags = fl;
+    d.sigma = float(p.noise_sigma);
+    d.ci = float(b.g.cx / p.dx);
+    d.cj = float(b.g.cy / p.dy);
+    d.ck = float(b.g.cz / p.dz);
+    d.sx = float(p.dx / (b.g.rx * b.g.rx));
+    d.sy = float(p.dy / (b.g.ry * b.g.ry));
+    d.sz = float(p.dz / (b.g.rz * b.g.rz));
+    d.hx = float(1.0 / (2.0 * p.dx)); d.hy = float(1.0 / (2.0 * p.dy)); d.hz = float(1.0 / (2.0 * p.dz));
+    d.fx = float(1.0 / p.dx); d.fy = float(1.0 / p.dy); d.fz = float(1.0 / p.dz);
+    d.ix2 = float(1.0 / (p.dx * p.dx)); d.iy2 = float(1.0 / (p.dy * p.dy)); d.iz2 = float(1.0 / (p.dz * p.dz));
+    d.i4xy = float(1.0 / (4.0 * p.dx * p.dy));
+    d.i4xz = float(1.0 / (4.0 * p.dx * p.dz));
+    d.i4yz = float(1.0 / (4.0 * p.dy * p.dz));
+    d.mL2 = float(m * p.L2);
+    d.mL3 = float(m * p.L3);
+    d.seed = p.seed;
+    return d;
+}
+
+// Does any interior output voxel of the tile change in a step?
+bool tile_active(const Box& b, int i0, int j0, int k0, int ch) {
+    const qsr_params& p = b.p;
+    if (p.periodic) return true;
+    const int ib = std::max(i0, 1), ie = std::min(i0 + ch, p.Nx - 1);
+    const int jb = std::max(j0, 1), je = std::min(j0 + qsrk::TJ, p.Ny - 1);
+    const int kb = std::max(k0, 1), ke = std::min(k0 + qsrk::TK, p.Nz - 1) - 1;
+    const bool bc = !(p.W > 0.0);
+    for (int i = ib; i < ie; ++i)
+        for (int j = jb; j < je; ++j) {
+            const int16_t* s = &b.masks.step[(size_t(i) * p.Ny + j) * 8];
+            // evolving interval, and the shell (where BC or the penalty acts)
+            const int lo = std::max<int>(kb, s[0]), hi = std::min<int>(ke, s[1]);
+            if (lo <= hi) return true;
+            if (bc) {
+                const int olo = std::max<int>(kb, s[2]), ohi = std::min<int>(ke, s[3]);
+                for (int k = olo; k <= ohi; ++k)
+                    if (!(k >= s[4] && k <= s[5])) return true;
+            }
+        }
+    return false;
+}
+
+int build_tiles(Ctx& c) {
+    std::vector<int4> nat;
+    c.tile_voxels = 0;
+    for (size_t bi = 0; bi < c.boxes.size(); ++bi) {
+        const Box& b = c.boxes[bi];
+        for (int i0 = 0; i0 < b.p.Nx; i0 += c.ch)
+            for (int j0 = 0; j0 < b.p.Ny; j0 += qsrk::TJ)
+                for (int k0 = 0; k0 < b.p.Nz; k0 += qsrk::TK)
+                    if (tile_active(b, i0, j0, k0, c.ch)) {
+                        nat.push_back(make_int4(int(bi), i0, j0, k0));
+                        c.tile_voxels += (long long)std::min(c.ch, b.p.Nx - i0) * std::min(qsrk::TJ, b.p.Ny - j0) *
+                                         std::min(qsrk::TK, b.p.Nz - k0);
+                    }
+    }
+    // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs,
+    // so launch position 8*r + x carries tile x*q + r: each XCD (and its L2)
+    // sweeps one contiguous run of neighbouring tiles and shares their halos.
+    const int T = int(nat.size());
+    const int q = (T + 7) / 8;
+    c.h_tiles.assign(size_t(8) * q, make_int4(-1, 0, 0, 0));
+    for (int x = 0; x < 8; ++x)
+        for (int r = 0; r < q; ++r) {
+            const int n = x * q + r;
+            if (n < T) c.h_tiles[size_t(8) * r + x] = nat[n];
+        }
+    c.ntiles = int(c.h_tiles.size());
+    discard(hipFree(c.d_tiles));
+    c.d_tiles = nullptr;
+    if (c.ntiles == 0) return QSR_OK;
+    HIP_TRY(hipMalloc(&c.d_tiles, sizeof(int4) * c.ntiles));
+    HIP_TRY(hipMemcpy(c.d_tiles, c.h_tiles.data(), sizeof(int4) * c.ntiles, hipMemcpyHostToDevice));
+    return QSR_OK;
+}
+
+Ctx* as_ctx(void* p) { return static_cast<Ctx*>(p); }
+
+int create_impl(const qsr_params* params, int n, int device, void** out) {
+    if (!params || n < 1 || !out) return fail(QSR_E_ARG, "null argument or n_boxes < 1");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(QSR_E_NODEVICE, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(QSR_E_ARG, "device index out of range");
+    for (int i = 0; i < n; ++i) {
+        const int rc = check_params(params[i], i);
+        if (rc) return rc;
+    }
+    auto c = std::make_unique<Ctx>();
+    c->device = device;
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&c->ev[0]));
+    HIP_TRY(hipEventCreate(&c->ev[1]));
+    long long off = 0, rows_off = 0, nface = 0;
+    std::vector<long long> face_prefix{0};
+    std::vector<qsrk::RowMask> step_rows, obs_rows;
+    c->boxes.resize(n);
+    c->active_voxels = 0;
+    c->box_voxels = 0;
+    for (int i = 0; i < n; ++i) {
+        Box& b = c->boxes[i];
+        b.p = params[i];
+        const qsr_params& p = b.p;
+        b.m = p.dt / p.gamma;
+        b.g = qsrh::make_geometry(p.Nx, p.Ny, p.Nz, p.dx, p.dy, p.dz, p.radius_index, p.center_index);
+        b.nvox = (long long)p.Nx * p.Ny * p.Nz;
+        b.off = off;
+        off += (b.nvox + 63) / 64 * 64;
+        b.rows_off = rows_off;
+        rows_off += (long long)p.Nx * p.Ny;
+        if (!p.periodic) {
+            std::string err;
+            if (!qsrh::build_row_masks(b.g, p.W, b.masks, err)) return fail(QSR_E_GEOMETRY, err);
+            b.thickness = qsrh::shell_thickness(b.g, b.masks.shell_count);
+            b.area = qsrh::surface_area(b.g);
+            b.nface = 2LL * p.Ny * p.Nz + 2LL * (p.Nx - 2) * p.Nz + 2LL * (p.Nx - 2) * (p.Ny - 2);
+            c->active_voxels += b.masks.active_count;
+        } else {
+            b.masks.step.assign(size_t(p.Nx) * p.Ny * 8, 0);
+            b.masks.obs.assign(size_t(p.Nx) * p.Ny * 8, 0);
+            b.nface = 0;
+            c->active_voxels += b.nvox;
+        }
+        c->box_voxels += b.nvox;
+        nface += b.nface;
+        face_prefix.push_back(nface);
+        const size_t rows = size_t(p.Nx) * p.Ny;
+        for (size_t r = 0; r < rows; ++r) {
+            qsrk::RowMask s{}, o{};
+            for (int t = 0; t < 8; ++t) {
+                s.v[t] = b.masks.step[r * 8 + t];
+                o.v[t] = b.masks.obs[r * 8 + t];
+            }
+            step_rows.push_back(s);
+            obs_rows.push_back(o);
+        }
+        b.last_T = b.T_at(0);
+        b.last_S = b.S_at(b.last_T);
+        if (p.L2 != 0.0 || p.L3 != 0.0) c->full = true;
+        if (p.noise_sigma > 0.0) c->noise = true;
+    }
+    c->cs = std::max<long long>(off, 64);
+    c->nface = nface;
+    for (int s = 0; s < 2; ++s) {
+        HIP_TRY(hipMalloc(&c->q[s], sizeof(float) * 5 * c->cs));
+        HIP_TRY(hipMemsetAsync(c->q[s], 0, sizeof(float) * 5 * c->cs, c->st));
+    }
+    c->h_boxes.resize(n);
+    for (int i = 0; i < n; ++i) c->h_boxes[i] = make_boxdev(c->boxes[i]);
+    HIP_TRY(hipMalloc(&c->d_boxes, sizeof(qsrk::BoxDev) * n));
+    HIP_TRY(hipMemcpy(c->d_boxes, c->h_boxes.data(), sizeof(qsrk::BoxDev) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&c->d_step_rows, sizeof(qsrk::RowMask) * step_rows.size()));
+    HIP_TRY(hipMalloc(&c->d_obs_rows, sizeof(qsrk::RowMask) * obs_rows.size()));
+    HIP_TRY(hipMemcpy(c->d_step_rows, step_rows.data(), sizeof(qsrk::RowMask) * step_rows.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_obs_rows, obs_rows.data(), sizeof(qsrk::RowMask) * obs_rows.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&c->d_face_prefix, sizeof(long long) * face_prefix.size()));
+    HIP_TRY(hipMemcpy(c->d_face_prefix, face_prefix.data(), sizeof(long long) * face_prefix.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&c->d_dyn, sizeof(qsrk::StepDyn) * kDynChunk * n));
+    HIP_TRY(hipHostMalloc((void**)&c->h_dyn, sizeof(qsrk::StepDyn) * kDynChunk * n, hipHostMallocDefault));
+    HIP_TRY(hipMalloc(&c->d_out, sizeof(double) * 32));
+    // planes per tile: aim at >= 2048 tiles (8 per CU) on the largest box
+    long long maxvox = 0;
+    for (const Box& b : c->boxes) maxvox = std::max(maxvox, b.nvox);
+    c->ch = 64;
+    while (c->ch > 8 && maxvox / ((long long)qsrk::TJ * qsrk::TK * c->ch) * (long long)n < 2048) c->ch /= 2;
+    const int rc = build_tiles(*c);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->st));
+    *out = c.release();
+    return QSR_OK;
+}
+
+int ensure_partials(Ctx& c, long long doubles) {
+    if (doubles <= c.part_cap) return QSR_OK;
+    discard(hipFree(c.d_part));
+    c.d_part = nullptr;
+    HIP_TRY(hipMalloc(&c.d_part, sizeof(double) * doubles));
+    c.part_cap = doubles;
+    return QSR_OK;
+}
+
+// Enqueue steps [0, nsteps) of the current chunk (dyn rows already uploaded).
+int enqueue_steps(Ctx& c, int nsteps) {
+    const int nb = int(c.boxes.size());
+    for (int s = 0; s < nsteps; ++s) {
+        const float* qin = c.q[c.cur];
+        float* qout = c.q[1 - c.cur];
+        const qsrk::StepDyn* dyn = c.d_dyn + (size_t)s * nb;
+        if (c.ntiles > 0)
+            HIP_TRY(qsrk::launch_step(c.full, c.noise, c.ntiles, c.st, qin, qout, c.cs, c.d_boxes, c.d_tiles, dyn, nb,
+                                      c.d_step_rows, c.ch));
+        HIP_TRY(qsrk::launch_faces(c.nface, c.st, qin, qout, c.cs, c.d_boxes, dyn, c.d_face_prefix, nb, c.d_step_rows));
+        c.cur = 1 - c.cur;
+    }
+    return QSR_OK;
+}
+
+int fill_dyn(Ctx& c, int nsteps) {
+    const int nb = int(c.boxes.size());
+    for (int s = 0; s < nsteps; ++s)
+        for (int b = 0; b < nb; ++b) c.h_dyn[(size_t)s * nb + b] = c.boxes[b].dyn_at(c.boxes[b].iter + s);
+    HIP_TRY(hipMemcpyAsync(c.d_dyn, c.h_dyn, sizeof(qsrk::StepDyn) * nsteps * nb, hipMemcpyHostToDevice, c.st));
+    return QSR_OK;
+}
+
+void advance_iters(Ctx& c, int nsteps) {
+    for (Box& b : c.boxes) {
+        b.iter += nsteps;
+        b.last_T = b.T_at(b.iter - 1);
+        b.last_S = b.S_at(b.last_T);
+    }
+}
+
+int check_uploaded(Ctx& c) {
+    for (size_t i = 0; i < c.boxes.size(); ++i)
+        if (!c.boxes[i].uploaded) return fail(QSR_E_STATE, "box " + std::to_string(i) + " has no field uploaded");
+    return QSR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* qsr_last_error(void) { return g_err.c_str(); }
+
+int qsr_device_count(int* n) {
+    if (!n) return fail(QSR_E_ARG, "null n");
+    if (hipGetDeviceCount(n) != hipSuccess) *n = 0;
+    return QSR_OK;
+}
+
+int qsr_create(const qsr_params* p, int device, void** ctx) { return create_impl(p, 1, device, ctx); }
+int qsr_create_batch(const qsr_params* p, int n_boxes, int device, void** ctx) {
+    return create_impl(p, n_boxes, device, ctx);
+}
+
+int qsr_destroy(void* ctx) {
+    if (!ctx) return QSR_OK;
+    Ctx* c = as_ctx(ctx);
+    discard(hipSetDevice(c->device));
+    discard(hipStreamSynchronize(c->st));
+    delete c;
+    return QSR_OK;
+}
+
+int qsr_upload_q(void* ctx, int box, const double* aos5, size_t n_voxels) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !aos5 || box < 0 || box >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad argument");
+    Box& b = c->boxes[box];
+    if ((long long)n_voxels != b.nvox) return fail(QSR_E_ARG, "n_voxels does not match Nx*Ny*Nz");
+    HIP_TRY(hipSetDevice(c->device));
+    double* stage = nullptr;
+    HIP_TRY(hipMalloc(&stage, sizeof(double) * 5 * n_voxels));
+    hipError_t e = hipMemcpyAsync(stage, aos5, sizeof(double) * 5 * n_voxels, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = qsrk::launch_aos_to_soa(c->st, stage, c->q[0], c->q[1], c->cs, b.off, b.nvox);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    discard(hipFree(stage));
+    if (e != hipSuccess) return fail(QSR_E_HIP, std::string("upload: ") + hipGetErrorString(e));
+    b.uploaded = true;
+    return QSR_OK;
+}
+
+int qsr_download_q(void* ctx, int box, double* aos5, size_t n_voxels) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !aos5 || box < 0 || box >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad argument");
+    Box& b = c->boxes[box];
+    if ((long long)n_voxels != b.nvox) return fail(QSR_E_ARG, "n_voxels does not match Nx*Ny*Nz");
+    HIP_TRY(hipSetDevice(c->device));
+    double* stage = nullptr;
+    HIP_TRY(hipMalloc(&stage, sizeof(double) * 5 * n_voxels));
+    hipError_t e = qsrk::launch_soa_to_aos(c->st, c->q[c->cur], stage, c->cs, b.off, b.nvox);
+    if (e == hipSuccess) e = hipMemcpyAsync(aos5, stage, sizeof(double) * 5 * n_voxels, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    discard(hipFree(stage));
+    if (e != hipSuccess) return fail(QSR_E_HIP, std::string("download: ") + hipGetErrorString(e));
+    return QSR_OK;
+}
+
+int qsr_download_nematic(void* ctx, int box, double* out4, size_t n_voxels) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !out4 || box < 0 || box >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad argument");
+    Box& b = c->boxes[box];
+    if ((long long)n_voxels != b.nvox) return fail(QSR_E_ARG, "n_voxels does not match Nx*Ny*Nz");
+    HIP_TRY(hipSetDevice(c->device));
+    double* stage = nullptr;
+    HIP_TRY(hipMalloc(&stage, sizeof(double) * 4 * n_voxels));
+    hipError_t e = qsro::launch_nematic_aos(c->st, c->q[c->cur], c->cs, b.off, b.nvox, stage);
+    if (e == hipSuccess) e = hipMemcpyAsync(out4, stage, sizeof(double) * 4 * n_voxels, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    discard(hipFree(stage));
+    if (e != hipSuccess) return fail(QSR_E_HIP, std::string("nematic: ") + hipGetErrorString(e));
+    return QSR_OK;
+}
+
+int qsr_step(void* ctx, int nsteps) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || nsteps < 0) return fail(QSR_E_ARG, "bad argument");
+    if (int rc = check_uploaded(*c)) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    if (nsteps == 0) return QSR_OK;
+    HIP_TRY(hipEventRecord(c->ev[0], c->st));
+    int done = 0;
+    while (done < nsteps) {
+        const int n = std::min(kDynChunk, nsteps - done);
+        HIP_TRY(hipStreamSynchronize(c->st));  // pinned staging is reused per chunk
+        if (int rc = fill_dyn(*c, n)) return rc;
+        if (int rc = enqueue_steps(*c, n)) return rc;
+        advance_iters(*c, n);
+        done += n;
+    }
+    HIP_TRY(hipEventRecord(c->ev[1], c->st));
+    HIP_TRY(hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->last_ms = double(ms) / nsteps;
+    return QSR_OK;
+}
+
+int qsr_step_graph(void* ctx, int nsteps, int chunk) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || nsteps < 0 || chunk < 2 || (chunk & 1) || chunk > kDynChunk)
+        return fail(QSR_E_ARG, "chunk must be even, 2..512");
+    if (int rc = check_uploaded(*c)) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    if (nsteps == 0) return QSR_OK;
+    if (!c->gexec || c->graph_chunk != chunk || c->graph_cur != c->cur) {
+        if (c->gexec) discard(hipGraphExecDestroy(c->gexec));
+        if (c->graph) discard(hipGraphDestroy(c->graph));
+        c->gexec = nullptr;
+        c->graph = nullptr;
+        HIP_TRY(hipStreamSynchronize(c->st));
+        const int save = c->cur;
+        HIP_TRY(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_steps(*c, chunk);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->st, &g);
+        c->cur = save;
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(QSR_E_HIP, std::string("capture: ") + hipGetErrorString(e));
+        c->graph = g;
+        HIP_TRY(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+        c->graph_chunk = chunk;
+        c->graph_cur = c->cur;
+    }
+    HIP_TRY(hipEventRecord(c->ev[0], c->st));
+    int done = 0;
+    while (nsteps - done >= chunk) {
+        HIP_TRY(hipStreamSynchronize(c->st));
+        if (int rc = fill_dyn(*c, chunk)) return rc;
+        HIP_TRY(hipGraphLaunch(c->gexec, c->st));
+        advance_iters(*c, chunk);  // chunk is even: buffer roles are unchanged
+        done += chunk;
+    }
+    if (done < nsteps) {
+        HIP_TRY(hipStreamSynchronize(c->st));
+        const int n = nsteps - done;
+        if (int rc = fill_dyn(*c, n)) return rc;
+        if (int rc = enqueue_steps(*c, n)) return rc;
+        advance_iters(*c, n);
+        // an odd remainder flips the buffers; the graph is re-captured next call
+    }
+    HIP_TRY(hipEventRecord(c->ev[1], c->st));
+    HIP_TRY(hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->last_ms = double(ms) / nsteps;
+    return QSR_OK;
+}
+
+int qsr_step_imex(void* ctx, int nsteps, double L_stab, int jacobi_iters) {
+    (void)ctx; (void)nsteps; (void)L_stab; (void)jacobi_iters;
+    return fail(QSR_E_STATE, "IMEX stepping is not available in this build");
+}
+
+int qsr_last_step_ms(void* ctx, double* ms) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !ms) return fail(QSR_E_ARG, "bad argument");
+    *ms = c->last_ms;
+    return QSR_OK;
+}
+
+int qsr_step_work(void* ctx, uint64_t* active, uint64_t* tile, uint64_t* box) {
+    Ctx* c = as_ctx(ctx);
+    if (!c) return fail(QSR_E_ARG, "bad argument");
+    if (active) *active = uint64_t(c->active_voxels);
+    if (tile) *tile = uint64_t(c->tile_voxels);
+    if (box) *box = uint64_t(c->box_voxels);
+    return QSR_OK;
+}
+
+int qsr_set_iteration(void* ctx, int box, int64_t iteration) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || box < 0 || box >= int(c->boxes.size()) || iteration < 0) return fail(QSR_E_ARG, "bad argument");
+    Box& b = c->boxes[box];
+    b.iter = iteration;
+    b.last_T = b.T_at(iteration > 0 ? iteration - 1 : 0);
+    b.last_S = b.S_at(b.last_T);
+    return QSR_OK;
+}
+
+int qsr_box_info_get(void* ctx, int box, qsr_box_info* out) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !out || box < 0 || box >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad argument");
+    const Box& b = c->boxes[box];
+    out->voxels = uint64_t(b.nvox);
+    out->shell_voxels = b.masks.shell_count;
+    out->active_voxels = b.p.periodic ? uint64_t(b.nvox) : b.masks.active_count;
+    out->shell_volume = double(b.masks.shell_count) * b.p.dx * b.p.dy * b.p.dz;
+    out->surface_area = b.area;
+    out->nominal_thickness = 2.0 * b.g.shell_half;
+    out->effective_thickness = b.thickness;
+    out->S_eq = qsrh::s_eq(qsrh::bulk_abc(b.p.a, b.p.b, b.p.c, b.p.T_high, b.p.T_star, b.p.modechoice));
+    out->iteration = b.iter;
+    return QSR_OK;
+}
+
+int qsr_observables(void* ctx, int box, const qsr_obs_opts* opts, qsr_obs* out) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !out || box < 0 || box >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad argument");
+    if (int rc = check_uploaded(*c)) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    const Box& b = c->boxes[box];
+    const qsr_params& p = b.p;
+    qsr_obs_opts o{};
+    o.z_slice = -1; o.S_threshold = 0.1; o.charge_cutoff = 0.25; o.xi_z_slice = -1; o.xi_S_threshold = -1;
+    o.want_defects = 1; o.want_xi = 1;
+    if (opts) o = *opts;
+    const int zs = std::min(std::max(o.z_slice < 0 ? p.Nz / 2 : o.z_slice, 0), p.Nz - 1);
+    const int xz = std::min(std::max(o.xi_z_slice < 0 ? zs : o.xi_z_slice, 0), p.Nz - 1);
+    const double xthr = o.xi_S_threshold < 0 ? o.S_threshold : o.xi_S_threshold;
+
+    qsro::ObsParams P{};
+    P.Nx = p.Nx; P.Ny = p.Ny; P.Nz = p.Nz; P.periodic = p.periodic;
+    P.off = b.off; P.rows_off = b.rows_off;
+    P.dx = p.dx; P.dy = p.dy; P.dz = p.dz;
+    P.cx = b.g.cx; P.cy = b.g.cy; P.cz = b.g.cz; P.rx = b.g.rx; P.ry = b.g.ry; P.rz = b.g.rz;
+    P.min_spacing = b.g.min_spacing;
+    const double T = b.last_T, dT = T - p.T_star;
+    if (p.modechoice == 1) { P.c2 = (3.0 * p.a * dT) / 2.0; P.c3 = (3.0 * p.b) / 2.0; P.c4 = (9.0 * p.c) / 8.0; }
+    else { P.c2 = (3.0 * p.a * dT) / 2.0; P.c3 = (9.0 * p.b) / 2.0; P.c4 = (9.0 * p.c) / 4.0; }
+    P.no_L = (p.L1 == 0.0 && p.L2 == 0.0 && p.L3 == 0.0);
+    P.kappa = p.periodic && !P.no_L ? p.L1 : p.kappa;
+    P.L1 = p.L1; P.L2 = p.L2; P.L3 = p.L3;
+    P.W_over_delta = (p.W != 0.0 && b.thickness > 0.0) ? p.W / b.thickness : 0.0;
+    P.S_shell = b.last_S;
+    const float* q = c->q[c->cur];
+
+    qsr_obs r{};
+    r.iteration = b.iter - 1;
+    r.T = T;
+    r.S_shell = b.last_S;
+    r.checker_rel = kNaN;
+    r.defect_density = kNaN;
+    r.xi_grad_proxy = kNaN;
+    r.defect_line_density = kNaN;
+    double vol[qsro::NPART];
+    long long nblk = (b.nvox + 255) / 256;
+    if (int rc = ensure_partials(*c, nblk * 10)) return rc;
+    if (!p.periodic) {
+        HIP_TRY(qsro::launch_obs_droplet(c->st, q, c->cs, P, c->d_step_rows, c->d_obs_rows, c->d_part, &nblk));
+        HIP_TRY(qsro::launch_obs_finalize(c->st, c->d_part, nblk, qsro::NPART, 1u << 7, c->d_out));
+        HIP_TRY(hipMemcpyAsync(vol, c->d_out, sizeof(vol), hipMemcpyDeviceToHost, c->st));
+        HIP_TRY(hipStreamSynchronize(c->st));
+        r.bulk = vol[0]; r.elastic = vol[1]; r.anchoring = vol[2];
+        r.total = r.bulk + r.elastic + r.anchoring;
+        r.avg_S = vol[4] > 0 ? vol[3] / vol[4] : 0.0;
+        r.max_S = vol[7];
+        r.radiality = vol[6] > 0 ? vol[5] / vol[6] : 0.0;
+        // slice pass: defects, xi and (if max_S >= 0.5) the checkerboard metric
+        const long long ns = (long long)p.Nx * p.Ny;
+        if (ns * 4 > c->nf_slice_cap) {
+            discard(hipFree(c->d_nf_slice[0]));
+            discard(hipFree(c->d_nf_slice[1]));
+            c->d_nf_slice[0] = c->d_nf_slice[1] = nullptr;
+            HIP_TRY(hipMalloc(&c->d_nf_slice[0], sizeof(double) * ns * 4));
+            HIP_TRY(hipMalloc(&c->d_nf_slice[1], sizeof(double) * ns * 4));
+            c->nf_slice_cap = ns * 4;
+        }
+        const bool want_checker = r.max_S >= 0.5;
+        HIP_TRY(qsro::launch_obs_slice_nematic(c->st, q, c->cs, P, zs, c->d_nf_slice[0]));
+        const double* nfx = c->d_nf_slice[0];
+        if (xz != zs) {
+            HIP_TRY(qsro::launch_obs_slice_nematic(c->st, q, c->cs, P, xz, c->d_nf_slice[1]));
+            nfx = c->d_nf_slice[1];
+        }
+        long long nb2 = 0;
+        HIP_TRY(qsro::launch_obs_slice_stats(c->st, c->d_nf_slice[0], nfx, P, c->d_obs_rows, zs, o.S_threshold,
+                                             o.charge_cutoff, xthr, want_checker ? 1 : 0, c->d_part, &nb2));
+        HIP_TRY(qsro::launch_obs_finalize(c->st, c->d_part, nb2, qsro::NSLICE, 0u, c->d_out));
+        double sl[qsro::NSLICE];
+        HIP_TRY(hipMemcpyAsync(sl, c->d_out, sizeof(sl), hipMemcpyDeviceToHost, c->st));
+        HIP_TRY(hipStreamSynchronize(c->st));
+        if (o.want_defects) {
+            r.defects = uint64_t(sl[0]);
+            r.plaquettes = uint64_t(sl[1]);
+            r.defect_density = r.plaquettes ? double(r.defects) / double(r.plaquettes) : kNaN;
+        }
+        if (o.want_xi) {
+            r.xi_edges = uint64_t(sl[3]);
+            if (r.xi_edges > 0) {
+                const double mean_sq = sl[2] / double(r.xi_edges);
+                const double eps = 1e-30;
+                const double inv = (mean_sq > 0.0) ? 1.0 / std::sqrt(mean_sq + eps) : 1.0 / std::sqrt(eps);
+                r.xi_grad_proxy = std::min(inv, 0.5 * double(std::min(p.Nx, p.Ny)));
+            }
+        }
+        if (want_checker && sl[5] > 10 && sl[7] > 10 && sl[9] > 20 && sl[8] != 0.0)
+            r.checker_rel = std::abs(sl[4] - sl[6]) / std::abs(sl[8]);
+    } else {
+        if (b.nvox * 4 > c->nf3_cap) {
+            discard(hipFree(c->d_nf3));
+            c->d_nf3 = nullptr;
+            HIP_TRY(hipMalloc(&c->d_nf3, sizeof(double) * b.nvox * 4));
+            c->nf3_cap = b.nvox * 4;
+        }
+        const qsrh::BulkABC k = qsrh::bulk_abc(p.a, p.b, p.c, T, p.T_star, p.modechoice);
+        P.c2 = k.A / 2.0; P.c3 = k.B / 3.0; P.c4 = k.C / 4.0;
+        HIP_TRY(qsro::launch_obs_periodic(c->st, q, c->cs, P, c->d_nf3, c->d_part, &nblk));
+        HIP_TRY(qsro::launch_obs_finalize(c->st, c->d_part, nblk, qsro::NPART, 1u << 7, c->d_out));
+        HIP_TRY(hipMemcpyAsync(vol, c->d_out, sizeof(vol), hipMemcpyDeviceToHost, c->st));
+        long long nb3 = 0;
+        HIP_TRY(qsro::launch_obs_plaquette3(c->st, c->d_nf3, P, o.S_threshold, o.charge_cutoff, c->d_part, &nb3));
+        HIP_TRY(qsro::launch_obs_finalize(c->st, c->d_part, nb3, qsro::NPLAQ, 0u, c->d_out + 16));
+        double pl[qsro::NPLAQ];
+        HIP_TRY(hipMemcpyAsync(pl, c->d_out + 16, sizeof(pl), hipMemcpyDeviceToHost, c->st));
+        HIP_TRY(hipStreamSynchronize(c->st));
+        const double V = p.dx * p.dy * p.dz * double(b.nvox);
+        r.bulk = vol[0] / V;
+        r.elastic = vol[1] / V;
+        r.anchoring = 0.0;
+        r.total = r.bulk + r.elastic;
+        r.radiality = kNaN;
+        r.avg_S = vol[4] / double(b.nvox);
+        r.max_S = vol[7];
+        r.defects = uint64_t(pl[0]);
+        r.plaquettes = uint64_t(pl[1]);
+        r.defect_density = r.plaquettes ? pl[0] / pl[1] : kNaN;
+        r.defect_line_density = V > 0 ? pl[2] / V : kNaN;
+        if (vol[3] > 0.0 && vol[2] > 0.0) r.xi_grad_proxy = std::sqrt(vol[2] / vol[3]);
+    }
+    *out = r;
+    return QSR_OK;
+}
+
+double qsr_S_eq(double a, double b, double c, double T, double T_star, int modechoice) {
+    return qsrh::s_eq(qsrh::bulk_abc(a, b, c, T, T_star, modechoice));
+}
+
+int qsr_dt_estimate(double dx, double dy, double dz, double gamma, double a, double b, double c, double T_star,
+                    int modechoice, double kappa, double L1, double L2, double L3, double S_ref_used, double S0,
+                    double T_a, double T_b, int use_semi_implicit, double L_stab, int jacobi_iters, double out[10]) {
+    if (!out) return fail(QSR_E_ARG, "null out");
+    qsrh::DtInput in;
+    in.dx = dx; in.dy = dy; in.dz = dz; in.gamma = gamma; in.a = a; in.b = b; in.c = c; in.T_star = T_star;
+    in.modechoice = modechoice; in.kappa = kappa; in.L1 = L1; in.L2 = L2; in.L3 = L3;
+    in.S_ref_used = S_ref_used; in.S0 = S0; in.T_a = T_a; in.T_b = T_b;
+    in.semi_implicit = use_semi_implicit != 0; in.L_stab = L_stab; in.jacobi_iters = jacobi_iters;
+    const qsrh::DtWindow w = qsrh::stable_dt(in);
+    const double v[10] = {w.min_dx, w.S_scale, w.L_eff_total, w.D_total, w.dt_diff_total,
+                          w.L_eff_explicit, w.dt_diff, w.bulk_rate, w.dt_bulk, w.dt_max};
+    std::memcpy(out, v, sizeof(v));
+    return QSR_OK;
+}
+
+int qsr_init_field(const qsr_params* p, int mode, double S0, double noise_amplitude, int64_t random_seed,
+                   double* aos5_out, size_t n_voxels) {
+    if (!p || !aos5_out) return fail(QSR_E_ARG, "null argument");
+    if ((long long)n_voxels != (long long)p->Nx * p->Ny * p->Nz) return fail(QSR_E_ARG, "n_voxels mismatch");
+    const qsrh::Geometry g = qsrh::make_geometry(p->Nx, p->Ny, p->Nz, p->dx, p->dy, p->dz, p->radius_index,
+                                                 p->center_index);
+    std::vector<double> Q;
+    if (mode == 2) qsrh::init_isotropic(Q, g, noise_amplitude, random_seed);
+    else qsrh::init_director(Q, g, S0, mode == 0 ? 0 : 1, random_seed);
+    std::memcpy(aos5_out, Q.data(), sizeof(double) * Q.size());
+    return QSR_OK;
+}
+
+}  // extern "C"

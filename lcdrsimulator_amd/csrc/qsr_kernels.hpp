@@ -1,0 +1,112 @@
+// qsr_kernels.hpp -- device-side data shared by the step and observable
+// kernels and the context code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace qsrk {
+
+// Box flags
+enum : int {
+    F_PERIODIC = 1,   // wrap on every face, no droplet (every voxel evolves)
+    F_ANCHOR = 2,     // Rapini-Papoular penalty on the shell (W != 0, QSR.cu:623-652)
+    F_BC = 4,         // strong anchoring re-imposed on the shell (W <= 0, QSR.cu:851-867)
+    F_SKIPSHELL = 8,  // shell voxels are not updated (W == 0, QSR.cu:705-706)
+    F_NOISE = 16,     // additive Philox noise on evolving voxels
+    F_CAP = 32,       // Frobenius-norm cap (QSR.cu:721-731)
+    F_L2 = 64,
+    F_L3 = 128,
+};
+
+// Static description of one box, FP32 coefficients pre-multiplied by the
+// mobility-step m = dt/gamma so the kernel computes dQ directly.
+struct BoxDev {
+    int Nx, Ny, Nz, flags;
+    long long off;       // voxel offset of the box inside each packed SoA plane
+    long long rows_off;  // row offset into the packed row-mask tables
+    float cB, cC;        // m*B, m*C (A varies per step, see StepDyn)
+    float lx, ly, lz;    // m*Lk/d^2, Lk = kappa when L1=L2=L3=0, else L1
+    float cW;            // m*W/delta_eff
+    float cap2;          // q_norm_cap^2
+    float sigma;         // noise standard deviation per component per step
+    float ci, cj, ck;    // droplet centre, index units
+    float sx, sy, sz;    // surface-normal scales d/r^2 per axis (index units)
+    // anisotropic elasticity (SI FP32, m-scaled products)
+    float hx, hy, hz;    // 1/(2d)  central first difference
+    float fx, fy, fz;    // 1/d     one-sided first difference (box faces)
+    float ix2, iy2, iz2; // 1/d^2
+    float i4xy, i4xz, i4yz; // 1/(4 d_a d_b)
+    float mL2, mL3;      // m*L2, m*L3
+    unsigned long long seed;
+};
+
+// Per step, per box: m*A(T), S_shell(T), iteration index for the noise counter.
+struct StepDyn {
+    float cA, S_shell;
+    unsigned int iter, pad;
+};
+
+// Row-mask entries: 8 x int16 per row (i, j), see qsrh::RowMasks.
+struct alignas(16) RowMask {
+    short v[8];
+};
+
+constexpr int TK = 64;  // k extent of a tile: one wavefront
+constexpr int TJ = 4;   // j rows per tile: one wavefront each
+
+}  // namespace qsrk
+
+namespace qsro {
+
+using qsrk::RowMask;
+
+// FP64 parameters of one box for the log pass.
+struct ObsParams {
+    int Nx, Ny, Nz, periodic;
+    long long off, rows_off;
+    double dx, dy, dz;
+    double cx, cy, cz, rx, ry, rz, min_spacing;
+    double c2, c3, c4;         // bulk energy coefficients (QSR.cu:887-899) or A/2, B/3, C/4 (periodic)
+    double kappa, L1, L2, L3;  // elastic
+    int no_L;                  // L1 = L2 = L3 = 0 -> kappa
+    double W_over_delta;       // 0 disables the anchoring energy
+    double S_shell;
+};
+
+constexpr int NPART = 8;     // volume partial slots
+constexpr int NSLICE = 10;   // slice partial slots
+constexpr int NPLAQ = 3;     // periodic plaquette slots
+
+}  // namespace qsro
+
+
+// Host launchers (defined in qsr_step.hip / qsr_obs.hip)
+namespace qsrk {
+hipError_t launch_step(bool full, bool noise, int ntiles, hipStream_t st, const float* qin, float* qout,
+                       long long cs, const BoxDev* boxes, const int4* tiles, const StepDyn* dyn, int nboxes,
+                       const RowMask* rows, int ch);
+hipError_t launch_faces(long long nface, hipStream_t st, const float* qin, float* qout, long long cs,
+                        const BoxDev* boxes, const StepDyn* dyn, const long long* face_prefix, int nboxes,
+                        const RowMask* rows);
+hipError_t launch_aos_to_soa(hipStream_t st, const double* aos, float* q0, float* q1, long long cs, long long off,
+                             long long n);
+hipError_t launch_soa_to_aos(hipStream_t st, const float* q, double* aos, long long cs, long long off, long long n);
+}  // namespace qsrk
+
+namespace qsro {
+hipError_t launch_obs_droplet(hipStream_t st, const float* q, long long cs, const ObsParams& P,
+                              const RowMask* step_rows, const RowMask* obs_rows, double* partials, long long* nblk);
+hipError_t launch_obs_periodic(hipStream_t st, const float* q, long long cs, const ObsParams& P, double* nf,
+                               double* partials, long long* nblk);
+hipError_t launch_obs_plaquette3(hipStream_t st, const double* nf, const ObsParams& P, double thr, double cutoff,
+                                 double* partials, long long* nblk);
+hipError_t launch_obs_slice_nematic(hipStream_t st, const float* q, long long cs, const ObsParams& P, int z, double* nf);
+hipError_t launch_obs_slice_stats(hipStream_t st, const double* nf_def, const double* nf_xi, const ObsParams& P,
+                                  const RowMask* obs_rows, int z_def, double S_thr, double cutoff, double xi_thr,
+                                  int want_checker, double* partials, long long* nblk);
+hipError_t launch_obs_finalize(hipStream_t st, const double* partials, long long nblk, int nslot, unsigned max_mask,
+                               double* out);
+hipError_t launch_nematic_aos(hipStream_t st, const float* q, long long cs, long long off, long long n, double* out);
+}  // namespace qsro

@@ -1,0 +1,284 @@
+"""ctypes binding of libqsr_amd.so (include/qsr_amd.h).
+
+The Python side of the engine: builds ``qsr_params`` records with the
+reference's defaults (QSR.cu:1944-1949, 1958-2139), drives a context
+(upload / step / observables / download) and exposes the host physics
+(S_eq, the stability window, the seeded initial states).
+
+Arrays use the reference interchange format: float64 ``(Nx, Ny, Nz, 5)`` with
+components (Qxx, Qxy, Qxz, Qyy, Qyz), flat index ``k + Nz*(j + Ny*i)``.
+There is no CPU fallback: every call goes to the HIP library and raises if
+the library or a GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass, field, fields
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "libqsr_amd.so")
+_lib = None
+
+PROTOCOL_CONSTANT, PROTOCOL_STEP, PROTOCOL_RAMP = 0, 1, 2
+
+
+class QSRError(RuntimeError):
+    pass
+
+
+class _Params(C.Structure):
+    _fields_ = [
+        ("Nx", C.c_int32), ("Ny", C.c_int32), ("Nz", C.c_int32),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("a", C.c_double), ("b", C.c_double), ("c", C.c_double), ("T_star", C.c_double),
+        ("modechoice", C.c_int32),
+        ("kappa", C.c_double), ("L1", C.c_double), ("L2", C.c_double), ("L3", C.c_double),
+        ("W", C.c_double), ("gamma", C.c_double), ("dt", C.c_double), ("q_norm_cap", C.c_double),
+        ("periodic", C.c_int32), ("radius_index", C.c_double), ("center_index", C.c_double * 3),
+        ("protocol", C.c_int32), ("T_high", C.c_double), ("T_low", C.c_double),
+        ("pre_equil_iters", C.c_int32), ("ramp_iters", C.c_int32),
+        ("boundary_custom", C.c_int32), ("boundary_S", C.c_double),
+        ("noise_sigma", C.c_double), ("seed", C.c_uint64),
+    ]
+
+
+class _Obs(C.Structure):
+    _fields_ = [
+        ("iteration", C.c_int64), ("T", C.c_double), ("S_shell", C.c_double),
+        ("bulk", C.c_double), ("elastic", C.c_double), ("anchoring", C.c_double), ("total", C.c_double),
+        ("radiality", C.c_double), ("avg_S", C.c_double), ("max_S", C.c_double), ("checker_rel", C.c_double),
+        ("defects", C.c_uint64), ("plaquettes", C.c_uint64), ("defect_density", C.c_double),
+        ("xi_grad_proxy", C.c_double), ("xi_edges", C.c_uint64), ("defect_line_density", C.c_double),
+    ]
+
+
+class _ObsOpts(C.Structure):
+    _fields_ = [("z_slice", C.c_int32), ("S_threshold", C.c_double), ("charge_cutoff", C.c_double),
+                ("xi_z_slice", C.c_int32), ("xi_S_threshold", C.c_double),
+                ("want_defects", C.c_int32), ("want_xi", C.c_int32)]
+
+
+class _BoxInfo(C.Structure):
+    _fields_ = [("voxels", C.c_uint64), ("shell_voxels", C.c_uint64), ("active_voxels", C.c_uint64),
+                ("shell_volume", C.c_double), ("surface_area", C.c_double), ("nominal_thickness", C.c_double),
+                ("effective_thickness", C.c_double), ("S_eq", C.c_double), ("iteration", C.c_int64)]
+
+
+EXPORTS = (
+    "qsr_create", "qsr_create_batch", "qsr_destroy", "qsr_last_error", "qsr_device_count",
+    "qsr_upload_q", "qsr_download_q", "qsr_download_nematic", "qsr_step", "qsr_step_graph",
+    "qsr_step_imex", "qsr_observables", "qsr_box_info_get", "qsr_set_iteration", "qsr_last_step_ms",
+    "qsr_step_work", "qsr_S_eq", "qsr_dt_estimate", "qsr_init_field",
+)
+
+
+def load():
+    """Load libqsr_amd.so; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise QSRError(f"{LIB_PATH} is missing: build it with `python -m lcdrsimulator_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, dp = C.c_void_p, C.POINTER(C.c_double)
+    L.qsr_last_error.restype = C.c_char_p
+    L.qsr_create.argtypes = [C.POINTER(_Params), C.c_int, C.POINTER(vp)]
+    L.qsr_create_batch.argtypes = [C.POINTER(_Params), C.c_int, C.c_int, C.POINTER(vp)]
+    L.qsr_destroy.argtypes = [vp]
+    L.qsr_upload_q.argtypes = [vp, C.c_int, dp, C.c_size_t]
+    L.qsr_download_q.argtypes = [vp, C.c_int, dp, C.c_size_t]
+    L.qsr_download_nematic.argtypes = [vp, C.c_int, dp, C.c_size_t]
+    L.qsr_step.argtypes = [vp, C.c_int]
+    L.qsr_step_graph.argtypes = [vp, C.c_int, C.c_int]
+    L.qsr_step_imex.argtypes = [vp, C.c_int, C.c_double, C.c_int]
+    L.qsr_observables.argtypes = [vp, C.c_int, C.POINTER(_ObsOpts), C.POINTER(_Obs)]
+    L.qsr_box_info_get.argtypes = [vp, C.c_int, C.POINTER(_BoxInfo)]
+    L.qsr_set_iteration.argtypes = [vp, C.c_int, C.c_int64]
+    L.qsr_last_step_ms.argtypes = [vp, dp]
+    L.qsr_step_work.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.qsr_S_eq.restype = C.c_double
+    L.qsr_S_eq.argtypes = [C.c_double] * 5 + [C.c_int]
+    L.qsr_dt_estimate.argtypes = [C.c_double] * 8 + [C.c_int] + [C.c_double] * 8 + [C.c_int, C.c_double, C.c_int, dp]
+    L.qsr_init_field.argtypes = [C.POINTER(_Params), C.c_int, C.c_double, C.c_double, C.c_int64, dp, C.c_size_t]
+    L.qsr_device_count.argtypes = [C.POINTER(C.c_int)]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise QSRError(f"qsr error {rc}: {load().qsr_last_error().decode()}")
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _check(load().qsr_device_count(C.byref(n)))
+    return n.value
+
+
+@dataclass
+class Params:
+    """One box.  Defaults are the reference's (QSR.cu:1944-1949, 2010-2116)."""
+    Nx: int = 100
+    Ny: int = 100
+    Nz: int = 100
+    dx: float = 1e-8
+    dy: float = 1e-8
+    dz: float = 1e-8
+    a: float = 0.044e6
+    b: float = 1.413e6
+    c: float = 1.153e6
+    T_star: float = 308.0
+    modechoice: int = 1
+    kappa: float = 6.5e-12
+    L1: float = 0.0
+    L2: float = 0.0
+    L3: float = 0.0
+    W: float = 0.0
+    gamma: float = 0.1
+    dt: float = 1e-10
+    q_norm_cap: float = 0.0
+    periodic: int = 0
+    radius_index: float = -1.0
+    center_index: tuple = (math.nan, math.nan, math.nan)
+    protocol: int = PROTOCOL_CONSTANT
+    T_high: float = 300.0
+    T_low: float = 295.0
+    pre_equil_iters: int = 0
+    ramp_iters: int = 1000
+    boundary_custom: int = 0
+    boundary_S: float = 0.0
+    noise_sigma: float = 0.0
+    seed: int = 0
+
+    def to_c(self) -> _Params:
+        p = _Params()
+        for f in fields(self):
+            v = getattr(self, f.name)
+            if f.name == "center_index":
+                p.center_index = (C.c_double * 3)(*v)
+            else:
+                setattr(p, f.name, v)
+        return p
+
+    @property
+    def dims(self):
+        return (self.Nx, self.Ny, self.Nz)
+
+    @property
+    def n(self):
+        return self.Nx * self.Ny * self.Nz
+
+
+OBS_KEYS = [n for n, _ in _Obs._fields_]
+
+
+class Solver:
+    """A context of one box (or a batch of boxes) on one GPU."""
+
+    def __init__(self, params, device: int = 0):
+        L = load()
+        self.params = list(params) if isinstance(params, (list, tuple)) else [params]
+        arr = (_Params * len(self.params))(*[p.to_c() for p in self.params])
+        h = C.c_void_p()
+        if len(self.params) == 1:
+            _check(L.qsr_create(arr, device, C.byref(h)))
+        else:
+            _check(L.qsr_create_batch(arr, len(self.params), device, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().qsr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, Q, box: int = 0):
+        p = self.params[box]
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        if Q.shape != p.dims + (5,):
+            raise ValueError(f"Q shape {Q.shape} != {p.dims + (5,)}")
+        _check(load().qsr_upload_q(self._h, box, Q.ctypes.data_as(C.POINTER(C.c_double)), p.n))
+
+    def download(self, box: int = 0) -> np.ndarray:
+        p = self.params[box]
+        Q = np.empty(p.dims + (5,))
+        _check(load().qsr_download_q(self._h, box, Q.ctypes.data_as(C.POINTER(C.c_double)), p.n))
+        return Q
+
+    def nematic(self, box: int = 0) -> np.ndarray:
+        p = self.params[box]
+        out = np.empty(p.dims + (4,))
+        _check(load().qsr_download_nematic(self._h, box, out.ctypes.data_as(C.POINTER(C.c_double)), p.n))
+        return out
+
+    def step(self, n: int = 1):
+        _check(load().qsr_step(self._h, int(n)))
+
+    def step_graph(self, n: int, chunk: int = 64):
+        _check(load().qsr_step_graph(self._h, int(n), int(chunk)))
+
+    def set_iteration(self, it: int, box: int = 0):
+        _check(load().qsr_set_iteration(self._h, box, int(it)))
+
+    def observables(self, box: int = 0, z_slice=-1, S_threshold=0.1, charge_cutoff=0.25, xi_z_slice=-1,
+                    xi_S_threshold=-1.0, want_defects=True, want_xi=True) -> dict:
+        o = _ObsOpts(z_slice, S_threshold, charge_cutoff, xi_z_slice, xi_S_threshold, int(want_defects),
+                     int(want_xi))
+        r = _Obs()
+        _check(load().qsr_observables(self._h, box, C.byref(o), C.byref(r)))
+        return {k: getattr(r, k) for k in OBS_KEYS}
+
+    def info(self, box: int = 0) -> dict:
+        r = _BoxInfo()
+        _check(load().qsr_box_info_get(self._h, box, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in _BoxInfo._fields_}
+
+    def last_step_ms(self) -> float:
+        v = C.c_double()
+        _check(load().qsr_last_step_ms(self._h, C.byref(v)))
+        return v.value
+
+    def work(self) -> dict:
+        a, t, b = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(load().qsr_step_work(self._h, C.byref(a), C.byref(t), C.byref(b)))
+        return {"active_voxels": a.value, "tile_voxels": t.value, "box_voxels": b.value}
+
+
+def S_eq(a, b, c, T, T_star, modechoice=1) -> float:
+    return load().qsr_S_eq(a, b, c, T, T_star, modechoice)
+
+
+DT_KEYS = ("min_dx", "S_scale", "L_eff_total", "D_total", "dt_diff_total", "L_eff_explicit", "dt_diff",
+           "bulk_rate", "dt_bulk", "dt_max")
+
+
+def dt_estimate(dx, dy, dz, gamma, a, b, c, T_star, modechoice, kappa, L1, L2, L3, S_ref_used, S0, T_a, T_b,
+                use_semi_implicit=False, L_stab=0.0, jacobi_iters=0) -> dict:
+    out = (C.c_double * 10)()
+    _check(load().qsr_dt_estimate(dx, dy, dz, gamma, a, b, c, T_star, modechoice, kappa, L1, L2, L3, S_ref_used,
+                                  S0, T_a, T_b, int(use_semi_implicit), L_stab, jacobi_iters, out))
+    return dict(zip(DT_KEYS, out))
+
+
+def init_field(p: Params, mode: int, S0: float = 0.5, noise_amplitude: float = 1e-3, seed: int = 12345) -> np.ndarray:
+    """Seeded initial state (QSR.cu:1841-1904): 0 random director, 1 radial, 2 isotropic noise."""
+    Q = np.empty(p.dims + (5,))
+    _check(load().qsr_init_field(C.byref(p.to_c()), mode, S0, noise_amplitude, seed,
+                                 Q.ctypes.data_as(C.POINTER(C.c_double)), p.n))
+    return Q

@@ -1058,7 +1058,9 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 /* Five N(0,1) variates for (seed, voxel, step): two Philox blocks with counter
  * {voxel_lo, voxel_hi, step, block}, key {seed_lo, seed_hi}; uniforms
- * u = (x + 0.5) * 2^-32 in (0,1); Box-Muller pairs (u0,u1), (u2,u3), (u4,u5). */
+ * u = ((x >> 9) + 0.5) * 2^-23 in (0,1) (exactly representable in binary32, so
+ * an FP32 device implementation sees the same uniforms); Box-Muller pairs
+ * (u0,u1), (u2,u3), (u4,u5). */
 void orc_noise5(uint64_t seed, uint64_t voxel, uint32_t step, double g[5]) {
     const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     uint32_t r[8];
@@ -1067,7 +1069,7 @@ void orc_noise5(uint64_t seed, uint64_t voxel, uint32_t step, double g[5]) {
         orc_philox4x32_10(ctr, key, r + 4 * blk);
     }
     double u[6];
-    for (int t = 0; t < 6; ++t) u[t] = ((double)r[t] + 0.5) * (1.0 / 4294967296.0);
+    for (int t = 0; t < 6; ++t) u[t] = ((double)(r[t] >> 9) + 0.5) * (1.0 / 8388608.0);
     for (int pr = 0; pr < 3; ++pr) {
         const double rad = sqrt(-2.0 * log(u[2 * pr]));
         const double ang = 2.0 * ORC_PI * u[2 * pr + 1];

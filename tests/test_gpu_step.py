@@ -1,0 +1,131 @@
+"""GPU parity of the fused FP32 step against the FP64 oracle (QSR.cu step order).
+
+Each case uploads the same float32-representable field to the HIP path and the
+oracle, advances both and compares.  Tolerances (FP32 stencil vs FP64 oracle):
+one step: max |dQ| <= 2e-6 absolute; many steps: see test_gpu_relax.py.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STEP_TOL = 2e-6
+
+
+def _f32(Q):
+    return Q.astype(np.float32).astype(np.float64)
+
+
+def _case(q, orc, dims, L=(0.0, 0.0, 0.0), kappa=6.5e-12, W=0.0, R=-1.0, ctr=(math.nan,) * 3,
+          d=(1e-8, 1e-8, 1e-8), T=300.0, Ts=307.0, init=0, custom=None, cap=0.0, seed=12345):
+    L1, L2, L3 = L
+    p = q.Params(Nx=dims[0], Ny=dims[1], Nz=dims[2], dx=d[0], dy=d[1], dz=d[2], T_star=Ts, T_high=T,
+                 kappa=kappa, L1=L1, L2=L2, L3=L3, W=W, radius_index=R, center_index=ctr, q_norm_cap=cap,
+                 boundary_custom=0 if custom is None else 1, boundary_S=0.0 if custom is None else custom)
+    g = orc.Grid(*dims, *d)
+    ph = orc.phys(T=T, T_star=Ts, kappa=kappa, L1=L1, L2=L2, L3=L3, W=W)
+    dto = orc.dt_estimate(*d, 0.1, ph.a, ph.b, ph.c, Ts, 1, kappa, L1, L2, L3, math.nan, 0.5, T, T)
+    p.dt = dto["dt_max"]
+    od = orc.DropletOracle(g, ph, gamma=0.1, radius_index=R, center=ctr, q_norm_cap=cap)
+    Q0 = od.init_director(0.5, init, seed) if init in (0, 1) else od.init_isotropic(1e-2, seed)
+    Q0 = _f32(Q0)
+    S_shell = orc.shell_order(custom is not None, custom or 0.0, ph.a, ph.b, ph.c, T, Ts)
+    return p, od, Q0, S_shell
+
+
+CASES = {
+    "l1_strong_64": dict(dims=(24, 24, 24), L=(6e-12, 0, 0)),
+    "kappa_strong_odd": dict(dims=(19, 23, 29), kappa=6.5e-12),
+    "l1_weak_custom": dict(dims=(24, 20, 30), L=(6e-12, 0, 0), W=1e-4, custom=0.5),
+    "l2_strong": dict(dims=(20, 21, 22), L=(6e-12, 3e-12, 0)),
+    "l2_l3_weak": dict(dims=(22, 26, 70), L=(6e-12, 3e-12, 1e-12), W=1e-4, custom=0.5),
+    "l3_only_weak": dict(dims=(18, 18, 18), L=(0, 0, 2e-12), W=1e-3),
+    "aniso_spacing_l2": dict(dims=(17, 25, 33), L=(6e-12, 3e-12, 1e-12), d=(1e-8, 1.2e-8, 0.9e-8), W=2e-4),
+    "offcentre_radius": dict(dims=(30, 30, 30), L=(6e-12, 0, 0), R=9.0, ctr=(12.0, 15.5, 16.0), W=1e-4),
+    "touching_faces_weak_l2": dict(dims=(16, 16, 16), L=(6e-12, 3e-12, 0), W=1e-3),
+    "cap": dict(dims=(20, 20, 20), L=(6e-12, 0, 0), cap=0.5 * math.sqrt(2 / 3)),
+    "isotropic_init_quench_T": dict(dims=(24, 24, 24), L=(6e-12, 0, 0), init=2, T=295.0),
+    "radial_init": dict(dims=(21, 21, 21), L=(6e-12, 2e-12, 0), init=1),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_single_step_matches_oracle(qsr, orc, name):
+    p, od, Q0, S_shell = _case(qsr, orc, **CASES[name])
+    print(f"[case] {name} dims={p.dims}", flush=True)
+    with qsr.Solver(p) as s:
+        s.upload(Q0)
+        back = s.download()
+        assert np.array_equal(back, Q0)
+        for nsteps in (1, 2):
+            s.step(1)
+            Qg = s.download()
+            Qo = od.step(Q0 if nsteps == 1 else Qo, S_shell, p.dt)
+            err = np.abs(Qg - Qo).max()
+            moved = np.abs(Qo - Q0).max()
+            print(f"  step {nsteps}: max|Qgpu-Qorc| = {err:.3e}  (max|dQ| = {moved:.3e})")
+            assert moved > 0
+            assert err <= STEP_TOL
+
+
+def test_static_voxels_untouched(qsr, orc):
+    p, od, Q0, S_shell = _case(qsr, orc, (24, 24, 24), L=(6e-12, 0, 0))
+    with qsr.Solver(p) as s:
+        s.upload(Q0)
+        s.step(3)
+        Qg = s.download()
+    Qo = od.step(Q0, S_shell, p.dt, steps=3)
+    outside = np.abs(Q0).sum(-1) == 0
+    assert np.array_equal(Qg[outside & (np.abs(Qo).sum(-1) == 0)], Qo[outside & (np.abs(Qo).sum(-1) == 0)])
+    assert np.abs(Qg - Qo).max() < 4 * STEP_TOL
+
+
+@pytest.mark.parametrize("noise", [0.0, 1.049e-3])
+@pytest.mark.parametrize("dims", [(16, 16, 16), (20, 13, 70)])
+def test_periodic_step_matches_oracle(qsr, orc, noise, dims):
+    T, Ts, kappa, gamma = 301.0, 307.0, 6e-12, 0.1
+    p = qsr.Params(Nx=dims[0], Ny=dims[1], Nz=dims[2], periodic=1, T_star=Ts, T_high=T, kappa=kappa,
+                   gamma=gamma, dt=1.5e-10, noise_sigma=noise, seed=4242)
+    g = orc.Grid(*dims)
+    rng = np.random.default_rng(7)
+    Q0 = _f32(rng.normal(0, 0.05, dims + (5,)))
+    A, B, Cc = orc.bulk_ABC(p.a, p.b, p.c, T, Ts, 1)
+    print(f"[case] periodic dims={dims} noise={noise}", flush=True)
+    with qsr.Solver(p) as s:
+        s.upload(Q0)
+        Qo = Q0
+        for it in range(3):
+            s.step(1)
+            Qo = orc.periodic_step(Qo, g, A, B, Cc, kappa, gamma, p.dt, noise, 4242, it)
+            err = np.abs(s.download() - Qo).max()
+            print(f"  step {it}: err {err:.3e}")
+            assert err <= STEP_TOL
+
+
+def test_batched_boxes_match_individual(qsr, orc):
+    cases = [CASES["l1_weak_custom"], CASES["l2_l3_weak"], CASES["kappa_strong_odd"]]
+    built = [_case(qsr, orc, **c, seed=100 + t) for t, c in enumerate(cases)]
+    params = [b[0] for b in built]
+    with qsr.Solver(params) as s:
+        for t, b in enumerate(built):
+            s.upload(b[2], box=t)
+        s.step(2)
+        for t, (p, od, Q0, S_shell) in enumerate(built):
+            Qo = od.step(Q0, S_shell, p.dt, steps=2)
+            err = np.abs(s.download(box=t) - Qo).max()
+            print(f"  box {t}: err {err:.3e}")
+            assert err <= 2 * STEP_TOL
+
+
+def test_graph_steps_equal_plain_steps(qsr, orc):
+    p, od, Q0, S_shell = _case(qsr, orc, **CASES["l2_l3_weak"])
+    with qsr.Solver(p) as a, qsr.Solver(p) as b:
+        a.upload(Q0)
+        b.upload(Q0)
+        a.step(37)
+        for _ in range(3):
+            b.step_graph(12, chunk=4)
+        b.step(1)
+        assert np.array_equal(a.download(), b.download())
