@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""Throughput bench of the fused FP32 LdG step on one MI355X.
+
+Workload (the production relaxation): a 256^3 box, droplet radius 120 voxels,
+dx = 10 nm, bulk a = 0.044e6, b = 1.413e6, c = 1.153e6, T* = 307 K, T = 300 K,
+L1/L2/L3 = 6e-12 / 3e-12 / 1e-12 N, Rapini-Papoular anchoring W = 1e-4 J/m^2
+with custom shell order 0.5, gamma = 0.1 Pa s, dt = dt_max of the stability
+window; seeded random-director start (S0 = 0.5, seed 12345).  A "step" is one
+explicit-Euler iteration of the whole box (fused stencil launch + box-face
+launch).  Inputs are resident in HBM when the timed region starts.
+
+Prints one JSON line: value = box voxel-updates/s; roofline = algorithmic
+bytes of the fused stencil (40 B per voxel of the launched tiles: read 5 FP32,
+write 5 FP32) / its average launch time from hipEvents on the context stream;
+cpu_baseline = the reference's own QSR.cpp step functions (oracle/_ref,
+OpenMP) timed on this host on a bounded sample of the same box.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_VOXEL = 40.0
+METRIC = "Q-voxel updates/s & % HBM peak; time-to-ΔF/F<1e-6 batched/surrogate vs plain"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def workload_params(q, n=256, R=120.0):
+    p = q.Params(Nx=n, Ny=n, Nz=n, dx=1e-8, dy=1e-8, dz=1e-8, a=0.044e6, b=1.413e6, c=1.153e6, T_star=307.0,
+                 T_high=300.0, kappa=0.0, L1=6e-12, L2=3e-12, L3=1e-12, W=1e-4, gamma=0.1, radius_index=R,
+                 boundary_custom=1, boundary_S=0.5, seed=12345)
+    w = q.dt_estimate(p.dx, p.dy, p.dz, p.gamma, p.a, p.b, p.c, p.T_star, 1, p.kappa, p.L1, p.L2, p.L3, math.nan,
+                      0.5, p.T_high, p.T_high)
+    p.dt = w["dt_max"]
+    return p
+
+
+def cpu_baseline(p, budget_s=15.0):
+    """Reference QSR.cpp step functions (oracle/_ref) on the same 256^3 box."""
+    import ctypes as C
+
+    import numpy as np
+
+    from oracle import oracle as O
+    R = O.ref_lib()
+    kind = "reference"
+    if R is None:
+        return None
+    Q = np.zeros(p.dims + (5,))
+    rng = np.random.default_rng(1)
+    Q[...] = rng.normal(0, 0.2, Q.shape)
+    params = (C.c_double * 9)(p.a, p.b, p.c, p.T_high, p.T_star, p.L1, p.L2, p.L3, p.W)
+    dp = C.POINTER(C.c_double)
+    args = lambda s: (Q.ctypes.data_as(dp), p.Nx, p.Ny, p.Nz, C.c_double(p.dx), C.c_double(p.dy),
+                      C.c_double(p.dz), params, C.c_double(p.kappa), 1, C.c_double(p.dt), C.c_double(p.gamma),
+                      C.c_double(0.5), s)
+    t1 = R.ref_time_steps(*args(1))
+    steps = max(1, min(20, int(budget_s / max(t1, 1e-3))))
+    t = R.ref_time_steps(*args(steps))
+    cores = R.ref_max_threads()
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next(ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
+    return {"value": p.n * steps / t, "unit": "voxel-updates/s", "cores": int(cores), "kind": kind,
+            "sample": f"QSR.cpp computeLaplacian + computeChemicalPotential(L1,L2,L3) + applyWeakAnchoringPenalty "
+                      f"+ updateQTensor + applyBoundaryConditions, {steps} steps of the {p.Nx}^3 box "
+                      f"(first step {t1:.2f} s excluded), OpenMP threads={cores} on {model}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--radius", type=float, default=120.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--graph", type=int, default=0, help="replay steps from a hipGraph of this many steps")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU slab decomposition is a design note only)", file=sys.stderr)
+        return 2
+    import numpy as np
+
+    import lcdrsimulator_amd as q
+    q.load()
+    if q.device_count() < 1:
+        print("bench.py: no HIP device", file=sys.stderr)
+        return 1
+    p = workload_params(q, args.size, args.radius)
+    log(f"box {p.dims} R={p.radius_index} dt={p.dt:.5e}")
+    Q0 = q.init_field(p, 0, 0.5, 0.0, 12345)
+    with q.Solver(p) as s:
+        s.upload(Q0)
+        del Q0
+        s.step(args.warmup)
+        log(f"warmup {args.warmup} steps done")
+        t0 = time.perf_counter()
+        if args.graph:
+            s.step_graph(args.steps, args.graph)
+        else:
+            s.step(args.steps)
+        wall = time.perf_counter() - t0
+        ev_ms = s.last_step_ms()
+        log(f"timed {args.steps} steps: wall {wall:.3f} s, event {ev_ms:.4f} ms/step")
+        main_ms, face_ms = s.profile_steps(min(100, args.steps))
+        work = s.work()
+        obs = s.observables()
+        log(f"fused stencil {main_ms:.4f} ms, faces {face_ms:.4f} ms; F={obs['total']:.6e}")
+    ms_per_step = 1e3 * wall / args.steps
+    value = work["box_voxels"] * args.steps / wall
+    traffic_bytes = BYTES_PER_VOXEL * work["tile_voxels"]
+    achieved = traffic_bytes / (main_ms * 1e-3) / 1e9
+    cpu = None if args.no_cpu else cpu_baseline(p)
+    out = {
+        "metric": METRIC, "value": value, "unit": "voxel-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded random-director droplet, reference init_mode 0)",
+        "config": {"workload": f"LdG droplet relaxation {p.Nx}^3, R={p.radius_index:g}, L1/L2/L3=6e-12/3e-12/1e-12, "
+                               f"weak anchoring W=1e-4 custom S=0.5, explicit Euler dt={p.dt:.5e}",
+                   "box_voxels": work["box_voxels"], "active_voxels": work["active_voxels"],
+                   "tile_voxels": work["tile_voxels"], "graph_chunk": args.graph,
+                   "droplet_voxel_updates_per_s": work["active_voxels"] * args.steps / wall,
+                   "fused_stencil_ms": main_ms, "face_ms": face_ms, "event_ms_per_step": ev_ms,
+                   "final_total_F": obs["total"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -144,6 +144,10 @@ int qsr_set_iteration(void* ctx, int box, int64_t iteration);
 /* Instrumentation for measurement: average duration (ms) of the last `qsr_step`
  * call's per-step launches, measured with hipEvents on the context stream. */
 int qsr_last_step_ms(void* ctx, double* ms_per_step);
+/* Instrumentation: run nsteps (<= 512) with hipEvents recorded on the
+ * context stream around every launch; returns the average duration (ms) of the
+ * fused stencil launch and of the box-face launch. */
+int qsr_profile_steps(void* ctx, int nsteps, double* main_ms, double* face_ms);
 /* Work description of one step: bytes moved per step if each active voxel is
  * read and written once (FP32 SoA, 40 B/voxel), voxels covered by launched
  * tiles, and box voxels. */

@@ -539,6 +539,46 @@ int qsr_step_graph(void* ctx, int nsteps, int chunk) {
     return QSR_OK;
 }
 
+int qsr_profile_steps(void* ctx, int nsteps, double* main_ms, double* face_ms) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || nsteps < 1 || nsteps > kDynChunk || !main_ms || !face_ms) return fail(QSR_E_ARG, "bad argument");
+    if (int rc = check_uploaded(*c)) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    if (int rc = fill_dyn(*c, nsteps)) return rc;
+    std::vector<hipEvent_t> ev(3 * size_t(nsteps));
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    const int nb = int(c->boxes.size());
+    for (int s = 0; s < nsteps; ++s) {
+        const float* qin = c->q[c->cur];
+        float* qout = c->q[1 - c->cur];
+        const qsrk::StepDyn* dyn = c->d_dyn + (size_t)s * nb;
+        HIP_TRY(hipEventRecord(ev[3 * s], c->st));
+        if (c->ntiles > 0)
+            HIP_TRY(qsrk::launch_step(c->full, c->noise, c->ntiles, c->st, qin, qout, c->cs, c->d_boxes, c->d_tiles,
+                                      dyn, nb, c->d_step_rows, c->ch));
+        HIP_TRY(hipEventRecord(ev[3 * s + 1], c->st));
+        HIP_TRY(qsrk::launch_faces(c->nface, c->st, qin, qout, c->cs, c->d_boxes, dyn, c->d_face_prefix, nb,
+                                   c->d_step_rows));
+        HIP_TRY(hipEventRecord(ev[3 * s + 2], c->st));
+        c->cur = 1 - c->cur;
+    }
+    HIP_TRY(hipStreamSynchronize(c->st));
+    advance_iters(*c, nsteps);
+    double tm = 0, tf = 0;
+    for (int s = 0; s < nsteps; ++s) {
+        float a = 0, b = 0;
+        HIP_TRY(hipEventElapsedTime(&a, ev[3 * s], ev[3 * s + 1]));
+        HIP_TRY(hipEventElapsedTime(&b, ev[3 * s + 1], ev[3 * s + 2]));
+        tm += a;
+        tf += b;
+    }
+    for (auto& e : ev) discard(hipEventDestroy(e));
+    *main_ms = tm / nsteps;
+    *face_ms = tf / nsteps;
+    return QSR_OK;
+}
+
 int qsr_step_imex(void* ctx, int nsteps, double L_stab, int jacobi_iters) {
     (void)ctx; (void)nsteps; (void)L_stab; (void)jacobi_iters;
     return fail(QSR_E_STATE, "IMEX stepping is not available in this build");

@@ -72,7 +72,7 @@ EXPORTS = (
     "qsr_create", "qsr_create_batch", "qsr_destroy", "qsr_last_error", "qsr_device_count",
     "qsr_upload_q", "qsr_download_q", "qsr_download_nematic", "qsr_step", "qsr_step_graph",
     "qsr_step_imex", "qsr_observables", "qsr_box_info_get", "qsr_set_iteration", "qsr_last_step_ms",
-    "qsr_step_work", "qsr_S_eq", "qsr_dt_estimate", "qsr_init_field",
+    "qsr_step_work", "qsr_profile_steps", "qsr_S_eq", "qsr_dt_estimate", "qsr_init_field",
 )
 
 
@@ -99,6 +99,7 @@ def load():
     L.qsr_box_info_get.argtypes = [vp, C.c_int, C.POINTER(_BoxInfo)]
     L.qsr_set_iteration.argtypes = [vp, C.c_int, C.c_int64]
     L.qsr_last_step_ms.argtypes = [vp, dp]
+    L.qsr_profile_steps.argtypes = [vp, C.c_int, dp, dp]
     L.qsr_step_work.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.qsr_S_eq.restype = C.c_double
     L.qsr_S_eq.argtypes = [C.c_double] * 5 + [C.c_int]
@@ -253,6 +254,12 @@ class Solver:
         v = C.c_double()
         _check(load().qsr_last_step_ms(self._h, C.byref(v)))
         return v.value
+
+    def profile_steps(self, n: int):
+        """(main-kernel ms, face-kernel ms) averaged over n steps, HIP events."""
+        a, b = C.c_double(), C.c_double()
+        _check(load().qsr_profile_steps(self._h, int(n), C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def work(self) -> dict:
         a, t, b = C.c_uint64(), C.c_uint64(), C.c_uint64()
