@@ -36,7 +36,27 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def workload_params(q, n=256, R=120.0):
+WORKLOADS = ("droplet_l1l2l3_256", "periodic_noise_256", "droplet_l1_64")
+
+
+def workload_params(q, n=256, R=120.0, workload="droplet_l1l2l3_256"):
+    if workload == "periodic_noise_256":
+        # periodic bulk quench: 256^3, T ramp 310 -> 295 K, kappa = 6e-12, noise 1e-3 * S_eq(295 K)
+        S295 = q.S_eq(0.044e6, 1.413e6, 1.153e6, 295.0, 307.0, 1)
+        p = q.Params(Nx=n, Ny=n, Nz=n, dx=1e-8, dy=1e-8, dz=1e-8, T_star=307.0, kappa=6e-12, gamma=0.1,
+                     periodic=1, protocol=q.PROTOCOL_RAMP, T_high=310.0, T_low=295.0, ramp_iters=20000,
+                     noise_sigma=1e-3 * S295, seed=0)
+        w = q.dt_estimate(p.dx, p.dy, p.dz, p.gamma, p.a, p.b, p.c, p.T_star, 1, p.kappa, 0, 0, 0, math.nan, 0.5,
+                          310.0, 295.0)
+        p.dt = w["dt_max"]
+        return p
+    if workload == "droplet_l1_64":
+        p = q.Params(Nx=64, Ny=64, Nz=64, dx=1e-8, dy=1e-8, dz=1e-8, T_star=307.0, T_high=300.0, kappa=0.0,
+                     L1=6e-12, gamma=0.1, radius_index=28.0)
+        w = q.dt_estimate(p.dx, p.dy, p.dz, p.gamma, p.a, p.b, p.c, p.T_star, 1, 0.0, p.L1, 0, 0, math.nan, 0.5,
+                          300.0, 300.0)
+        p.dt = w["dt_max"]
+        return p
     p = q.Params(Nx=n, Ny=n, Nz=n, dx=1e-8, dy=1e-8, dz=1e-8, a=0.044e6, b=1.413e6, c=1.153e6, T_star=307.0,
                  T_high=300.0, kappa=0.0, L1=6e-12, L2=3e-12, L3=1e-12, W=1e-4, gamma=0.1, radius_index=R,
                  boundary_custom=1, boundary_S=0.5, seed=12345)
@@ -81,6 +101,16 @@ def cpu_baseline(p, budget_s=15.0):
                       f"(first step {t1:.2f} s excluded), OpenMP threads={cores} on {model}"}
 
 
+def describe(p, workload):
+    if workload == "periodic_noise_256":
+        return (f"periodic bulk LdG quench {p.Nx}^3, kappa=6e-12, T ramp 310->295 K, Philox noise sigma="
+                f"{p.noise_sigma:.4e}, dt={p.dt:.5e}")
+    if workload == "droplet_l1_64":
+        return f"LdG droplet relaxation {p.Nx}^3, R=28, L1=6e-12, strong anchoring, dt={p.dt:.5e}"
+    return (f"LdG droplet relaxation {p.Nx}^3, R={p.radius_index:g}, L1/L2/L3=6e-12/3e-12/1e-12, "
+            f"weak anchoring W=1e-4 custom S=0.5, explicit Euler dt={p.dt:.5e}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,11 +120,28 @@ def main():
     ap.add_argument("--radius", type=float, default=120.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--graph", type=int, default=0, help="replay steps from a hipGraph of this many steps")
+    ap.add_argument("--tuning", default="", metavar="VARIANT,PLANES",
+                    help="fixed stencil variant and planes per tile instead of autotuning (profiling runs)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of this many MiB (counter calibration) and exit")
+    ap.add_argument("--workload", default=WORKLOADS[0], choices=WORKLOADS,
+                    help="droplet_l1l2l3_256 is the headline line; the others are the periodic noisy quench "
+                         "and the 64^3 L1 droplet")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU slab decomposition is a design note only)", file=sys.stderr)
         return 2
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 * n, "launches": 5}), flush=True)
+        return 0
     import numpy as np
 
     import lcdrsimulator_amd as q
@@ -102,12 +149,22 @@ def main():
     if q.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
-    p = workload_params(q, args.size, args.radius)
+    p = workload_params(q, args.size, args.radius, args.workload)
     log(f"box {p.dims} R={p.radius_index} dt={p.dt:.5e}")
-    Q0 = q.init_field(p, 0, 0.5, 0.0, 12345)
+    if p.periodic:
+        Q0 = np.random.default_rng(7).normal(0.0, 1e-3, p.dims + (5,))
+    else:
+        Q0 = q.init_field(p, 0, 0.5, 0.0, 12345)
     with q.Solver(p) as s:
         s.upload(Q0)
         del Q0
+        if args.tuning:
+            v, ch = (int(x) for x in args.tuning.split(","))
+            s.set_tuning(v, ch)
+            tune = {"variant": v, "planes_per_tile": ch, "fixed": True}
+        else:
+            tune = s.autotune(10)
+        log(f"tuning: {tune}")
         s.step(args.warmup)
         log(f"warmup {args.warmup} steps done")
         t0 = time.perf_counter()
@@ -126,16 +183,15 @@ def main():
     value = work["box_voxels"] * args.steps / wall
     traffic_bytes = BYTES_PER_VOXEL * work["tile_voxels"]
     achieved = traffic_bytes / (main_ms * 1e-3) / 1e9
-    cpu = None if args.no_cpu else cpu_baseline(p)
+    cpu = None if (args.no_cpu or args.workload != WORKLOADS[0]) else cpu_baseline(p)
     out = {
         "metric": METRIC, "value": value, "unit": "voxel-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic (seeded random-director droplet, reference init_mode 0)",
-        "config": {"workload": f"LdG droplet relaxation {p.Nx}^3, R={p.radius_index:g}, L1/L2/L3=6e-12/3e-12/1e-12, "
-                               f"weak anchoring W=1e-4 custom S=0.5, explicit Euler dt={p.dt:.5e}",
+        "config": {"workload": describe(p, args.workload),
                    "box_voxels": work["box_voxels"], "active_voxels": work["active_voxels"],
-                   "tile_voxels": work["tile_voxels"], "graph_chunk": args.graph,
+                   "tile_voxels": work["tile_voxels"], "graph_chunk": args.graph, "autotune": tune,
                    "droplet_voxel_updates_per_s": work["active_voxels"] * args.steps / wall,
                    "fused_stencil_ms": main_ms, "face_ms": face_ms, "event_ms_per_step": ev_ms,
                    "final_total_F": obs["total"]},

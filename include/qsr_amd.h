@@ -144,6 +144,14 @@ int qsr_set_iteration(void* ctx, int box, int64_t iteration);
 /* Instrumentation for measurement: average duration (ms) of the last `qsr_step`
  * call's per-step launches, measured with hipEvents on the context stream. */
 int qsr_last_step_ms(void* ctx, double* ms_per_step);
+/* Autotuner: times every stencil-kernel variant and planes-per-tile choice
+ * for trial_steps launches on the resident field (the trial launches write the
+ * spare buffer only, so the state and iteration count do not change) and keeps
+ * the fastest.  Results are bit-identical across choices. */
+int qsr_autotune(void* ctx, int trial_steps, int* best_variant, int* best_ch, double* best_ms);
+/* Set the stencil variant (0: 3 waves/SIMD, 1: 4 waves/SIMD register budget;
+ * ignored for the L1 kernel) and planes per tile explicitly. */
+int qsr_set_tuning(void* ctx, int variant, int planes_per_tile);
 /* Instrumentation: run nsteps (<= 512) with hipEvents recorded on the
  * context stream around every launch; returns the average duration (ms) of the
  * fused stencil launch and of the box-face launch. */

@@ -82,6 +82,7 @@ struct Ctx {
     int cur = 0;
     bool full = false, noise = false;
     int ch = 32;
+    int variant = 0;  // stencil kernel variant (waves/SIMD target), chosen by qsr_autotune
     std::vector<qsrk::BoxDev> h_boxes;
     qsrk::BoxDev* d_boxes = nullptr;
     std::vector<int4> h_tiles;
@@ -359,7 +360,7 @@ int enqueue_steps(Ctx& c, int nsteps) {
         float* qout = c.q[1 - c.cur];
         const qsrk::StepDyn* dyn = c.d_dyn + (size_t)s * nb;
         if (c.ntiles > 0)
-            HIP_TRY(qsrk::launch_step(c.full, c.noise, c.ntiles, c.st, qin, qout, c.cs, c.d_boxes, c.d_tiles, dyn, nb,
+            HIP_TRY(qsrk::launch_step(c.full, c.variant, c.noise, c.ntiles, c.st, qin, qout, c.cs, c.d_boxes, c.d_tiles, dyn, nb,
                                       c.d_step_rows, c.ch));
         HIP_TRY(qsrk::launch_faces(c.nface, c.st, qin, qout, c.cs, c.d_boxes, dyn, c.d_face_prefix, nb, c.d_step_rows));
         c.cur = 1 - c.cur;
@@ -555,7 +556,7 @@ int qsr_profile_steps(void* ctx, int nsteps, double* main_ms, double* face_ms) {
         const qsrk::StepDyn* dyn = c->d_dyn + (size_t)s * nb;
         HIP_TRY(hipEventRecord(ev[3 * s], c->st));
         if (c->ntiles > 0)
-            HIP_TRY(qsrk::launch_step(c->full, c->noise, c->ntiles, c->st, qin, qout, c->cs, c->d_boxes, c->d_tiles,
+            HIP_TRY(qsrk::launch_step(c->full, c->variant, c->noise, c->ntiles, c->st, qin, qout, c->cs, c->d_boxes, c->d_tiles,
                                       dyn, nb, c->d_step_rows, c->ch));
         HIP_TRY(hipEventRecord(ev[3 * s + 1], c->st));
         HIP_TRY(qsrk::launch_faces(c->nface, c->st, qin, qout, c->cs, c->d_boxes, dyn, c->d_face_prefix, nb,
@@ -576,6 +577,74 @@ int qsr_profile_steps(void* ctx, int nsteps, double* main_ms, double* face_ms) {
     for (auto& e : ev) discard(hipEventDestroy(e));
     *main_ms = tm / nsteps;
     *face_ms = tf / nsteps;
+    return QSR_OK;
+}
+
+int qsr_autotune(void* ctx, int trial_steps, int* best_variant, int* best_ch, double* best_ms) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || trial_steps < 1) return fail(QSR_E_ARG, "bad argument");
+    if (int rc = check_uploaded(*c)) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    if (int rc = fill_dyn(*c, 1)) return rc;
+    const int nb = int(c->boxes.size());
+    int maxNx = 0;
+    for (const Box& b : c->boxes) maxNx = std::max(maxNx, b.p.Nx);
+    double best = 1e300;
+    int bv = c->variant, bch = c->ch;
+    const int nvar = c->full ? 2 : 1;
+    for (int ch : {8, 16, 32, 64}) {
+        if (ch > 8 && ch / 2 >= maxNx) break;
+        c->ch = ch;
+        if (int rc = build_tiles(*c)) return rc;
+        for (int v = 0; v < nvar; ++v) {
+            c->variant = v;
+            // trial launches read the current buffer and write the spare one: the state does not advance
+            for (int s = -1; s < trial_steps; ++s) {
+                if (s == 0) HIP_TRY(hipEventRecord(c->ev[0], c->st));
+                if (c->ntiles > 0)
+                    HIP_TRY(qsrk::launch_step(c->full, c->variant, c->noise, c->ntiles, c->st, c->q[c->cur],
+                                              c->q[1 - c->cur], c->cs, c->d_boxes, c->d_tiles, c->d_dyn, nb,
+                                              c->d_step_rows, c->ch));
+            }
+            HIP_TRY(hipEventRecord(c->ev[1], c->st));
+            HIP_TRY(hipEventSynchronize(c->ev[1]));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            const double per = double(ms) / trial_steps;
+            if (per < best) {
+                best = per;
+                bv = v;
+                bch = ch;
+            }
+        }
+    }
+    c->variant = bv;
+    c->ch = bch;
+    if (int rc = build_tiles(*c)) return rc;
+    if (c->gexec) discard(hipGraphExecDestroy(c->gexec));
+    if (c->graph) discard(hipGraphDestroy(c->graph));
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    if (best_variant) *best_variant = bv;
+    if (best_ch) *best_ch = bch;
+    if (best_ms) *best_ms = best;
+    return QSR_OK;
+}
+
+int qsr_set_tuning(void* ctx, int variant, int planes_per_tile) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || variant < 0 || variant > 1 || planes_per_tile < 1 || planes_per_tile > 4096)
+        return fail(QSR_E_ARG, "variant must be 0/1 and planes_per_tile in [1, 4096]");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    c->variant = c->full ? variant : 0;
+    c->ch = planes_per_tile;
+    if (int rc = build_tiles(*c)) return rc;
+    if (c->gexec) discard(hipGraphExecDestroy(c->gexec));
+    if (c->graph) discard(hipGraphDestroy(c->graph));
+    c->gexec = nullptr;
+    c->graph = nullptr;
     return QSR_OK;
 }
 

@@ -84,7 +84,7 @@ constexpr int NPLAQ = 3;     // periodic plaquette slots
 
 // Host launchers (defined in qsr_step.hip / qsr_obs.hip)
 namespace qsrk {
-hipError_t launch_step(bool full, bool noise, int ntiles, hipStream_t st, const float* qin, float* qout,
+hipError_t launch_step(bool full, int variant, bool noise, int ntiles, hipStream_t st, const float* qin, float* qout,
                        long long cs, const BoxDev* boxes, const int4* tiles, const StepDyn* dyn, int nboxes,
                        const RowMask* rows, int ch);
 hipError_t launch_faces(long long nface, hipStream_t st, const float* qin, float* qout, long long cs,

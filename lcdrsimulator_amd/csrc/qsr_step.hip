@@ -257,168 +257,248 @@ __global__ __launch_bounds__(TK* TJ) void step_l1_kernel(const float* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// Anisotropic kernel (L2 and/or L3): five LDS Q-plane slots with halo 2 and two
-// Dcol-plane slots with halo 1; Dcol = div of Q's columns (QSR.cu:325-371) is
-// computed once per point on the halo ring and reused by the L2 derivatives
-// (QSR.cu:436-460) and the L3 terms (QSR.cu:474-621).
+// Anisotropic kernel (L2 and/or L3).  Streams along i with
+//   * one "lead" plane (i = t+1) in LDS with an in-plane halo of 2,
+//     double-buffered, from which every thread takes its own first derivatives
+//     d_y Q, d_z Q, second differences and the mixed d_y d_z Q;
+//   * per-thread register queues along i for Q(t-1..t+2), d_yQ/d_zQ(t-1..t+1)
+//     and Dcol(t-1..t+1), so x-derivatives need no LDS: the mixed
+//     d_x d_y Q = (d_yQ(t+1) - d_yQ(t-1)) / (2 dx) is exactly the reference's
+//     4-point formula (QSR.cu:583-588);
+//   * Dcol = d_k Q_kj (QSR.cu:325-371) in an LDS plane with a halo of 1: own
+//     points reuse their first derivatives, the halo ring is computed by ring
+//     threads that keep a 3-component x-queue of their ring point.
+// Output plane t is produced after lead plane t+1 has been processed.
 // ---------------------------------------------------------------------------
-template <bool NOISE>
-__global__ __launch_bounds__(TK* TJ) void step_full_kernel(const float* __restrict__ qin, float* __restrict__ qout,
-                                                            long long cs, const BoxDev* __restrict__ boxes,
-                                                            const int4* __restrict__ tiles,
-                                                            const StepDyn* __restrict__ dyn, int nboxes,
-                                                            const RowMask* __restrict__ rows, int ch) {
-    constexpr int W2 = TK + 4, R2 = TJ + 4, P2 = W2 * R2, QSLOT = 5 * P2;
-    constexpr int W1 = TK + 2, R1 = TJ + 2, P1 = W1 * R1, DSLOT = 3 * P1;
-    constexpr int NT = TK * TJ;
-    constexpr int NPF = (P2 + NT - 1) / NT;  // plane points per thread
-    __shared__ float qs[5 * QSLOT];
+namespace {
+
+template <int TJv>
+struct AnisoGeom {
+    static constexpr int W2 = TK + 4, R2 = TJv + 4, P2 = W2 * R2, QSLOT = 5 * P2;
+    static constexpr int W1 = TK + 2, R1 = TJv + 2, P1 = W1 * R1, DSLOT = 3 * P1;
+    static constexpr int NT = TK * TJv;
+    static constexpr int NRING = 2 * TK + 2 * TJv;
+    static constexpr int NHALO = P2 - TK * TJv;               // halo points of a lead plane
+    static constexpr int NHPT = (NHALO + NT - 1) / NT;         // halo points per thread
+};
+
+}  // namespace
+
+template <int TJv, int MINW, bool NOISE>
+__global__ __launch_bounds__(TK* TJv, MINW) void step_aniso_kernel(const float* __restrict__ qin, float* __restrict__ qout,
+                                                             long long cs, const BoxDev* __restrict__ boxes,
+                                                             const int4* __restrict__ tiles,
+                                                             const StepDyn* __restrict__ dyn, int nboxes,
+                                                             const RowMask* __restrict__ rows, int ch) {
+    using G = AnisoGeom<TJv>;
+    constexpr int W2 = G::W2, P2 = G::P2, QSLOT = G::QSLOT, W1 = G::W1, P1 = G::P1, DSLOT = G::DSLOT;
+    __shared__ float qs[2 * QSLOT];
     __shared__ float ds[2 * DSLOT];
-    const int4 t = tiles[blockIdx.x];
-    if (t.x < 0) return;  // padding tile of the XCD-ordered list (uniform over the block)
-    const BoxDev B = boxes[t.x];
-    const StepDyn D = dyn[t.x];
+    const int4 t4 = tiles[blockIdx.x];
+    if (t4.x < 0) return;  // padding tile of the XCD-ordered list (uniform over the block)
+    const BoxDev B = boxes[t4.x];
+    const StepDyn D = dyn[t4.x];
     const int tid = threadIdx.x, tk = tid & (TK - 1), tj = tid / TK;
     const int Nx = B.Nx, Ny = B.Ny, Nz = B.Nz;
     const bool per = (B.flags & F_PERIODIC) != 0;
-    int ib = t.y, ie = min(t.y + ch, Nx);
+    int ib = t4.y, ie = min(t4.y + ch, Nx);
     if (!per) {
         ib = max(ib, 1);
         ie = min(ie, Nx - 1);
     }
     if (ib >= ie) return;
-    const int j0 = t.z, k0 = t.w;
+    const int j0 = t4.z, k0 = t4.w;
     const int j = j0 + tj, k = k0 + tk;
     const bool out_ok = per ? (j < Ny && k < Nz) : (j >= 1 && j <= Ny - 2 && k >= 1 && k <= Nz - 2);
     const long long plane = (long long)Ny * Nz;
     const float* base = qin + B.off;
     float* obase = qout + B.off;
     const bool hasL2 = (B.flags & F_L2) != 0, hasL3 = (B.flags & F_L3) != 0;
+    auto face = [&](int x, int n) { return !per && (x == 0 || x == n - 1); };
 
-    // per-thread in-plane offsets of the plane points it loads
-    long long poff[NPF];
-    int pidx[NPF];
-#pragma unroll
-    for (int s = 0; s < NPF; ++s) {
-        const int id = tid + s * NT;
-        pidx[s] = id < P2 ? id : -1;
-        const int r = id / W2, c = id % W2;
-        const int jj = wrap_or_clamp(j0 - 2 + r, Ny, per), kk = wrap_or_clamp(k0 - 2 + c, Nz, per);
-        poff[s] = (long long)jj * Nz + kk;
+    // own column
+    const long long o_own = (long long)wrap_or_clamp(j, Ny, per) * Nz + wrap_or_clamp(k, Nz, per);
+    const float ay_own = face(j, Ny) ? B.fy : B.hy, az_own = face(k, Nz) ? B.fz : B.hz;
+    // ring point of this thread (if any), in region coordinates (rj, rk)
+    const bool ring = tid < G::NRING;
+    int rj = 0, rk = 0;
+    if (ring) {
+        if (tid < TK) { rj = j0 - 1; rk = k0 + tid; }
+        else if (tid < 2 * TK) { rj = j0 + TJv; rk = k0 + tid - TK; }
+        else if (tid < 2 * TK + TJv) { rj = j0 + tid - 2 * TK; rk = k0 - 1; }
+        else { rj = j0 + tid - 2 * TK - TJv; rk = k0 + TK; }
     }
-    float pf[NPF][5];
-    auto fetch = [&](int p) {
-        const long long pb = (long long)wrap_or_clamp(p, Nx, per) * plane;
+    const long long o_ring = (long long)wrap_or_clamp(rj, Ny, per) * Nz + wrap_or_clamp(rk, Nz, per);
+    const float ay_ring = face(rj, Ny) ? B.fy : B.hy, az_ring = face(rk, Nz) ? B.fz : B.hz;
+    const int lq_ring = (rj - j0 + 2) * W2 + (rk - k0 + 2);   // position in a Q slot
+    const int ld_ring = (rj - j0 + 1) * W1 + (rk - k0 + 1);   // position in a D slot
+    const int lq = (tj + 2) * W2 + tk + 2, ld = (tj + 1) * W1 + tk + 1;
+    // halo points of a lead plane handled by this thread (halo = Q slot minus the interior)
+    long long hoff[G::NHPT];
+    int hpos[G::NHPT];
 #pragma unroll
-        for (int s = 0; s < NPF; ++s)
-            if (pidx[s] >= 0) {
+    for (int s = 0; s < G::NHPT; ++s) {
+        const int h = tid + s * G::NT;
+        hpos[s] = -1;
+        hoff[s] = 0;
+        if (h < G::NHALO) {
+            // enumerate the slot row-major and skip the interior block
+            int id = h, r = 0, c = 0;
+            const int top = 2 * W2;                       // rows 0, 1
+            const int mid = TJv * 4;                      // rows 2..TJv+1: 4 halo points each
+            if (id < top) { r = id / W2; c = id % W2; }
+            else if (id < top + mid) { id -= top; r = 2 + id / 4; c = id % 4; c = c < 2 ? c : TK + c; }
+            else { id -= top + mid; r = TJv + 2 + id / W2; c = id % W2; }
+            hpos[s] = r * W2 + c;
+            hoff[s] = (long long)wrap_or_clamp(j0 - 2 + r, Ny, per) * Nz + wrap_or_clamp(k0 - 2 + c, Nz, per);
+        }
+    }
+
+    auto pl = [&](int p) { return (long long)wrap_or_clamp(p, Nx, per) * plane; };
+    auto ld5 = [&](int p, long long o, float v[5]) {
+        const float* s = base + pl(p) + o;
 #pragma unroll
-                for (int c = 0; c < 5; ++c) pf[s][c] = base[c * cs + pb + poff[s]];
-            }
+        for (int c = 0; c < 5; ++c) v[c] = s[c * cs];
     };
-    auto commit = [&](int slot) {
+    auto ld3 = [&](int p, long long o, float v[3]) {
+        const float* s = base + pl(p) + o;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c] = s[c * cs];
+    };
+    // write plane p (own value `own` + halo straight from global) into Q slot `slot`
+    auto fill_slot = [&](int slot, int p, const float own[5]) {
         float* Q = qs + slot * QSLOT;
+        const long long pb = pl(p);
 #pragma unroll
-        for (int s = 0; s < NPF; ++s)
-            if (pidx[s] >= 0) {
+        for (int c = 0; c < 5; ++c) Q[c * P2 + lq] = own[c];
 #pragma unroll
-                for (int c = 0; c < 5; ++c) Q[c * P2 + pidx[s]] = pf[s][c];
+        for (int s = 0; s < G::NHPT; ++s)
+            if (hpos[s] >= 0) {
+#pragma unroll
+                for (int c = 0; c < 5; ++c) Q[c * P2 + hpos[s]] = base[c * cs + pb + hoff[s]];
             }
     };
-    // first-derivative scale along an axis at coordinate x: one-sided on faces
-    auto dscale = [&](int x, int n, float half, float full) {
-        return (!per && (x == 0 || x == n - 1)) ? full : half;
-    };
-    // Dcol on the (TJ+2) x (TK+2) ring-including region of plane p (loop plane
-    // index, may be outside [0, Nx) only for clamped-away planes that are unused)
-    auto dcol = [&](int sm, int sc, int sp, int dslot, int p) {
-        const float* Qm = qs + sm * QSLOT;
-        const float* Qc = qs + sc * QSLOT;
-        const float* Qp = qs + sp * QSLOT;
-        float* Dd = ds + dslot * DSLOT;
-        const float ax = dscale(p, Nx, B.hx, B.fx);
-        for (int id = tid; id < P1; id += NT) {
-            const int rr = id / W1, cc = id % W1;
-            const int r = rr + 1, c = cc + 1, at = r * W2 + c;
-            const int jj = j0 - 1 + rr, kk = k0 - 1 + cc;
-            const float ay = dscale(jj, Ny, B.hy, B.fy), az = dscale(kk, Nz, B.hz, B.fz);
-            float dx_[5], dy_[5], dz_[5];
+    // own first derivatives of a lead plane (face-aware scales)
+    auto own_d1 = [&](int slot, float dy[5], float dz[5]) {
+        const float* Q = qs + slot * QSLOT;
 #pragma unroll
-            for (int q = 0; q < 5; ++q) {
-                dx_[q] = (Qp[q * P2 + at] - Qm[q * P2 + at]) * ax;
-                dy_[q] = (Qc[q * P2 + at + W2] - Qc[q * P2 + at - W2]) * ay;
-                dz_[q] = (Qc[q * P2 + at + 1] - Qc[q * P2 + at - 1]) * az;
-            }
-            Dd[0 * P1 + id] = dx_[0] + dy_[1] + dz_[2];
-            Dd[1 * P1 + id] = dx_[1] + dy_[3] + dz_[4];
-            Dd[2 * P1 + id] = dx_[2] + dy_[4] - (dz_[0] + dz_[3]);
+        for (int c = 0; c < 5; ++c) {
+            const float* A = Q + c * P2;
+            dy[c] = (A[lq + W2] - A[lq - W2]) * ay_own;
+            dz[c] = (A[lq + 1] - A[lq - 1]) * az_own;
         }
     };
+    // own in-plane second differences and mixed d_y d_z of a lead plane
+    auto own_d2 = [&](int slot, float syy[5], float szz[5], float dyz[5]) {
+        const float* Q = qs + slot * QSLOT;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            const float* A = Q + c * P2;
+            const float q2 = 2.0f * A[lq];
+            syy[c] = A[lq + W2] + A[lq - W2] - q2;
+            szz[c] = A[lq + 1] + A[lq - 1] - q2;
+            dyz[c] = (A[lq + W2 + 1] - A[lq + W2 - 1] - A[lq - W2 + 1] + A[lq - W2 - 1]) * B.i4yz;
+        }
+    };
+    // Dcol from x-part (three comps xx, xy, xz differenced along x) and in-plane derivatives
+    auto dcol_own = [&](const float xm[5], const float xp[5], float ax, const float dy[5], const float dz[5],
+                        float Dv[3]) {
+        Dv[0] = (xp[0] - xm[0]) * ax + dy[1] + dz[2];
+        Dv[1] = (xp[1] - xm[1]) * ax + dy[3] + dz[4];
+        Dv[2] = (xp[2] - xm[2]) * ax + dy[4] - (dz[0] + dz[3]);
+    };
+    auto dcol_ring = [&](int slot, const float xm[3], const float xp[3], float ax, float Dv[3]) {
+        const float* Q = qs + slot * QSLOT;
+        auto dyv = [&](int c) { return (Q[c * P2 + lq_ring + W2] - Q[c * P2 + lq_ring - W2]) * ay_ring; };
+        auto dzv = [&](int c) { return (Q[c * P2 + lq_ring + 1] - Q[c * P2 + lq_ring - 1]) * az_ring; };
+        Dv[0] = (xp[0] - xm[0]) * ax + dyv(1) + dzv(2);
+        Dv[1] = (xp[1] - xm[1]) * ax + dyv(3) + dzv(4);
+        Dv[2] = (xp[2] - xm[2]) * ax + dyv(4) - (dzv(0) + dzv(3));
+    };
+    auto axs = [&](int p) { return face(p, Nx) ? B.fx : B.hx; };
+    auto store_d = [&](int dslot, int pos, const float Dv[3]) {
+        float* Dd = ds + dslot * DSLOT;
+        Dd[0 * P1 + pos] = Dv[0];
+        Dd[1 * P1 + pos] = Dv[1];
+        Dd[2 * P1 + pos] = Dv[2];
+    };
 
-    // prologue: planes ib-2 .. ib+1 into slots 0..3 (slot of plane p = (p - ib + 2) % 5)
-    auto qslot = [&](int p) { return (p - ib + 2) % 5; };
-    auto dslt = [&](int p) { return (p - ib + 2) & 1; };
-    for (int p = ib - 2; p <= ib + 1; ++p) {
-        fetch(p);
-        commit(qslot(p));
-    }
-    __syncthreads();
-    dcol(qslot(ib - 2), qslot(ib - 1), qslot(ib), dslt(ib - 1), ib - 1);
-    dcol(qslot(ib - 1), qslot(ib), qslot(ib + 1), dslt(ib), ib);
-    __syncthreads();
-    const int lq = (tj + 2) * W2 + tk + 2;  // own point in a Q slot
-    const int ld = (tj + 1) * W1 + tk + 1;  // own point in a D slot
-    float dm[3];
+    // ---- prologue: lead planes ib-1 and ib --------------------------------
+    float q0[5], q1[5], q2[5], q3[5];           // own Q(t-1), Q(t), Q(t+1), Q(t+2)
+    float dy0[5], dz0[5], dy1[5], dz1[5], dy2[5], dz2[5];
+    float syy1[5], szz1[5], dyz1[5];            // in-plane second derivatives of plane t
+    float D0[3], D1[3], D2[3];                  // own Dcol(t-1), Dcol(t), Dcol(t+1)
+    float r0[3] = {}, r1[3] = {}, r2[3] = {};   // ring comps xx,xy,xz at t, t+1, t+2
     {
-        const float* Dd = ds + dslt(ib - 1) * DSLOT;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) dm[a] = Dd[a * P1 + ld];
+        float qm2[5];
+        ld5(ib - 2, o_own, qm2);
+        ld5(ib - 1, o_own, q0);
+        ld5(ib, o_own, q1);
+        ld5(ib + 1, o_own, q2);
+        ld5(ib + 2, o_own, q3);
+        float rm[3] = {};
+        if (ring) {
+            ld3(ib - 1, o_ring, rm);
+            ld3(ib, o_ring, r0);
+            ld3(ib + 1, o_ring, r1);
+            ld3(ib + 2, o_ring, r2);
+        }
+        fill_slot(0, ib - 1, q0);
+        fill_slot(1, ib, q1);
+        __syncthreads();
+        own_d1(0, dy0, dz0);
+        dcol_own(qm2, q1, axs(ib - 1), dy0, dz0, D0);
+        own_d1(1, dy1, dz1);
+        own_d2(1, syy1, szz1, dyz1);
+        dcol_own(q0, q2, axs(ib), dy1, dz1, D1);
+        store_d(ib & 1, ld, D1);
+        if (ring) {
+            float Dr[3];
+            dcol_ring(1, rm, r1, axs(ib), Dr);
+            store_d(ib & 1, ld_ring, Dr);
+        }
+        // r-queue now holds planes ib, ib+1, ib+2 as r0, r1, r2
+        __syncthreads();
+        fill_slot(0, ib + 1, q2);
+        __syncthreads();
     }
-    fetch(ib + 2);
 
-    for (int p = ib; p < ie; ++p) {
-        commit(qslot(p + 2));
+    for (int t = ib; t < ie; ++t) {
+        const int qsl = (t - ib) & 1;  // Q slot holding lead plane t+1 (plane L sits in slot (L - ib + 1) & 1)
+        const int dsl = (t + 1) & 1;   // D slot of Dcol(t+1) (Dcol(L) sits in slot L & 1)
+        // (b) lead plane t+1: first derivatives, own and ring Dcol(t+1)
+        own_d1(qsl, dy2, dz2);
+        dcol_own(q1, q3, axs(t + 1), dy2, dz2, D2);
+        store_d(dsl, ld, D2);
+        if (ring) {
+            float Dr[3];
+            dcol_ring(qsl, r0, r2, axs(t + 1), Dr);
+            store_d(dsl, ld_ring, Dr);
+        }
         __syncthreads();
-        dcol(qslot(p), qslot(p + 1), qslot(p + 2), dslt(p + 1), p + 1);
-        __syncthreads();
-        const float* Qm = qs + qslot(p - 1) * QSLOT;
-        const float* Qc = qs + qslot(p) * QSLOT;
-        const float* Qp = qs + qslot(p + 1) * QSLOT;
-        const float* Dc = ds + dslt(p) * DSLOT;
-        const float* Dn = ds + dslt(p + 1) * DSLOT;
-        float dcen[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) dcen[a] = Dc[a * P1 + ld];
+        // (d) output plane t
         if (out_ok) {
             bool evolve, shell;
-            decode_mask(rows, B, p, j, k, evolve, shell);
+            decode_mask(rows, B, t, j, k, evolve, shell);
             const bool live = (evolve && !((B.flags & F_SKIPSHELL) && shell)) || ((B.flags & F_BC) && shell);
             if (live) {
-                float q[5], sx_[5], sy_[5], sz_[5];
+                const float* Dc = ds + (t & 1) * DSLOT;
+                float sx_[5], h[5];
 #pragma unroll
-                for (int c = 0; c < 5; ++c) {
-                    const float* A = Qc + c * P2;
-                    q[c] = A[lq];
-                    sx_[c] = Qp[c * P2 + lq] + Qm[c * P2 + lq] - 2.0f * q[c];
-                    sy_[c] = A[lq + W2] + A[lq - W2] - 2.0f * q[c];
-                    sz_[c] = A[lq + 1] + A[lq - 1] - 2.0f * q[c];
-                }
-                float h[5];
-                bulk_field(q, D.cA, B.cB, B.cC, h);
+                for (int c = 0; c < 5; ++c) sx_[c] = q2[c] + q0[c] - 2.0f * q1[c];
+                bulk_field(q1, D.cA, B.cB, B.cC, h);
 #pragma unroll
-                for (int c = 0; c < 5; ++c) h[c] -= B.lx * sx_[c] + B.ly * sy_[c] + B.lz * sz_[c];
+                for (int c = 0; c < 5; ++c) h[c] -= B.lx * sx_[c] + B.ly * syy1[c] + B.lz * szz1[c];
                 if (hasL2) {
-                    // central derivatives of Dcol at an interior voxel
-                    float dn[3], dyp[3], dym[3], dzp[3], dzm[3];
+                    float dyp[3], dym[3], dzp[3], dzm[3];
 #pragma unroll
                     for (int a = 0; a < 3; ++a) {
-                        dn[a] = Dn[a * P1 + ld];
                         dyp[a] = Dc[a * P1 + ld + W1];
                         dym[a] = Dc[a * P1 + ld - W1];
                         dzp[a] = Dc[a * P1 + ld + 1];
                         dzm[a] = Dc[a * P1 + ld - 1];
                     }
-                    const float dDx_dx = (dn[0] - dm[0]) * B.hx, dDy_dx = (dn[1] - dm[1]) * B.hx, dDz_dx = (dn[2] - dm[2]) * B.hx;
+                    const float dDx_dx = (D2[0] - D0[0]) * B.hx, dDy_dx = (D2[1] - D0[1]) * B.hx, dDz_dx = (D2[2] - D0[2]) * B.hx;
                     const float dDx_dy = (dyp[0] - dym[0]) * B.hy, dDy_dy = (dyp[1] - dym[1]) * B.hy, dDz_dy = (dyp[2] - dym[2]) * B.hy;
                     const float dDx_dz = (dzp[0] - dzm[0]) * B.hz, dDy_dz = (dzp[1] - dzm[1]) * B.hz, dDz_dz = (dzp[2] - dzm[2]) * B.hz;
                     const float gxx = B.mL2 * dDx_dx, gyy = B.mL2 * dDy_dy, gzz = B.mL2 * dDz_dz;
@@ -430,14 +510,13 @@ __global__ __launch_bounds__(TK* TJ) void step_full_kernel(const float* __restri
                     h[4] -= B.mL2 * (dDz_dy + dDy_dz);
                 }
                 if (hasL3) {
-                    // first derivatives d[c][axis], c = xx,xy,xz,yy,yz,zz
+                    // d[c][axis] at plane t (interior: central differences)
                     float d[6][3];
 #pragma unroll
                     for (int c = 0; c < 5; ++c) {
-                        const float* A = Qc + c * P2;
-                        d[c][0] = (Qp[c * P2 + lq] - Qm[c * P2 + lq]) * B.hx;
-                        d[c][1] = (A[lq + W2] - A[lq - W2]) * B.hy;
-                        d[c][2] = (A[lq + 1] - A[lq - 1]) * B.hz;
+                        d[c][0] = (q2[c] - q0[c]) * B.hx;
+                        d[c][1] = dy1[c];
+                        d[c][2] = dz1[c];
                     }
 #pragma unroll
                     for (int a = 0; a < 3; ++a) d[5][a] = -(d[0][a] + d[3][a]);
@@ -448,21 +527,17 @@ __global__ __launch_bounds__(TK* TJ) void step_full_kernel(const float* __restri
                         for (int b = a; b < 3; ++b)
                             M[a][b] = d[0][a] * d[0][b] + d[3][a] * d[3][b] + d[5][a] * d[5][b] +
                                       2.0f * (d[1][a] * d[1][b] + d[2][a] * d[2][b] + d[4][a] * d[4][b]);
-                    const float xx = q[0], xy = q[1], xz = q[2], yy = q[3], yz = q[4], zz = -(xx + yy);
+                    const float xx = q1[0], xy = q1[1], xz = q1[2], yy = q1[3], yz = q1[4], zz = -(xx + yy);
+                    // Q_ij d_i d_j with the grid scales folded into the coefficients
+                    const float cxx = xx * B.ix2, cyy = yy * B.iy2, czz = zz * B.iz2;
+                    const float cxy = 2.0f * xy * B.hx, cxz = 2.0f * xz * B.hx, cyz = 2.0f * yz;
                     float Bc[5];
 #pragma unroll
                     for (int c = 0; c < 5; ++c) {
-                        const float* A = Qc + c * P2;
-                        const float* Ap = Qp + c * P2;
-                        const float* Am = Qm + c * P2;
-                        const float d2xx = sx_[c] * B.ix2, d2yy = sy_[c] * B.iy2, d2zz = sz_[c] * B.iz2;
-                        const float d2xy = (Ap[lq + W2] - Ap[lq - W2] - Am[lq + W2] + Am[lq - W2]) * B.i4xy;
-                        const float d2xz = (Ap[lq + 1] - Ap[lq - 1] - Am[lq + 1] + Am[lq - 1]) * B.i4xz;
-                        const float d2yz = (A[lq + W2 + 1] - A[lq + W2 - 1] - A[lq - W2 + 1] + A[lq - W2 - 1]) * B.i4yz;
-                        const float contract = xx * d2xx + yy * d2yy + zz * d2zz + 2.0f * (xy * d2xy + xz * d2xz + yz * d2yz);
-                        Bc[c] = dcen[0] * d[c][0] + dcen[1] * d[c][1] + dcen[2] * d[c][2] + contract;
+                        const float contract = cxx * sx_[c] + cyy * syy1[c] + czz * szz1[c] +
+                                               cxy * (dy2[c] - dy0[c]) + cxz * (dz2[c] - dz0[c]) + cyz * dyz1[c];
+                        Bc[c] = D1[0] * d[c][0] + D1[1] * d[c][1] + D1[2] * d[c][2] + contract;
                     }
-                    // h += (L3/2) M - L3 B, B_zz = -(B_xx + B_yy), then remove the trace
                     const float hl = 0.5f * B.mL3;
                     const float tr = hl * (M[0][0] + M[1][1] + M[2][2]) * (1.0f / 3.0f);
                     h[0] += hl * M[0][0] - B.mL3 * Bc[0] - tr;
@@ -472,16 +547,34 @@ __global__ __launch_bounds__(TK* TJ) void step_full_kernel(const float* __restri
                     h[4] += hl * M[1][2] - B.mL3 * Bc[4];
                 }
                 float o[5];
-                const long long local = (long long)p * plane + (long long)j * Nz + k;
-                if (finish_voxel<NOISE>(B, D, p, j, k, local, evolve, shell, q, h, o)) {
+                const long long local = (long long)t * plane + (long long)j * Nz + k;
+                if (finish_voxel<NOISE>(B, D, t, j, k, local, evolve, shell, q1, h, o)) {
 #pragma unroll
                     for (int c = 0; c < 5; ++c) __builtin_nontemporal_store(o[c], obase + c * cs + local);
                 }
             }
         }
+        // in-plane second derivatives of plane t+1 while its slot is resident
+        own_d2(qsl, syy1, szz1, dyz1);
+        // rotate the queues: t+1 becomes t
 #pragma unroll
-        for (int a = 0; a < 3; ++a) dm[a] = dcen[a];
-        if (p + 1 < ie) fetch(p + 3);
+        for (int c = 0; c < 5; ++c) {
+            dy0[c] = dy1[c]; dz0[c] = dz1[c];
+            dy1[c] = dy2[c]; dz1[c] = dz2[c];
+            q0[c] = q1[c]; q1[c] = q2[c]; q2[c] = q3[c];
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            D0[a] = D1[a]; D1[a] = D2[a];
+            r0[a] = r1[a]; r1[a] = r2[a];
+        }
+        if (t + 1 < ie) {
+            // (e) lead plane t+2 into the other Q slot; fetch plane t+3 into the queues
+            fill_slot(qsl ^ 1, t + 2, q2);
+            ld5(t + 3, o_own, q3);
+            if (ring) ld3(t + 3, o_ring, r2);
+        }
+        __syncthreads();
     }
 }
 
@@ -648,27 +741,38 @@ template __global__ void step_l1_kernel<false>(const float*, float*, long long, 
                                                const StepDyn*, int, const RowMask*, int);
 template __global__ void step_l1_kernel<true>(const float*, float*, long long, const BoxDev*, const int4*,
                                               const StepDyn*, int, const RowMask*, int);
-template __global__ void step_full_kernel<false>(const float*, float*, long long, const BoxDev*, const int4*,
-                                                 const StepDyn*, int, const RowMask*, int);
-template __global__ void step_full_kernel<true>(const float*, float*, long long, const BoxDev*, const int4*,
-                                                const StepDyn*, int, const RowMask*, int);
+#define QSR_INST_ANISO(MINW, NZ)                                                                         \
+    template __global__ void step_aniso_kernel<TJ, MINW, NZ>(const float*, float*, long long, const BoxDev*, \
+                                                             const int4*, const StepDyn*, int, const RowMask*, int);
+QSR_INST_ANISO(3, false)
+QSR_INST_ANISO(3, true)
+QSR_INST_ANISO(4, false)
+QSR_INST_ANISO(4, true)
+#undef QSR_INST_ANISO
 
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 namespace qsrk {
 
-hipError_t launch_step(bool full, bool noise, int ntiles, hipStream_t st, const float* qin, float* qout,
+hipError_t launch_step(bool full, int variant, bool noise, int ntiles, hipStream_t st, const float* qin, float* qout,
                        long long cs, const BoxDev* boxes, const int4* tiles, const StepDyn* dyn, int nboxes,
                        const RowMask* rows, int ch) {
     const dim3 grid(ntiles), block(TK * TJ);
+#define QSR_GO(...) hipLaunchKernelGGL(HIP_KERNEL_NAME(__VA_ARGS__), grid, block, 0, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch)
     if (full) {
-        if (noise) hipLaunchKernelGGL(step_full_kernel<true>, grid, block, 0, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch);
-        else hipLaunchKernelGGL(step_full_kernel<false>, grid, block, 0, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch);
+        if (variant == 1) {
+            if (noise) QSR_GO(step_aniso_kernel<TJ, 4, true>);
+            else QSR_GO(step_aniso_kernel<TJ, 4, false>);
+        } else {
+            if (noise) QSR_GO(step_aniso_kernel<TJ, 3, true>);
+            else QSR_GO(step_aniso_kernel<TJ, 3, false>);
+        }
     } else {
-        if (noise) hipLaunchKernelGGL(step_l1_kernel<true>, grid, block, 0, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch);
-        else hipLaunchKernelGGL(step_l1_kernel<false>, grid, block, 0, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch);
+        if (noise) QSR_GO(step_l1_kernel<true>);
+        else QSR_GO(step_l1_kernel<false>);
     }
+#undef QSR_GO
     return hipGetLastError();
 }
 

@@ -72,7 +72,7 @@ EXPORTS = (
     "qsr_create", "qsr_create_batch", "qsr_destroy", "qsr_last_error", "qsr_device_count",
     "qsr_upload_q", "qsr_download_q", "qsr_download_nematic", "qsr_step", "qsr_step_graph",
     "qsr_step_imex", "qsr_observables", "qsr_box_info_get", "qsr_set_iteration", "qsr_last_step_ms",
-    "qsr_step_work", "qsr_profile_steps", "qsr_S_eq", "qsr_dt_estimate", "qsr_init_field",
+    "qsr_step_work", "qsr_profile_steps", "qsr_autotune", "qsr_set_tuning", "qsr_S_eq", "qsr_dt_estimate", "qsr_init_field",
 )
 
 
@@ -100,6 +100,8 @@ def load():
     L.qsr_set_iteration.argtypes = [vp, C.c_int, C.c_int64]
     L.qsr_last_step_ms.argtypes = [vp, dp]
     L.qsr_profile_steps.argtypes = [vp, C.c_int, dp, dp]
+    L.qsr_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), dp]
+    L.qsr_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.qsr_step_work.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.qsr_S_eq.restype = C.c_double
     L.qsr_S_eq.argtypes = [C.c_double] * 5 + [C.c_int]
@@ -260,6 +262,15 @@ class Solver:
         a, b = C.c_double(), C.c_double()
         _check(load().qsr_profile_steps(self._h, int(n), C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def autotune(self, trial_steps: int = 10) -> dict:
+        """Pick the fastest stencil variant / planes-per-tile on the resident field."""
+        v, ch, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(load().qsr_autotune(self._h, int(trial_steps), C.byref(v), C.byref(ch), C.byref(ms)))
+        return {"variant": v.value, "planes_per_tile": ch.value, "ms": ms.value}
+
+    def set_tuning(self, variant: int, planes_per_tile: int):
+        _check(load().qsr_set_tuning(self._h, int(variant), int(planes_per_tile)))
 
     def work(self) -> dict:
         a, t, b = C.c_uint64(), C.c_uint64(), C.c_uint64()

@@ -129,3 +129,17 @@ def test_graph_steps_equal_plain_steps(qsr, orc):
             b.step_graph(12, chunk=4)
         b.step(1)
         assert np.array_equal(a.download(), b.download())
+
+
+@pytest.mark.parametrize("name", ["l2_l3_weak", "l1_weak_custom"])
+def test_autotune_keeps_state_and_results(qsr, orc, name):
+    p, od, Q0, S_shell = _case(qsr, orc, **CASES[name])
+    with qsr.Solver(p) as a, qsr.Solver(p) as b:
+        a.upload(Q0)
+        b.upload(Q0)
+        tune = a.autotune(2)
+        print(f"  autotune {tune}")
+        assert np.array_equal(a.download(), Q0)
+        a.step(5)
+        b.step(5)
+        assert np.array_equal(a.download(), b.download())
