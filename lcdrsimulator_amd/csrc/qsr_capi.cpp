@@ -107,6 +107,12 @@ struct Ctx {
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_chunk = 0, graph_cur = -1;
+    // semi-implicit path
+    float* q_rhs = nullptr;
+    float4* d_imex = nullptr;   // per box alpha/d^2 (RHS mode of the fused kernels)
+    float4* d_jcoef = nullptr;  // per box alpha/d^2 and the Jacobi denominator
+    long long* d_vox_prefix = nullptr;
+    long long nvox_total = 0;
 
     ~Ctx() {
         if (gexec) discard(hipGraphExecDestroy(gexec));
@@ -114,7 +120,7 @@ struct Ctx {
         for (float* p : q) discard(hipFree(p));
         for (void* p : {(void*)d_boxes, (void*)d_tiles, (void*)d_step_rows, (void*)d_obs_rows, (void*)d_face_prefix,
                         (void*)d_dyn, (void*)d_part, (void*)d_out, (void*)d_nf_slice[0], (void*)d_nf_slice[1],
-                        (void*)d_nf3})
+                        (void*)d_nf3, (void*)q_rhs, (void*)d_imex, (void*)d_jcoef, (void*)d_vox_prefix})
             discard(hipFree(p));
         if (h_dyn) discard(hipHostFree(h_dyn));
         for (hipEvent_t e : ev)
@@ -326,6 +332,13 @@ int create_impl(const qsr_params* params, int n, int device, void** out) {
     HIP_TRY(hipMalloc(&c->d_obs_rows, sizeof(qsrk::RowMask) * obs_rows.size()));
     HIP_TRY(hipMemcpy(c->d_step_rows, step_rows.data(), sizeof(qsrk::RowMask) * step_rows.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->d_obs_rows, obs_rows.data(), sizeof(qsrk::RowMask) * obs_rows.size(), hipMemcpyHostToDevice));
+    {
+        std::vector<long long> vp{0};
+        for (const Box& b : c->boxes) vp.push_back(vp.back() + b.nvox);
+        c->nvox_total = vp.back();
+        HIP_TRY(hipMalloc(&c->d_vox_prefix, sizeof(long long) * vp.size()));
+        HIP_TRY(hipMemcpy(c->d_vox_prefix, vp.data(), sizeof(long long) * vp.size(), hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipMalloc(&c->d_face_prefix, sizeof(long long) * face_prefix.size()));
     HIP_TRY(hipMemcpy(c->d_face_prefix, face_prefix.data(), sizeof(long long) * face_prefix.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&c->d_dyn, sizeof(qsrk::StepDyn) * kDynChunk * n));
@@ -361,7 +374,7 @@ int enqueue_steps(Ctx& c, int nsteps) {
         const qsrk::StepDyn* dyn = c.d_dyn + (size_t)s * nb;
         if (c.ntiles > 0)
             HIP_TRY(qsrk::launch_step(c.full, c.variant, c.noise, c.ntiles, c.st, qin, qout, c.cs, c.d_boxes, c.d_tiles, dyn, nb,
-                                      c.d_step_rows, c.ch));
+                                      c.d_step_rows, c.ch, nullptr));
         HIP_TRY(qsrk::launch_faces(c.nface, c.st, qin, qout, c.cs, c.d_boxes, dyn, c.d_face_prefix, nb, c.d_step_rows));
         c.cur = 1 - c.cur;
     }
@@ -557,7 +570,7 @@ int qsr_profile_steps(void* ctx, int nsteps, double* main_ms, double* face_ms) {
         HIP_TRY(hipEventRecord(ev[3 * s], c->st));
         if (c->ntiles > 0)
             HIP_TRY(qsrk::launch_step(c->full, c->variant, c->noise, c->ntiles, c->st, qin, qout, c->cs, c->d_boxes, c->d_tiles,
-                                      dyn, nb, c->d_step_rows, c->ch));
+                                      dyn, nb, c->d_step_rows, c->ch, nullptr));
         HIP_TRY(hipEventRecord(ev[3 * s + 1], c->st));
         HIP_TRY(qsrk::launch_faces(c->nface, c->st, qin, qout, c->cs, c->d_boxes, dyn, c->d_face_prefix, nb,
                                    c->d_step_rows));
@@ -605,7 +618,7 @@ int qsr_autotune(void* ctx, int trial_steps, int* best_variant, int* best_ch, do
                 if (c->ntiles > 0)
                     HIP_TRY(qsrk::launch_step(c->full, c->variant, c->noise, c->ntiles, c->st, c->q[c->cur],
                                               c->q[1 - c->cur], c->cs, c->d_boxes, c->d_tiles, c->d_dyn, nb,
-                                              c->d_step_rows, c->ch));
+                                              c->d_step_rows, c->ch, nullptr));
             }
             HIP_TRY(hipEventRecord(c->ev[1], c->st));
             HIP_TRY(hipEventSynchronize(c->ev[1]));
@@ -649,8 +662,60 @@ int qsr_set_tuning(void* ctx, int variant, int planes_per_tile) {
 }
 
 int qsr_step_imex(void* ctx, int nsteps, double L_stab, int jacobi_iters) {
-    (void)ctx; (void)nsteps; (void)L_stab; (void)jacobi_iters;
-    return fail(QSR_E_STATE, "IMEX stepping is not available in this build");
+    Ctx* c = as_ctx(ctx);
+    if (!c || nsteps < 0 || jacobi_iters < 0 || !std::isfinite(L_stab)) return fail(QSR_E_ARG, "bad argument");
+    // the reference falls back to the explicit update without a stabiliser (QSR.cu:2721, 2750-2751)
+    if (!(L_stab > 0.0) || jacobi_iters == 0) return qsr_step(ctx, nsteps);
+    if (int rc = check_uploaded(*c)) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    if (nsteps == 0) return QSR_OK;
+    const int nb = int(c->boxes.size());
+    if (!c->q_rhs) {
+        HIP_TRY(hipMalloc(&c->q_rhs, sizeof(float) * 5 * c->cs));
+        HIP_TRY(hipMalloc(&c->d_imex, sizeof(float4) * nb));
+        HIP_TRY(hipMalloc(&c->d_jcoef, sizeof(float4) * nb));
+    }
+    std::vector<float4> im(nb), jc(nb);
+    for (int b = 0; b < nb; ++b) {
+        const qsr_params& p = c->boxes[b].p;
+        const double alpha = (p.dt / p.gamma) * L_stab;  // QSR.cu:766, 2729
+        const double ax = alpha / (p.dx * p.dx), ay = alpha / (p.dy * p.dy), az = alpha / (p.dz * p.dz);
+        im[b] = make_float4(float(ax), float(ay), float(az), 1.0f);
+        jc[b] = make_float4(float(ax), float(ay), float(az), float(1.0 + 2.0 * (ax + ay + az)));
+    }
+    HIP_TRY(hipStreamSynchronize(c->st));
+    HIP_TRY(hipMemcpy(c->d_imex, im.data(), sizeof(float4) * nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_jcoef, jc.data(), sizeof(float4) * nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipEventRecord(c->ev[0], c->st));
+    int done = 0;
+    while (done < nsteps) {
+        const int n = std::min(kDynChunk, nsteps - done);
+        HIP_TRY(hipStreamSynchronize(c->st));
+        if (int rc = fill_dyn(*c, n)) return rc;
+        for (int s = 0; s < n; ++s) {
+            const qsrk::StepDyn* dyn = c->d_dyn + (size_t)s * nb;
+            if (c->ntiles > 0)
+                HIP_TRY(qsrk::launch_step(c->full, c->variant, false, c->ntiles, c->st, c->q[c->cur], c->q_rhs, c->cs,
+                                          c->d_boxes, c->d_tiles, dyn, nb, c->d_step_rows, c->ch, c->d_imex));
+            int src = c->cur;
+            for (int it = 0; it < jacobi_iters; ++it) {
+                HIP_TRY(qsrk::launch_jacobi(c->nvox_total, c->st, c->q_rhs, c->q[src], c->q[1 - src], c->cs, c->d_boxes,
+                                            c->d_vox_prefix, nb, c->d_step_rows, c->d_jcoef));
+                src = 1 - src;
+            }
+            c->cur = src;
+            HIP_TRY(qsrk::launch_bc(c->nvox_total, c->st, c->q[c->cur], c->cs, c->d_boxes, dyn, c->d_vox_prefix, nb,
+                                    c->d_step_rows));
+        }
+        advance_iters(*c, n);
+        done += n;
+    }
+    HIP_TRY(hipEventRecord(c->ev[1], c->st));
+    HIP_TRY(hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->last_ms = double(ms) / nsteps;
+    return QSR_OK;
 }
 
 int qsr_last_step_ms(void* ctx, double* ms) {

@@ -86,10 +86,15 @@ constexpr int NPLAQ = 3;     // periodic plaquette slots
 namespace qsrk {
 hipError_t launch_step(bool full, int variant, bool noise, int ntiles, hipStream_t st, const float* qin, float* qout,
                        long long cs, const BoxDev* boxes, const int4* tiles, const StepDyn* dyn, int nboxes,
-                       const RowMask* rows, int ch);
+                       const RowMask* rows, int ch, const float4* imex);
 hipError_t launch_faces(long long nface, hipStream_t st, const float* qin, float* qout, long long cs,
                         const BoxDev* boxes, const StepDyn* dyn, const long long* face_prefix, int nboxes,
                         const RowMask* rows);
+hipError_t launch_jacobi(long long nvox, hipStream_t st, const float* rhs, const float* qold, float* qnew, long long cs,
+                         const BoxDev* boxes, const long long* vox_prefix, int nboxes, const RowMask* rows,
+                         const float4* coef);
+hipError_t launch_bc(long long nvox, hipStream_t st, float* q, long long cs, const BoxDev* boxes, const StepDyn* dyn,
+                     const long long* vox_prefix, int nboxes, const RowMask* rows);
 hipError_t launch_aos_to_soa(hipStream_t st, const double* aos, float* q0, float* q1, long long cs, long long off,
                              long long n);
 hipError_t launch_soa_to_aos(hipStream_t st, const float* q, double* aos, long long cs, long long off, long long n);

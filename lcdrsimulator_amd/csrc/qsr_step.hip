@@ -107,9 +107,22 @@ __device__ __forceinline__ void noise5(unsigned long long seed, unsigned long lo
 template <bool NOISE>
 __device__ __forceinline__ bool finish_voxel(const BoxDev& B, const StepDyn& D, int i, int j, int k,
                                              long long local, bool evolve, bool shell, const float q[5],
-                                             float h[5], float out[5]) {
+                                             float h[5], float out[5], bool rhs_only = false) {
     const int fl = B.flags;
     const bool update = evolve && !((fl & F_SKIPSHELL) && shell);
+    if (rhs_only) {
+        // semi-implicit right-hand side (QSR.cu:734-784): frozen voxels are never read by the sweeps
+        if (!update) return false;
+        float q0a[5];
+        if ((fl & F_ANCHOR) && shell) {
+            shell_target(B, i, j, k, D.S_shell, q0a);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) h[c] += B.cW * (q[c] - q0a[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) out[c] = q[c] - h[c];
+        return true;
+    }
     const bool bc = (fl & F_BC) && shell;
     if (!update && !bc) return false;
     float q0[5];
@@ -168,13 +181,15 @@ __global__ __launch_bounds__(TK* TJ) void step_l1_kernel(const float* __restrict
                                                           long long cs, const BoxDev* __restrict__ boxes,
                                                           const int4* __restrict__ tiles,
                                                           const StepDyn* __restrict__ dyn, int nboxes,
-                                                          const RowMask* __restrict__ rows, int ch) {
+                                                          const RowMask* __restrict__ rows, int ch,
+                                                          const float4* __restrict__ imex_tab) {
     constexpr int W = TK + 2, R = TJ + 2, PL = W * R, SLOT = 5 * PL;
     __shared__ float lds[2 * SLOT];
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;  // padding tile of the XCD-ordered list (uniform over the block)
     const BoxDev B = boxes[t.x];
     const StepDyn D = dyn[t.x];
+    const float4 imex = imex_tab ? imex_tab[t.x] : make_float4(0.f, 0.f, 0.f, 0.f);
     const int tk = threadIdx.x & (TK - 1), tj = threadIdx.x / TK;
     const int Nx = B.Nx, Ny = B.Ny, Nz = B.Nz;
     const bool per = (B.flags & F_PERIODIC) != 0;
@@ -234,13 +249,14 @@ __global__ __launch_bounds__(TK* TJ) void step_l1_kernel(const float* __restrict
                 for (int c = 0; c < 5; ++c) {
                     const float* Lc = L + c * PL;
                     const float q2 = 2.0f * qc[c];
-                    const float lap = B.lx * (qn[c] + qm[c] - q2) + B.ly * (Lc[lrow + W] + Lc[lrow - W] - q2) +
-                                      B.lz * (Lc[lrow + 1] + Lc[lrow - 1] - q2);
-                    h[c] -= lap;
+                    const float sxx = qn[c] + qm[c] - q2, syy = Lc[lrow + W] + Lc[lrow - W] - q2,
+                                szz = Lc[lrow + 1] + Lc[lrow - 1] - q2;
+                    h[c] -= B.lx * sxx + B.ly * syy + B.lz * szz;
+                    if (imex.w != 0.0f) h[c] += imex.x * sxx + imex.y * syy + imex.z * szz;
                 }
                 float o[5];
                 const long long local = (long long)p * plane + (long long)j * Nz + k;
-                if (finish_voxel<NOISE>(B, D, p, j, k, local, evolve, shell, qc, h, o)) {
+                if (finish_voxel<NOISE>(B, D, p, j, k, local, evolve, shell, qc, h, o, imex.w != 0.0f)) {
 #pragma unroll
                     for (int c = 0; c < 5; ++c) __builtin_nontemporal_store(o[c], obase + c * cs + local);
                 }
@@ -289,7 +305,8 @@ __global__ __launch_bounds__(TK* TJv, MINW) void step_aniso_kernel(const float* 
                                                              long long cs, const BoxDev* __restrict__ boxes,
                                                              const int4* __restrict__ tiles,
                                                              const StepDyn* __restrict__ dyn, int nboxes,
-                                                             const RowMask* __restrict__ rows, int ch) {
+                                                             const RowMask* __restrict__ rows, int ch,
+                                                          const float4* __restrict__ imex_tab) {
     using G = AnisoGeom<TJv>;
     constexpr int W2 = G::W2, P2 = G::P2, QSLOT = G::QSLOT, W1 = G::W1, P1 = G::P1, DSLOT = G::DSLOT;
     __shared__ float qs[2 * QSLOT];
@@ -298,6 +315,7 @@ __global__ __launch_bounds__(TK* TJv, MINW) void step_aniso_kernel(const float* 
     if (t4.x < 0) return;  // padding tile of the XCD-ordered list (uniform over the block)
     const BoxDev B = boxes[t4.x];
     const StepDyn D = dyn[t4.x];
+    const float4 imex = imex_tab ? imex_tab[t4.x] : make_float4(0.f, 0.f, 0.f, 0.f);
     const int tid = threadIdx.x, tk = tid & (TK - 1), tj = tid / TK;
     const int Nx = B.Nx, Ny = B.Ny, Nz = B.Nz;
     const bool per = (B.flags & F_PERIODIC) != 0;
@@ -546,9 +564,13 @@ __global__ __launch_bounds__(TK* TJv, MINW) void step_aniso_kernel(const float* 
                     h[2] += hl * M[0][2] - B.mL3 * Bc[2];
                     h[4] += hl * M[1][2] - B.mL3 * Bc[4];
                 }
+                if (imex.w != 0.0f) {
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) h[c] += imex.x * sx_[c] + imex.y * syy1[c] + imex.z * szz1[c];
+                }
                 float o[5];
                 const long long local = (long long)t * plane + (long long)j * Nz + k;
-                if (finish_voxel<NOISE>(B, D, t, j, k, local, evolve, shell, q1, h, o)) {
+                if (finish_voxel<NOISE>(B, D, t, j, k, local, evolve, shell, q1, h, o, imex.w != 0.0f)) {
 #pragma unroll
                     for (int c = 0; c < 5; ++c) __builtin_nontemporal_store(o[c], obase + c * cs + local);
                 }
@@ -714,6 +736,88 @@ __global__ __launch_bounds__(256) void step_face_kernel(const float* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Semi-implicit Helmholtz sweep and strong boundary condition (IMEX path)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int box_of(long long g, const long long* prefix, int nboxes) {
+    int b = 0;
+    while (b + 1 < nboxes && prefix[b + 1] <= g) ++b;
+    return b;
+}
+
+// One Jacobi sweep of (I - alpha lap) Q_new = RHS (QSR.cu:786-849): box faces,
+// frozen strong shells and non-evolving voxels keep Q_old.  Voxels that never
+// change are skipped (both buffers hold them).  coef[b] = (alpha/dx^2,
+// alpha/dy^2, alpha/dz^2, 1 + 2 alpha sum 1/d^2).
+__global__ __launch_bounds__(256) void jacobi_kernel(const float* __restrict__ rhs, const float* __restrict__ qold,
+                                                     float* __restrict__ qnew, long long cs,
+                                                     const BoxDev* __restrict__ boxes,
+                                                     const long long* __restrict__ vox_prefix, int nboxes,
+                                                     const RowMask* __restrict__ rows,
+                                                     const float4* __restrict__ coef) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= vox_prefix[nboxes]) return;
+    const int b = box_of(g, vox_prefix, nboxes);
+    const BoxDev B = boxes[b];
+    const long long v = g - vox_prefix[b];
+    const int Nx = B.Nx, Ny = B.Ny, Nz = B.Nz;
+    const int k = int(v % Nz), j = int((v / Nz) % Ny), i = int(v / ((long long)Ny * Nz));
+    const bool per = (B.flags & F_PERIODIC) != 0;
+    bool evolve, shell;
+    decode_mask(rows, B, i, j, k, evolve, shell);
+    const bool face = !per && (i == 0 || j == 0 || k == 0 || i == Nx - 1 || j == Ny - 1 || k == Nz - 1);
+    const bool frozen = face || ((B.flags & F_SKIPSHELL) && shell) || !evolve;
+    const float4 cf = coef[b];
+    const float* qo = qold + B.off;
+    float* qn = qnew + B.off;
+    if (frozen || !(cf.w > 0.0f)) {
+        if (evolve || ((B.flags & F_BC) && shell)) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) qn[c * cs + v] = qo[c * cs + v];
+        }
+        return;
+    }
+    const long long plane = (long long)Ny * Nz;
+    auto at = [&](int ii, int jj, int kk) {
+        if (per) {
+            ii = (ii + Nx) % Nx;
+            jj = (jj + Ny) % Ny;
+            kk = (kk + Nz) % Nz;
+        }
+        return (long long)ii * plane + (long long)jj * Nz + kk;
+    };
+    const long long xp = at(i + 1, j, k), xm = at(i - 1, j, k), yp = at(i, j + 1, k), ym = at(i, j - 1, k);
+    const long long zp = at(i, j, k + 1), zm = at(i, j, k - 1);
+    const float* r = rhs + B.off;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        const float* Q = qo + c * cs;
+        qn[c * cs + v] = (r[c * cs + v] + cf.x * (Q[xp] + Q[xm]) + cf.y * (Q[yp] + Q[ym]) + cf.z * (Q[zp] + Q[zm])) / cf.w;
+    }
+}
+
+// Strong anchoring re-imposed on the shell after the sweeps (QSR.cu:851-867)
+__global__ __launch_bounds__(256) void bc_kernel(float* __restrict__ q, long long cs, const BoxDev* __restrict__ boxes,
+                                                 const StepDyn* __restrict__ dyn,
+                                                 const long long* __restrict__ vox_prefix, int nboxes,
+                                                 const RowMask* __restrict__ rows) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= vox_prefix[nboxes]) return;
+    const int b = box_of(g, vox_prefix, nboxes);
+    const BoxDev B = boxes[b];
+    if (!(B.flags & F_BC) || (B.flags & F_PERIODIC)) return;
+    const long long v = g - vox_prefix[b];
+    const int Nz = B.Nz, Ny = B.Ny;
+    const int k = int(v % Nz), j = int((v / Nz) % Ny), i = int(v / ((long long)Ny * Nz));
+    bool evolve, shell;
+    decode_mask(rows, B, i, j, k, evolve, shell);
+    if (!shell) return;
+    float q0[5];
+    shell_target(B, i, j, k, dyn[b].S_shell, q0);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) q[c * cs + B.off + v] = q0[c];
+}
+
+// ---------------------------------------------------------------------------
 // Interchange-format conversion (AoS double <-> SoA float)
 // ---------------------------------------------------------------------------
 __global__ void aos_to_soa_kernel(const double* __restrict__ aos, float* __restrict__ q0, float* __restrict__ q1,
@@ -738,12 +842,12 @@ __global__ void soa_to_aos_kernel(const float* __restrict__ q, double* __restric
 
 // Explicit instantiations used by the context code.
 template __global__ void step_l1_kernel<false>(const float*, float*, long long, const BoxDev*, const int4*,
-                                               const StepDyn*, int, const RowMask*, int);
+                                               const StepDyn*, int, const RowMask*, int, const float4*);
 template __global__ void step_l1_kernel<true>(const float*, float*, long long, const BoxDev*, const int4*,
-                                              const StepDyn*, int, const RowMask*, int);
+                                              const StepDyn*, int, const RowMask*, int, const float4*);
 #define QSR_INST_ANISO(MINW, NZ)                                                                         \
     template __global__ void step_aniso_kernel<TJ, MINW, NZ>(const float*, float*, long long, const BoxDev*, \
-                                                             const int4*, const StepDyn*, int, const RowMask*, int);
+                                                             const int4*, const StepDyn*, int, const RowMask*, int, const float4*);
 QSR_INST_ANISO(3, false)
 QSR_INST_ANISO(3, true)
 QSR_INST_ANISO(4, false)
@@ -757,9 +861,9 @@ namespace qsrk {
 
 hipError_t launch_step(bool full, int variant, bool noise, int ntiles, hipStream_t st, const float* qin, float* qout,
                        long long cs, const BoxDev* boxes, const int4* tiles, const StepDyn* dyn, int nboxes,
-                       const RowMask* rows, int ch) {
+                       const RowMask* rows, int ch, const float4* imex) {
     const dim3 grid(ntiles), block(TK * TJ);
-#define QSR_GO(...) hipLaunchKernelGGL(HIP_KERNEL_NAME(__VA_ARGS__), grid, block, 0, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch)
+#define QSR_GO(...) hipLaunchKernelGGL(HIP_KERNEL_NAME(__VA_ARGS__), grid, block, 0, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex)
     if (full) {
         if (variant == 1) {
             if (noise) QSR_GO(step_aniso_kernel<TJ, 4, true>);
@@ -782,6 +886,21 @@ hipError_t launch_faces(long long nface, hipStream_t st, const float* qin, float
     if (nface <= 0) return hipSuccess;
     const dim3 grid(unsigned((nface + 255) / 256)), block(256);
     hipLaunchKernelGGL(step_face_kernel, grid, block, 0, st, qin, qout, cs, boxes, dyn, face_prefix, nboxes, rows);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi(long long nvox, hipStream_t st, const float* rhs, const float* qold, float* qnew, long long cs,
+                         const BoxDev* boxes, const long long* vox_prefix, int nboxes, const RowMask* rows,
+                         const float4* coef) {
+    hipLaunchKernelGGL(jacobi_kernel, dim3(unsigned((nvox + 255) / 256)), dim3(256), 0, st, rhs, qold, qnew, cs, boxes,
+                       vox_prefix, nboxes, rows, coef);
+    return hipGetLastError();
+}
+
+hipError_t launch_bc(long long nvox, hipStream_t st, float* q, long long cs, const BoxDev* boxes, const StepDyn* dyn,
+                     const long long* vox_prefix, int nboxes, const RowMask* rows) {
+    hipLaunchKernelGGL(bc_kernel, dim3(unsigned((nvox + 255) / 256)), dim3(256), 0, st, q, cs, boxes, dyn, vox_prefix,
+                       nboxes, rows);
     return hipGetLastError();
 }
 

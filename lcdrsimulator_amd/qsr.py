@@ -236,6 +236,10 @@ class Solver:
     def step_graph(self, n: int, chunk: int = 64):
         _check(load().qsr_step_graph(self._h, int(n), int(chunk)))
 
+    def step_imex(self, n: int, L_stab: float, jacobi_iters: int = 25):
+        """Semi-implicit steps (explicit RHS + Jacobi sweeps, QSR.cu:2721-2749)."""
+        _check(load().qsr_step_imex(self._h, int(n), float(L_stab), int(jacobi_iters)))
+
     def set_iteration(self, it: int, box: int = 0):
         _check(load().qsr_set_iteration(self._h, box, int(it)))
 
