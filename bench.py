@@ -66,6 +66,26 @@ def workload_params(q, n=256, R=120.0, workload="droplet_l1l2l3_256"):
     return p
 
 
+def measured_traffic(workload, variant, planes):
+    """HBM bytes per fused-stencil launch from the committed counter summaries
+    (FETCH_SIZE and WRITE_SIZE passes, each scaled by a plain-copy calibration),
+    for the same workload and tuning; None if that combination was not profiled."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None, None
+    for fn in sorted(os.listdir(pdir)):
+        if not fn.endswith(".md"):
+            continue
+        for line in open(os.path.join(pdir, fn)):
+            if not line.startswith("bench-traffic:"):
+                continue
+            f = line.split()
+            kv = dict(x.split("=", 1) for x in f[2:])
+            if f[1] == workload and int(kv["variant"]) == variant and int(kv["planes"]) == planes:
+                return float(kv["bytes"]), "profiles/" + fn
+    return None, None
+
+
 def cpu_baseline(p, budget_s=15.0):
     """Reference QSR.cpp step functions (oracle/_ref) on the same 256^3 box."""
     import ctypes as C
@@ -183,6 +203,7 @@ def main():
     value = work["box_voxels"] * args.steps / wall
     traffic_bytes = BYTES_PER_VOXEL * work["tile_voxels"]
     achieved = traffic_bytes / (main_ms * 1e-3) / 1e9
+    hbm_bytes, hbm_src = measured_traffic(args.workload, int(tune["variant"]), int(tune["planes_per_tile"]))
     cpu = None if (args.no_cpu or args.workload != WORKLOADS[0]) else cpu_baseline(p)
     out = {
         "metric": METRIC, "value": value, "unit": "voxel-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -194,9 +215,10 @@ def main():
                    "tile_voxels": work["tile_voxels"], "graph_chunk": args.graph, "autotune": tune,
                    "droplet_voxel_updates_per_s": work["active_voxels"] * args.steps / wall,
                    "fused_stencil_ms": main_ms, "face_ms": face_ms, "event_ms_per_step": ev_ms,
-                   "final_total_F": obs["total"]},
+                   "final_total_F": obs["total"], "algorithmic_bytes_per_launch": traffic_bytes,
+                   "traffic_source": hbm_src},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": hbm_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -372,3 +372,130 @@ def test_jacobi_solves_helmholtz():
     inner[3:-3, 3:-3, 3:-3] = True
     res = (Qn - alpha * lap - rhs)[inner]
     assert np.abs(res).max() < 1e-10
+
+
+# ---- anchoring, log-point diagnostics, limiter, periodic energies -------------------------------
+
+def _dp(a):
+    import ctypes as C
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def test_anchoring_penalty_is_anchoring_energy_gradient():
+    """mu_anch = (W/delta)(Q - Q0) (QSR.cu:623-652) is dF_anch/dQ of F_anch = W/(2 delta)|Q - Q0|_F^2 dV
+    (QSR.cu:654-695); F_anch vanishes at Q = S_shell (nu nu - I/3) with nu = r_hat on a sphere."""
+    import ctypes as C
+    g = O.Grid(16, 16, 16)
+    W, S_shell = 3e-4, 0.55
+    d = O.DropletOracle(g, O.phys(W=W))
+    rng = np.random.default_rng(11)
+    Q = rng.normal(0, 0.2, g.dims + (5,))
+    mu = np.zeros_like(Q)
+    O.lib().orc_weak_anchor(_dp(mu), _dp(Q), *g.dims, d.shell.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(d.g),
+                            C.c_double(S_shell), C.c_double(W), C.c_double(d.thickness))
+    assert np.all(mu[~d.shell.astype(bool)] == 0)
+    dV = 1e-24
+    shell_pts = list(zip(*np.nonzero(d.shell)))[::37][:6]
+    for (i, j, k) in shell_pts:
+        for c in range(5):
+            Qp, Qm = Q.copy(), Q.copy()
+            Qp[i, j, k, c] += 1e-4
+            Qm[i, j, k, c] -= 1e-4
+            dF = (d.energy(Qp, S_shell)[2] - d.energy(Qm, S_shell)[2]) / 2e-4 / dV
+            h = mu[i, j, k]
+            hzz = -(h[0] + h[3])
+            expect = {0: h[0] - hzz, 3: h[3] - hzz, 1: 2 * h[1], 2: 2 * h[2], 4: 2 * h[4]}[c]
+            assert dF == pytest.approx(expect, rel=1e-7, abs=1e-9 * np.abs(mu).max())
+    idx = np.indices(g.dims).transpose(1, 2, 3, 0) - 7.5
+    n = idx / np.linalg.norm(idx, axis=-1, keepdims=True)
+    Q0 = S_shell * np.stack([n[..., 0] ** 2 - 1 / 3, n[..., 0] * n[..., 1], n[..., 0] * n[..., 2],
+                             n[..., 1] ** 2 - 1 / 3, n[..., 1] * n[..., 2]], -1)
+    assert abs(d.energy(Q0, S_shell)[2]) < 1e-12 * abs(d.energy(Q, S_shell)[2])
+
+
+def _uniform_S_field(dims, S_of_ij, n=(0.48, 0.6, 0.64)):
+    n = np.asarray(n) / np.linalg.norm(n)
+    M = np.outer(n, n) - np.eye(3) / 3
+    base = np.array([M[0, 0], M[0, 1], M[0, 2], M[1, 1], M[1, 2]])
+    I, J = np.meshgrid(np.arange(dims[0]), np.arange(dims[1]), indexing="ij")
+    S = S_of_ij(I, J)
+    return np.broadcast_to((S[..., None] * base)[:, :, None, :], dims + (5,)).copy()
+
+
+def test_avg_max_S_and_checkerboard_closed_forms():
+    """avg/max S over sd < -2 min_spacing (QSR.cu:2285-2321) and the parity projection
+    |sum_even - sum_odd| / sum_all on the slice, evaluated only once max_S >= 0.5 (QSR.cu:2829-2874)."""
+    import ctypes as C
+    dims = (22, 22, 22)
+    g = O.Grid(*dims)
+    d = O.DropletOracle(g, O.phys())
+    idx = np.indices(dims).transpose(1, 2, 3, 0) - 10.5
+    interior = np.linalg.norm(idx, axis=-1) - 10.5 < -2.0  # sphere: sd = r - R (voxel units)
+    Se, So = 0.72, 0.52
+    Q = _uniform_S_field(dims, lambda I, J: np.where((I + J) % 2 == 0, Se, So))
+    am = (C.c_double * 2)()
+    O.lib().orc_avg_max_S(_dp(Q), *dims, C.byref(d.g), am)
+    par = (np.indices(dims)[0] + np.indices(dims)[1]) % 2 == 0
+    ne, no = (interior & par).sum(), (interior & ~par).sum()
+    assert am[0] == pytest.approx((ne * Se + no * So) / (ne + no), rel=1e-6)
+    assert am[1] == pytest.approx(Se, rel=1e-6)
+    z = 11
+    sl = interior[:, :, z]
+    pe, po = (sl & par[:, :, z]).sum(), (sl & ~par[:, :, z]).sum()
+    want = abs(pe * Se - po * So) / (pe * Se + po * So)
+    assert O.lib().orc_checker(_dp(Q), *dims, C.byref(d.g), z, C.c_double(am[1])) == pytest.approx(want, rel=1e-6)
+    low = _uniform_S_field(dims, lambda I, J: np.full(I.shape, 0.3))
+    assert math.isnan(O.lib().orc_checker(_dp(low), *dims, C.byref(d.g), z, C.c_double(0.3)))
+
+
+def test_q_norm_cap_limits_frobenius_norm():
+    """The optional limiter rescales Q to |Q|_F = cap where the Euler update exceeds it (QSR.cu:721-731)."""
+    g = O.Grid(16, 16, 16)
+    p = O.phys(T=300.0, T_star=307.0, kappa=0.0, L1=6e-12)
+    cap = 0.8 * math.sqrt(2 / 3)
+    free = O.DropletOracle(g, p)
+    capped = O.DropletOracle(g, p, q_norm_cap=cap)
+    Q = free.init_director(0.95, 0, 9)
+    S_shell = 0.9
+    a, b = free.step(Q, S_shell, 1.5e-10), capped.step(Q, S_shell, 1.5e-10)
+    full = lambda q: np.sqrt(q[..., 0] ** 2 + q[..., 3] ** 2 + (q[..., 0] + q[..., 3]) ** 2
+                             + 2 * (q[..., 1] ** 2 + q[..., 2] ** 2 + q[..., 4] ** 2))
+    over = (full(a) > cap) & (np.abs(a - Q).sum(-1) > 0) & ~free.shell.astype(bool)
+    assert over.sum() > 50
+    assert np.allclose(full(b)[over], cap, rtol=1e-12)
+    assert np.allclose(b[over], a[over] * (cap / full(a)[over])[:, None], rtol=1e-12)
+    assert np.array_equal(b[~over & ~free.shell.astype(bool)], a[~over & ~free.shell.astype(bool)])
+
+
+def test_periodic_energy_and_xi_plane_wave():
+    """Q_xy = a sin(2 pi x / N): forward-difference elastic density 2 kappa a^2 sin^2(pi/N)/dx^2, bulk
+    density A a^2/2 + 3 C a^4/8 (tr Q^3 = 0), xi = dx / (2 sin(pi/N)) (KZM_bulk_ldg.cu:543-711)."""
+    N, amp, kappa, dx = 16, 0.3, 6e-12, 1e-8
+    dims = (N, 5, 6)
+    Q = np.zeros(dims + (5,))
+    Q[..., 1] = amp * np.sin(2 * np.pi * np.arange(N) / N)[:, None, None]
+    A, B, Cc = O.bulk_ABC(A5, B5, C5, 300.0, 307.0, 1)
+    out = O.periodic_observables(Q, O.Grid(*dims, dx, dx, dx), A, B, Cc, kappa)
+    assert out["elastic"] == pytest.approx(2 * kappa * amp ** 2 * math.sin(math.pi / N) ** 2 / dx ** 2, rel=1e-12)
+    assert out["bulk"] == pytest.approx(A * amp ** 2 / 2 + 3 * Cc * amp ** 4 / 8, rel=1e-12)
+    assert out["xi_grad_proxy"] == pytest.approx(dx / (2 * math.sin(math.pi / N)), rel=1e-12)
+
+
+def test_xi_proxy_linear_twist():
+    """In-plane director angle theta = c i: every x edge has wrapped d(2 theta) = 2c, y edges 0; edges with
+    S <= threshold are dropped (QSR.cu:1036-1106); xi = 1/sqrt(mean d^2), capped at min(Nx, Ny)/2 (QSR.cu:2401-2407)."""
+    import ctypes as C
+    dims, c, S, z = (24, 24, 5), 0.08, 0.6, 2
+    I, J = np.meshgrid(np.arange(dims[0]), np.arange(dims[1]), indexing="ij")
+    th = c * I
+    n = np.stack([np.cos(th), np.sin(th), np.zeros_like(th)], -1)
+    Sij = np.where(J < 3, 0.05, S)
+    Q2 = np.stack([Sij * (n[..., 0] ** 2 - 1 / 3), Sij * n[..., 0] * n[..., 1], Sij * n[..., 0] * n[..., 2],
+                   Sij * (n[..., 1] ** 2 - 1 / 3), Sij * n[..., 1] * n[..., 2]], -1)
+    Q = np.broadcast_to(Q2[:, :, None, :], dims + (5,)).copy()
+    xi, ne = C.c_double(), C.c_uint()
+    O.lib().orc_xi2d(_dp(Q), *dims, z, C.c_double(0.1), C.byref(xi), C.byref(ne))
+    nx_edges, ny_edges = (dims[0] - 1) * (dims[1] - 3), dims[0] * (dims[1] - 4)
+    assert ne.value == nx_edges + ny_edges
+    mean_sq = (2 * c) ** 2 * nx_edges / (nx_edges + ny_edges)
+    assert xi.value == pytest.approx(1 / math.sqrt(mean_sq), rel=1e-5)
