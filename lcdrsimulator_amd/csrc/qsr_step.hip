@@ -31,6 +31,59 @@ __device__ __forceinline__ int wrap_or_clamp(int x, int n, bool periodic) {
 
 __device__ __forceinline__ bool in_iv(int k, short lo, short hi) { return k >= lo && k <= hi; }
 
+// Workgroup barrier that orders LDS only.  The step kernels never exchange data
+// through global memory inside a launch (they read one buffer and write the
+// other), so the barrier need not drain outstanding global loads: prefetches of
+// the next planes stay in flight across it (a full __syncthreads() waits for
+// vmcnt(0) first).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Field access for the stencils.  The context's 5 SoA planes are addressed as
+// base(box) + component * cs + plane offset + in-plane offset.  BUF: raw buffer
+// loads/stores from the box base, the byte offset split into a per-lane part
+// (in-plane position, fixed for a thread) and a wave-uniform part (component and
+// i-plane), so an access costs no vector address arithmetic; valid when every
+// offset of the context fits 32 bits (20 B x cs < 4 GiB).  Otherwise flat
+// 64-bit addressing.  Stores are non-temporal in both.
+template <bool BUF>
+struct Field;
+
+template <>
+struct Field<false> {
+    const float* in;
+    float* out;
+    long long cs;
+    __device__ Field(const float* qin, float* qout, long long cs_, long long off)
+        : in(qin + off), out(qout + off), cs(cs_) {}
+    __device__ float ld(int c, long long pbase, long long o) const { return in[c * cs + pbase + o]; }
+    __device__ void st(int c, long long pbase, long long o, float v) const {
+        __builtin_nontemporal_store(v, out + c * cs + pbase + o);
+    }
+};
+
+template <>
+struct Field<true> {
+    static constexpr int kDword3 = 0x00020000;  // raw 32-bit access, gfx9 buffer descriptor
+    __amdgpu_buffer_rsrc_t rin, rout;
+    unsigned cs4;
+    __device__ Field(const float* qin, float* qout, long long cs_, long long off)
+        : rin(__builtin_amdgcn_make_buffer_rsrc((void*)(qin + off), 0, -1, kDword3)),
+          rout(__builtin_amdgcn_make_buffer_rsrc((void*)(qout + off), 0, -1, kDword3)),
+          cs4(unsigned(cs_) * 4u) {}
+    __device__ float ld(int c, long long pbase, long long o) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rin, unsigned(o) * 4u,
+                                                                              unsigned(c) * cs4 + unsigned(pbase) * 4u, 0));
+    }
+    __device__ void st(int c, long long pbase, long long o, float v) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rout, unsigned(o) * 4u,
+                                              unsigned(c) * cs4 + unsigned(pbase) * 4u, 2 /* nt */);
+    }
+};
+
 // h = m * [A Q - B (Q^2 - tr(Q^2)/3 I) + C tr(Q^2) Q]   (QSR.cu:399-415)
 __device__ __forceinline__ void bulk_field(const float q[5], float cA, float cB, float cC, float h[5]) {
     const float xx = q[0], xy = q[1], xz = q[2], yy = q[3], yz = q[4], zz = -(xx + yy);
@@ -81,24 +134,29 @@ __device__ __forceinline__ void philox10(unsigned c[4], unsigned k0, unsigned k1
     }
 }
 
-// Five N(0,1) variates keyed by (seed, voxel, iteration): uniforms with 23
-// random bits, ((x >> 9) + 0.5) * 2^-23, exact in binary32; Box-Muller pairs.
+// Five N(0,1) variates keyed by (seed, voxel, iteration): one Philox4x32-10
+// block with counter (voxel lo, voxel hi, iteration, 0) and key (seed lo,
+// seed hi); its 128 bits are cut into six 21-bit fields f_t = bits
+// [21t, 21t + 21), turned into uniforms u_t = (f_t + 1/2) 2^-21 (exact in
+// binary32), and paired by Box-Muller: sqrt(-2 ln u_2p) (cos, sin)(2 pi u_2p+1).
+// The transform uses the hardware log2, sqrt and revolution-input sin/cos.
+__device__ __forceinline__ unsigned field21(unsigned lo, unsigned hi, int shift) {
+    return unsigned(((((unsigned long long)hi) << 32) | lo) >> shift) & 0x1FFFFFu;
+}
+
 __device__ __forceinline__ void noise5(unsigned long long seed, unsigned long long voxel, unsigned iter, float g[5]) {
-    unsigned a[4] = {unsigned(voxel), unsigned(voxel >> 32), iter, 0u};
-    unsigned b[4] = {unsigned(voxel), unsigned(voxel >> 32), iter, 1u};
-    philox10(a, unsigned(seed), unsigned(seed >> 32));
-    philox10(b, unsigned(seed), unsigned(seed >> 32));
-    const unsigned r[6] = {a[0], a[1], a[2], a[3], b[0], b[1]};
-    float u[6];
-#pragma unroll
-    for (int t = 0; t < 6; ++t) u[t] = (float(r[t] >> 9) + 0.5f) * (1.0f / 8388608.0f);
+    unsigned r[4] = {unsigned(voxel), unsigned(voxel >> 32), iter, 0u};
+    philox10(r, unsigned(seed), unsigned(seed >> 32));
+    const unsigned f[6] = {r[0] & 0x1FFFFFu,      field21(r[0], r[1], 21), (r[1] >> 10) & 0x1FFFFFu,
+                           field21(r[1], r[2], 31), field21(r[2], r[3], 20), (r[3] >> 9) & 0x1FFFFFu};
+    constexpr float kScale = 1.0f / 2097152.0f;       // 2^-21
+    constexpr float kM2Ln2 = -1.3862943611198906f;    // -2 ln 2
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
-        const float rad = sqrtf(-2.0f * logf(u[2 * p]));
-        float s, c;
-        sincosf(6.283185307179586f * u[2 * p + 1], &s, &c);
-        g[2 * p] = rad * c;
-        if (2 * p + 1 < 5) g[2 * p + 1] = rad * s;
+        const float u0 = (float(f[2 * p]) + 0.5f) * kScale, u1 = (float(f[2 * p + 1]) + 0.5f) * kScale;
+        const float rad = __builtin_amdgcn_sqrtf(kM2Ln2 * __builtin_amdgcn_logf(u0));
+        g[2 * p] = rad * __builtin_amdgcn_cosf(u1);
+        if (2 * p + 1 < 5) g[2 * p + 1] = rad * __builtin_amdgcn_sinf(u1);
     }
 }
 
@@ -159,16 +217,41 @@ __device__ __forceinline__ bool finish_voxel(const BoxDev& B, const StepDyn& D, 
     return true;
 }
 
-__device__ __forceinline__ void decode_mask(const RowMask* rows, const BoxDev& B, int i, int j, int k,
-                                            bool& evolve, bool& shell) {
+__device__ __forceinline__ void decode_row(const RowMask& m, const BoxDev& B, int k, bool& evolve, bool& shell) {
     if (B.flags & F_PERIODIC) {
         evolve = true;
         shell = false;
         return;
     }
-    const RowMask m = rows[B.rows_off + (long long)i * B.Ny + j];
     evolve = in_iv(k, m.v[0], m.v[1]);
     shell = in_iv(k, m.v[2], m.v[3]) && !in_iv(k, m.v[4], m.v[5]);
+}
+
+__device__ __forceinline__ void decode_mask(const RowMask* rows, const BoxDev& B, int i, int j, int k,
+                                            bool& evolve, bool& shell) {
+    RowMask m = {};
+    if (!(B.flags & F_PERIODIC)) m = rows[B.rows_off + (long long)i * B.Ny + j];
+    decode_row(m, B, k, evolve, shell);
+}
+
+// Row-mask record of row (i, j) as raw 16 bytes, for prefetching a plane ahead:
+// kept packed so the conditional load has no consumer before the loop moves on
+// (unpacking right after the load would force a wait for it, and with it for
+// every prefetch issued earlier).  Callers only pass rows inside a clamped box.
+__device__ __forceinline__ int4 load_row4(const RowMask* rows, const BoxDev& B, int i, int j) {
+    return *reinterpret_cast<const int4*>(rows + B.rows_off + (long long)i * B.Ny + j);
+}
+
+__device__ __forceinline__ void decode_row4(int4 m, const BoxDev& B, int k, bool& evolve, bool& shell) {
+    if (B.flags & F_PERIODIC) {
+        evolve = true;
+        shell = false;
+        return;
+    }
+    const int e0 = (short)(m.x & 0xffff), e1 = m.x >> 16, o0 = (short)(m.y & 0xffff), o1 = m.y >> 16;
+    const int n0 = (short)(m.z & 0xffff), n1 = m.z >> 16;
+    evolve = k >= e0 && k <= e1;
+    shell = (k >= o0 && k <= o1) && !(k >= n0 && k <= n1);
 }
 
 }  // namespace
@@ -176,7 +259,7 @@ __device__ __forceinline__ void decode_mask(const RowMask* rows, const BoxDev& B
 // ---------------------------------------------------------------------------
 // L1 / kappa kernel: x-register queue + one LDS plane (halo 1), double-buffered.
 // ---------------------------------------------------------------------------
-template <bool NOISE>
+template <bool NOISE, bool BUF>
 __global__ __launch_bounds__(TK* TJ) void step_l1_kernel(const float* __restrict__ qin, float* __restrict__ qout,
                                                           long long cs, const BoxDev* __restrict__ boxes,
                                                           const int4* __restrict__ tiles,
@@ -208,28 +291,40 @@ __global__ __launch_bounds__(TK* TJ) void step_l1_kernel(const float* __restrict
     const int kh = wrap_or_clamp(tk == 0 ? t.w - 1 : t.w + TK, Nz, per);
     const long long plane = (long long)Ny * Nz;
     const long long o_own = (long long)jc * Nz + kc, o_row = (long long)jh * Nz + kc, o_col = (long long)jc * Nz + kh;
-    const float* base = qin + B.off;
-    float* obase = qout + B.off;
+    const Field<BUF> F(qin, qout, cs, B.off);
 
     auto ld5 = [&](int p, long long o, float v[5]) {
-        const float* s = base + (long long)p * plane + o;
+        const long long pb = (long long)p * plane;
 #pragma unroll
-        for (int c = 0; c < 5; ++c) v[c] = s[c * cs];
+        for (int c = 0; c < 5; ++c) v[c] = F.ld(c, pb, o);
     };
+    // own queue Q(p-1), Q(p), Q(p+1) plus Q(p+2) in flight: every global load is
+    // consumed one whole iteration after it is issued
     float qm[5], qc[5], qn[5], hr[5] = {}, hcol[5] = {}, hrn[5] = {}, hcoln[5] = {};
     ld5(wrap_or_clamp(ib - 1, Nx, per), o_own, qm);
     ld5(ib, o_own, qc);
+    ld5(wrap_or_clamp(ib + 1, Nx, per), o_own, qn);
     if (rowh) ld5(ib, o_row, hr);
     if (colh) ld5(ib, o_col, hcol);
     const int lrow = (tj + 1) * W + tk + 1;
     const int hrow = (tj == 0 ? 0 : (TJ + 1)) * W + tk + 1;
     const int hcolx = (tj + 1) * W + (tk == 0 ? 0 : TK + 1);
 
+    const bool pre_rows = out_ok && !per;  // rows with a mask record to prefetch
+    int4 mcur = make_int4(0, 0, 0, 0);
+    if (pre_rows) mcur = load_row4(rows, B, ib, j);
     for (int p = ib; p < ie; ++p) {
-        const int pn = (p + 1 == Nx) ? 0 : p + 1;  // clamped boxes never reach Nx-1 here
-        ld5(pn, o_own, qn);
-        if (rowh) ld5(pn, o_row, hrn);
-        if (colh) ld5(pn, o_col, hcoln);
+        // next plane's halo and row mask, and own Q(p+2), a whole iteration ahead of use
+        const bool more = p + 1 < ie;
+        const int pn = wrap_or_clamp(p + 1, Nx, per);
+        int4 mnext = make_int4(0, 0, 0, 0);
+        float qf[5];
+        if (more) {
+            if (pre_rows) mnext = load_row4(rows, B, p + 1, j);
+            ld5(wrap_or_clamp(p + 2, Nx, per), o_own, qf);
+            if (rowh) ld5(pn, o_row, hrn);
+            if (colh) ld5(pn, o_col, hcoln);
+        }
         float* L = lds + (p & 1) * SLOT;
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
@@ -237,10 +332,10 @@ __global__ __launch_bounds__(TK* TJ) void step_l1_kernel(const float* __restrict
             if (rowh) L[c * PL + hrow] = hr[c];
             if (colh) L[c * PL + hcolx] = hcol[c];
         }
-        __syncthreads();
+        lds_barrier();
         if (out_ok) {
             bool evolve, shell;
-            decode_mask(rows, B, p, j, k, evolve, shell);
+            decode_row4(mcur, B, k, evolve, shell);
             const bool live = (evolve && !((B.flags & F_SKIPSHELL) && shell)) || ((B.flags & F_BC) && shell);
             if (live) {
                 float h[5];
@@ -255,10 +350,10 @@ __global__ __launch_bounds__(TK* TJ) void step_l1_kernel(const float* __restrict
                     if (imex.w != 0.0f) h[c] += imex.x * sxx + imex.y * syy + imex.z * szz;
                 }
                 float o[5];
-                const long long local = (long long)p * plane + (long long)j * Nz + k;
-                if (finish_voxel<NOISE>(B, D, p, j, k, local, evolve, shell, qc, h, o, imex.w != 0.0f)) {
+                const long long inpl = (long long)j * Nz + k, pb = (long long)p * plane;
+                if (finish_voxel<NOISE>(B, D, p, j, k, pb + inpl, evolve, shell, qc, h, o, imex.w != 0.0f)) {
 #pragma unroll
-                    for (int c = 0; c < 5; ++c) __builtin_nontemporal_store(o[c], obase + c * cs + local);
+                    for (int c = 0; c < 5; ++c) F.st(c, pb, inpl, o[c]);
                 }
             }
         }
@@ -266,9 +361,11 @@ __global__ __launch_bounds__(TK* TJ) void step_l1_kernel(const float* __restrict
         for (int c = 0; c < 5; ++c) {
             qm[c] = qc[c];
             qc[c] = qn[c];
+            qn[c] = more ? qf[c] : qn[c];
             hr[c] = hrn[c];
             hcol[c] = hcoln[c];
         }
+        mcur = mnext;
     }
 }
 
@@ -300,7 +397,7 @@ struct AnisoGeom {
 
 }  // namespace
 
-template <int TJv, int MINW, bool NOISE>
+template <int TJv, int MINW, bool NOISE, bool BUF>
 __global__ __launch_bounds__(TK* TJv, MINW) void step_aniso_kernel(const float* __restrict__ qin, float* __restrict__ qout,
                                                              long long cs, const BoxDev* __restrict__ boxes,
                                                              const int4* __restrict__ tiles,
@@ -329,8 +426,7 @@ __global__ __launch_bounds__(TK* TJv, MINW) void step_aniso_kernel(const float* 
     const int j = j0 + tj, k = k0 + tk;
     const bool out_ok = per ? (j < Ny && k < Nz) : (j >= 1 && j <= Ny - 2 && k >= 1 && k <= Nz - 2);
     const long long plane = (long long)Ny * Nz;
-    const float* base = qin + B.off;
-    float* obase = qout + B.off;
+    const Field<BUF> F(qin, qout, cs, B.off);
     const bool hasL2 = (B.flags & F_L2) != 0, hasL3 = (B.flags & F_L3) != 0;
     auto face = [&](int x, int n) { return !per && (x == 0 || x == n - 1); };
 
@@ -374,27 +470,41 @@ __global__ __launch_bounds__(TK* TJv, MINW) void step_aniso_kernel(const float* 
 
     auto pl = [&](int p) { return (long long)wrap_or_clamp(p, Nx, per) * plane; };
     auto ld5 = [&](int p, long long o, float v[5]) {
-        const float* s = base + pl(p) + o;
+        const long long pb = pl(p);
 #pragma unroll
-        for (int c = 0; c < 5; ++c) v[c] = s[c * cs];
+        for (int c = 0; c < 5; ++c) v[c] = F.ld(c, pb, o);
     };
     auto ld3 = [&](int p, long long o, float v[3]) {
-        const float* s = base + pl(p) + o;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) v[c] = s[c * cs];
-    };
-    // write plane p (own value `own` + halo straight from global) into Q slot `slot`
-    auto fill_slot = [&](int slot, int p, const float own[5]) {
-        float* Q = qs + slot * QSLOT;
         const long long pb = pl(p);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c] = F.ld(c, pb, o);
+    };
+    // halo of plane p (this thread's share) into registers
+    auto load_halo = [&](int p, float hv[G::NHPT][5]) {
+        const long long pb = pl(p);
+#pragma unroll
+        for (int s = 0; s < G::NHPT; ++s)
+            if (hpos[s] >= 0) {
+#pragma unroll
+                for (int c = 0; c < 5; ++c) hv[s][c] = F.ld(c, pb, hoff[s]);
+            }
+    };
+    // write a plane (own value `own` + halo `hv`) into Q slot `slot`
+    auto write_slot = [&](int slot, const float own[5], const float hv[G::NHPT][5]) {
+        float* Q = qs + slot * QSLOT;
 #pragma unroll
         for (int c = 0; c < 5; ++c) Q[c * P2 + lq] = own[c];
 #pragma unroll
         for (int s = 0; s < G::NHPT; ++s)
             if (hpos[s] >= 0) {
 #pragma unroll
-                for (int c = 0; c < 5; ++c) Q[c * P2 + hpos[s]] = base[c * cs + pb + hoff[s]];
+                for (int c = 0; c < 5; ++c) Q[c * P2 + hpos[s]] = hv[s][c];
             }
+    };
+    auto fill_slot = [&](int slot, int p, const float own[5]) {
+        float hv[G::NHPT][5];
+        load_halo(p, hv);
+        write_slot(slot, own, hv);
     };
     // own first derivatives of a lead plane (face-aware scales)
     auto own_d1 = [&](int slot, float dy[5], float dz[5]) {
@@ -463,7 +573,7 @@ __global__ __launch_bounds__(TK* TJv, MINW) void step_aniso_kernel(const float* 
         }
         fill_slot(0, ib - 1, q0);
         fill_slot(1, ib, q1);
-        __syncthreads();
+        lds_barrier();
         own_d1(0, dy0, dz0);
         dcol_own(qm2, q1, axs(ib - 1), dy0, dz0, D0);
         own_d1(1, dy1, dz1);
@@ -476,14 +586,27 @@ __global__ __launch_bounds__(TK* TJv, MINW) void step_aniso_kernel(const float* 
             store_d(ib & 1, ld_ring, Dr);
         }
         // r-queue now holds planes ib, ib+1, ib+2 as r0, r1, r2
-        __syncthreads();
+        lds_barrier();
         fill_slot(0, ib + 1, q2);
-        __syncthreads();
+        lds_barrier();
     }
 
+    const bool pre_rows = out_ok && !per;  // rows with a mask record to prefetch
+    int4 mcur = make_int4(0, 0, 0, 0);
+    if (pre_rows) mcur = load_row4(rows, B, ib, j);
     for (int t = ib; t < ie; ++t) {
         const int qsl = (t - ib) & 1;  // Q slot holding lead plane t+1 (plane L sits in slot (L - ib + 1) & 1)
         const int dsl = (t + 1) & 1;   // D slot of Dcol(t+1) (Dcol(L) sits in slot L & 1)
+        // prefetch, a whole iteration ahead of use: halo of lead plane t+2 and own/ring Q(t+3)
+        const bool more = t + 1 < ie;
+        float hv[G::NHPT][5], q4[5], r3[3] = {};
+        int4 mnext = make_int4(0, 0, 0, 0);
+        if (more) {
+            if (pre_rows) mnext = load_row4(rows, B, t + 1, j);
+            load_halo(t + 2, hv);
+            ld5(t + 3, o_own, q4);
+            if (ring) ld3(t + 3, o_ring, r3);
+        }
         // (b) lead plane t+1: first derivatives, own and ring Dcol(t+1)
         own_d1(qsl, dy2, dz2);
         dcol_own(q1, q3, axs(t + 1), dy2, dz2, D2);
@@ -493,11 +616,11 @@ __global__ __launch_bounds__(TK* TJv, MINW) void step_aniso_kernel(const float* 
             dcol_ring(qsl, r0, r2, axs(t + 1), Dr);
             store_d(dsl, ld_ring, Dr);
         }
-        __syncthreads();
+        lds_barrier();
         // (d) output plane t
         if (out_ok) {
             bool evolve, shell;
-            decode_mask(rows, B, t, j, k, evolve, shell);
+            decode_row4(mcur, B, k, evolve, shell);
             const bool live = (evolve && !((B.flags & F_SKIPSHELL) && shell)) || ((B.flags & F_BC) && shell);
             if (live) {
                 const float* Dc = ds + (t & 1) * DSLOT;
@@ -569,10 +692,10 @@ __global__ __launch_bounds__(TK* TJv, MINW) void step_aniso_kernel(const float* 
                     for (int c = 0; c < 5; ++c) h[c] += imex.x * sx_[c] + imex.y * syy1[c] + imex.z * szz1[c];
                 }
                 float o[5];
-                const long long local = (long long)t * plane + (long long)j * Nz + k;
-                if (finish_voxel<NOISE>(B, D, t, j, k, local, evolve, shell, q1, h, o, imex.w != 0.0f)) {
+                const long long inpl = (long long)j * Nz + k, pb = (long long)t * plane;
+                if (finish_voxel<NOISE>(B, D, t, j, k, pb + inpl, evolve, shell, q1, h, o, imex.w != 0.0f)) {
 #pragma unroll
-                    for (int c = 0; c < 5; ++c) __builtin_nontemporal_store(o[c], obase + c * cs + local);
+                    for (int c = 0; c < 5; ++c) F.st(c, pb, inpl, o[c]);
                 }
             }
         }
@@ -583,20 +706,17 @@ __global__ __launch_bounds__(TK* TJv, MINW) void step_aniso_kernel(const float* 
         for (int c = 0; c < 5; ++c) {
             dy0[c] = dy1[c]; dz0[c] = dz1[c];
             dy1[c] = dy2[c]; dz1[c] = dz2[c];
-            q0[c] = q1[c]; q1[c] = q2[c]; q2[c] = q3[c];
+            q0[c] = q1[c]; q1[c] = q2[c]; q2[c] = q3[c]; q3[c] = q4[c];
         }
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             D0[a] = D1[a]; D1[a] = D2[a];
-            r0[a] = r1[a]; r1[a] = r2[a];
+            r0[a] = r1[a]; r1[a] = r2[a]; r2[a] = r3[a];
         }
-        if (t + 1 < ie) {
-            // (e) lead plane t+2 into the other Q slot; fetch plane t+3 into the queues
-            fill_slot(qsl ^ 1, t + 2, q2);
-            ld5(t + 3, o_own, q3);
-            if (ring) ld3(t + 3, o_ring, r2);
-        }
-        __syncthreads();
+        // (e) lead plane t+2 into the other Q slot (its own value is now q2)
+        if (more) write_slot(qsl ^ 1, q2, hv);
+        mcur = mnext;
+        lds_barrier();
     }
 }
 
@@ -841,17 +961,26 @@ __global__ void soa_to_aos_kernel(const float* __restrict__ q, double* __restric
 }
 
 // Explicit instantiations used by the context code.
-template __global__ void step_l1_kernel<false>(const float*, float*, long long, const BoxDev*, const int4*,
-                                               const StepDyn*, int, const RowMask*, int, const float4*);
-template __global__ void step_l1_kernel<true>(const float*, float*, long long, const BoxDev*, const int4*,
-                                              const StepDyn*, int, const RowMask*, int, const float4*);
-#define QSR_INST_ANISO(MINW, NZ)                                                                         \
-    template __global__ void step_aniso_kernel<TJ, MINW, NZ>(const float*, float*, long long, const BoxDev*, \
-                                                             const int4*, const StepDyn*, int, const RowMask*, int, const float4*);
-QSR_INST_ANISO(3, false)
-QSR_INST_ANISO(3, true)
-QSR_INST_ANISO(4, false)
-QSR_INST_ANISO(4, true)
+#define QSR_INST_L1(NZ, BF)                                                                          \
+    template __global__ void step_l1_kernel<NZ, BF>(const float*, float*, long long, const BoxDev*, const int4*, \
+                                                    const StepDyn*, int, const RowMask*, int, const float4*);
+QSR_INST_L1(false, false)
+QSR_INST_L1(true, false)
+QSR_INST_L1(false, true)
+QSR_INST_L1(true, true)
+#undef QSR_INST_L1
+#define QSR_INST_ANISO(MINW, NZ, BF)                                                                        \
+    template __global__ void step_aniso_kernel<TJ, MINW, NZ, BF>(const float*, float*, long long, const BoxDev*, \
+                                                                 const int4*, const StepDyn*, int, const RowMask*, int, \
+                                                                 const float4*);
+QSR_INST_ANISO(3, false, false)
+QSR_INST_ANISO(3, true, false)
+QSR_INST_ANISO(4, false, false)
+QSR_INST_ANISO(4, true, false)
+QSR_INST_ANISO(3, false, true)
+QSR_INST_ANISO(3, true, true)
+QSR_INST_ANISO(4, false, true)
+QSR_INST_ANISO(4, true, true)
 #undef QSR_INST_ANISO
 
 // ---------------------------------------------------------------------------
@@ -864,17 +993,32 @@ hipError_t launch_step(bool full, int variant, bool noise, int ntiles, hipStream
                        const RowMask* rows, int ch, const float4* imex) {
     const dim3 grid(ntiles), block(TK * TJ);
 #define QSR_GO(...) hipLaunchKernelGGL(HIP_KERNEL_NAME(__VA_ARGS__), grid, block, 0, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex)
+    // 32-bit buffer offsets reach every element when 5 planes x cs floats fit 4 GiB
+    const bool buf = cs < (1LL << 32) / 20;
     if (full) {
         if (variant == 1) {
-            if (noise) QSR_GO(step_aniso_kernel<TJ, 4, true>);
-            else QSR_GO(step_aniso_kernel<TJ, 4, false>);
+            if (buf) {
+                if (noise) QSR_GO(step_aniso_kernel<TJ, 4, true, true>);
+                else QSR_GO(step_aniso_kernel<TJ, 4, false, true>);
+            } else {
+                if (noise) QSR_GO(step_aniso_kernel<TJ, 4, true, false>);
+                else QSR_GO(step_aniso_kernel<TJ, 4, false, false>);
+            }
         } else {
-            if (noise) QSR_GO(step_aniso_kernel<TJ, 3, true>);
-            else QSR_GO(step_aniso_kernel<TJ, 3, false>);
+            if (buf) {
+                if (noise) QSR_GO(step_aniso_kernel<TJ, 3, true, true>);
+                else QSR_GO(step_aniso_kernel<TJ, 3, false, true>);
+            } else {
+                if (noise) QSR_GO(step_aniso_kernel<TJ, 3, true, false>);
+                else QSR_GO(step_aniso_kernel<TJ, 3, false, false>);
+            }
         }
+    } else if (buf) {
+        if (noise) QSR_GO(step_l1_kernel<true, true>);
+        else QSR_GO(step_l1_kernel<false, true>);
     } else {
-        if (noise) QSR_GO(step_l1_kernel<true>);
-        else QSR_GO(step_l1_kernel<false>);
+        if (noise) QSR_GO(step_l1_kernel<true, false>);
+        else QSR_GO(step_l1_kernel<false, false>);
     }
 #undef QSR_GO
     return hipGetLastError();

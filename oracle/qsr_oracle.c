@@ -1063,13 +1063,18 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
  * (u0,u1), (u2,u3), (u4,u5). */
 void orc_noise5(uint64_t seed, uint64_t voxel, uint32_t step, double g[5]) {
     const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
-    uint32_t r[8];
-    for (uint32_t blk = 0; blk < 2; ++blk) {
-        const uint32_t ctr[4] = {(uint32_t)voxel, (uint32_t)(voxel >> 32), step, blk};
-        orc_philox4x32_10(ctr, key, r + 4 * blk);
-    }
+    const uint32_t ctr[4] = {(uint32_t)voxel, (uint32_t)(voxel >> 32), step, 0u};
+    uint32_t r[4];
+    orc_philox4x32_10(ctr, key, r);
+    /* the 128-bit block as r0 + 2^32 r1 + 2^64 r2 + 2^96 r3, cut into six 21-bit fields */
     double u[6];
-    for (int t = 0; t < 6; ++t) u[t] = ((double)(r[t] >> 9) + 0.5) * (1.0 / 8388608.0);
+    for (int t = 0; t < 6; ++t) {
+        const int bit = 21 * t, w = bit / 32, sh = bit % 32;
+        uint64_t two = r[w];
+        if (w + 1 < 4) two |= (uint64_t)r[w + 1] << 32;
+        const uint32_t f = (uint32_t)(two >> sh) & 0x1FFFFFu;
+        u[t] = ((double)f + 0.5) * (1.0 / 2097152.0);
+    }
     for (int pr = 0; pr < 3; ++pr) {
         const double rad = sqrt(-2.0 * log(u[2 * pr]));
         const double ang = 2.0 * ORC_PI * u[2 * pr + 1];

@@ -117,6 +117,10 @@ void orc_normal_stream(uint32_t seed, double mean, double stddev, double* out, s
 
 /* ---- periodic bulk branch (KZM_bulk_ldg.cu:351-469, 543-713) ---- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* Five N(0,1) variates for (seed, voxel, step): one Philox4x32-10 block, counter
+ * (voxel lo, voxel hi, step, 0), key (seed lo, seed hi); six 21-bit fields of the
+ * 128 bits -> uniforms (f + 1/2) 2^-21 -> three Box-Muller pairs.  This build's
+ * noise design (the reference's periodic benchmark uses splitmix64). */
 void orc_noise5(uint64_t seed, uint64_t voxel, uint32_t step, double g[5]);
 void orc_periodic_step(const double* cur, double* next, int Nx, int Ny, int Nz,
                        double dx, double dy, double dz, double A, double B, double C, double kappa,

@@ -120,6 +120,23 @@ def test_distributions_moments():
     assert O.noise5(1, 2, 3) != O.noise5(1, 2, 4) and O.noise5(1, 2, 3) == O.noise5(1, 2, 3)
 
 
+def test_noise_is_standard_normal_and_uncorrelated():
+    """Every one of the five components is N(0,1) (Kolmogorov-Smirnov), the components are uncorrelated
+    (a wrong bit field or a shared uniform would correlate them), and 21-bit uniforms bound the tails at
+    sqrt(2 ln 2^22)."""
+    from scipy import stats
+    z = np.array([O.noise5(777, v, 11) for v in range(30000)])
+    for c in range(5):
+        assert stats.kstest(z[:, c], "norm").pvalue > 1e-3, c
+    corr = np.corrcoef(z.T) - np.eye(5)
+    assert np.abs(corr).max() < 0.03
+    assert np.abs(z).max() <= math.sqrt(2 * 22 * math.log(2))
+    # neighbouring voxels and steps draw independent blocks
+    z2 = np.array([O.noise5(777, v + 1, 11) for v in range(30000)])
+    z3 = np.array([O.noise5(777, v, 12) for v in range(30000)])
+    assert abs(np.corrcoef(z[:, 0], z2[:, 0])[0, 1]) < 0.03 and abs(np.corrcoef(z[:, 4], z3[:, 4])[0, 1]) < 0.03
+
+
 # ---- golden vectors from the reference's own QSR.cpp functions ------------------------------
 
 @pytest.fixture(scope="module")
