@@ -142,6 +142,9 @@ def main():
     ap.add_argument("--graph", type=int, default=0, help="replay steps from a hipGraph of this many steps")
     ap.add_argument("--tuning", default="", metavar="VARIANT,PLANES",
                     help="fixed stencil variant and planes per tile instead of autotuning (profiling runs)")
+    ap.add_argument("--imex", type=int, default=0, metavar="J",
+                    help="time the semi-implicit quench step (RHS + J Jacobi sweeps, L_stab = L1 or kappa) "
+                         "instead of the explicit step")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     ap.add_argument("--workload", default=WORKLOADS[0], choices=WORKLOADS,
@@ -185,13 +188,20 @@ def main():
         else:
             tune = s.autotune(10)
         log(f"tuning: {tune}")
-        s.step(args.warmup)
+        L_stab = p.L1 if p.L1 else p.kappa
+
+        def advance(n):
+            if args.imex:
+                s.step_imex(n, L_stab, args.imex)
+            elif args.graph:
+                s.step_graph(n, args.graph)
+            else:
+                s.step(n)
+
+        advance(args.warmup)
         log(f"warmup {args.warmup} steps done")
         t0 = time.perf_counter()
-        if args.graph:
-            s.step_graph(args.steps, args.graph)
-        else:
-            s.step(args.steps)
+        advance(args.steps)
         wall = time.perf_counter() - t0
         ev_ms = s.last_step_ms()
         log(f"timed {args.steps} steps: wall {wall:.3f} s, event {ev_ms:.4f} ms/step")
@@ -210,7 +220,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic (seeded random-director droplet, reference init_mode 0)",
-        "config": {"workload": describe(p, args.workload),
+        "config": {"workload": describe(p, args.workload) + (f", semi-implicit J={args.imex}" if args.imex else ""),
                    "box_voxels": work["box_voxels"], "active_voxels": work["active_voxels"],
                    "tile_voxels": work["tile_voxels"], "graph_chunk": args.graph, "autotune": tune,
                    "droplet_voxel_updates_per_s": work["active_voxels"] * args.steps / wall,

@@ -111,6 +111,8 @@ struct Ctx {
     float* q_rhs = nullptr;
     float4* d_imex = nullptr;   // per box alpha/d^2 (RHS mode of the fused kernels)
     float4* d_jcoef = nullptr;  // per box alpha/d^2 and the Jacobi denominator
+    unsigned* d_jflags = nullptr;  // sweep convergence flags, 2 rows x jflag_n
+    int jflag_n = 0, jflag_row = 0;
     long long* d_vox_prefix = nullptr;
     long long nvox_total = 0;
 
@@ -120,7 +122,8 @@ struct Ctx {
         for (float* p : q) discard(hipFree(p));
         for (void* p : {(void*)d_boxes, (void*)d_tiles, (void*)d_step_rows, (void*)d_obs_rows, (void*)d_face_prefix,
                         (void*)d_dyn, (void*)d_part, (void*)d_out, (void*)d_nf_slice[0], (void*)d_nf_slice[1],
-                        (void*)d_nf3, (void*)q_rhs, (void*)d_imex, (void*)d_jcoef, (void*)d_vox_prefix})
+                        (void*)d_nf3, (void*)q_rhs, (void*)d_imex, (void*)d_jcoef, (void*)d_vox_prefix,
+                        (void*)d_jflags})
             discard(hipFree(p));
         if (h_dyn) discard(hipHostFree(h_dyn));
         for (hipEvent_t e : ev)
@@ -681,11 +684,22 @@ int qsr_step_imex(void* ctx, int nsteps, double L_stab, int jacobi_iters) {
         const double alpha = (p.dt / p.gamma) * L_stab;  // QSR.cu:766, 2729
         const double ax = alpha / (p.dx * p.dx), ay = alpha / (p.dy * p.dy), az = alpha / (p.dz * p.dz);
         im[b] = make_float4(float(ax), float(ay), float(az), 1.0f);
-        jc[b] = make_float4(float(ax), float(ay), float(az), float(1.0 + 2.0 * (ax + ay + az)));
+        jc[b] = make_float4(float(ax), float(ay), float(az), float(1.0 / (1.0 + 2.0 * (ax + ay + az))));
     }
     HIP_TRY(hipStreamSynchronize(c->st));
     HIP_TRY(hipMemcpy(c->d_imex, im.data(), sizeof(float4) * nb, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->d_jcoef, jc.data(), sizeof(float4) * nb, hipMemcpyHostToDevice));
+    if (jacobi_iters != c->jflag_n) {
+        // the flag rings start cleared; each sweep keeps the next step's row cleared after that
+        discard(hipFree(c->d_jflags));
+        c->d_jflags = nullptr;
+        HIP_TRY(hipMalloc(&c->d_jflags, sizeof(unsigned) * 2 * jacobi_iters));
+        HIP_TRY(hipMemsetAsync(c->d_jflags, 0, sizeof(unsigned) * 2 * jacobi_iters, c->st));
+        c->jflag_n = jacobi_iters;
+        c->jflag_row = 0;
+    }
+    bool any_bc = false;  // strong anchoring re-imposed after the sweeps (QSR.cu:2753)
+    for (const qsrk::BoxDev& d : c->h_boxes) any_bc |= (d.flags & qsrk::F_BC) != 0;
     HIP_TRY(hipEventRecord(c->ev[0], c->st));
     int done = 0;
     while (done < nsteps) {
@@ -698,14 +712,20 @@ int qsr_step_imex(void* ctx, int nsteps, double L_stab, int jacobi_iters) {
                 HIP_TRY(qsrk::launch_step(c->full, c->variant, false, c->ntiles, c->st, c->q[c->cur], c->q_rhs, c->cs,
                                           c->d_boxes, c->d_tiles, dyn, nb, c->d_step_rows, c->ch, c->d_imex));
             int src = c->cur;
+            HIP_TRY(qsrk::launch_face_copy(c->nface, c->st, c->q[src], c->q[1 - src], c->cs, c->d_boxes,
+                                           c->d_face_prefix, nb, c->d_step_rows));
             for (int it = 0; it < jacobi_iters; ++it) {
-                HIP_TRY(qsrk::launch_jacobi(c->nvox_total, c->st, c->q_rhs, c->q[src], c->q[1 - src], c->cs, c->d_boxes,
-                                            c->d_vox_prefix, nb, c->d_step_rows, c->d_jcoef));
+                if (c->ntiles > 0)
+                    HIP_TRY(qsrk::launch_jacobi(c->ntiles, c->st, c->q_rhs, c->q[src], c->q[1 - src], c->cs,
+                                                c->d_boxes, c->d_tiles, c->d_step_rows, c->ch, c->d_jcoef,
+                                                c->d_jflags, it, jacobi_iters, c->jflag_row));
                 src = 1 - src;
             }
+            c->jflag_row ^= 1;
             c->cur = src;
-            HIP_TRY(qsrk::launch_bc(c->nvox_total, c->st, c->q[c->cur], c->cs, c->d_boxes, dyn, c->d_vox_prefix, nb,
-                                    c->d_step_rows));
+            if (any_bc)
+                HIP_TRY(qsrk::launch_bc(c->nvox_total, c->st, c->q[c->cur], c->cs, c->d_boxes, dyn, c->d_vox_prefix,
+                                        nb, c->d_step_rows));
         }
         advance_iters(*c, n);
         done += n;

@@ -90,9 +90,13 @@ hipError_t launch_step(bool full, int variant, bool noise, int ntiles, hipStream
 hipError_t launch_faces(long long nface, hipStream_t st, const float* qin, float* qout, long long cs,
                         const BoxDev* boxes, const StepDyn* dyn, const long long* face_prefix, int nboxes,
                         const RowMask* rows);
-hipError_t launch_jacobi(long long nvox, hipStream_t st, const float* rhs, const float* qold, float* qnew, long long cs,
-                         const BoxDev* boxes, const long long* vox_prefix, int nboxes, const RowMask* rows,
-                         const float4* coef);
+// One semi-implicit sweep over the step tiles.  flags: 2 x nsweeps words, zero
+// before the first step; row alternates between steps.
+hipError_t launch_jacobi(int ntiles, hipStream_t st, const float* rhs, const float* qold, float* qnew, long long cs,
+                         const BoxDev* boxes, const int4* tiles, const RowMask* rows, int ch, const float4* coef,
+                         unsigned* flags, int sweep, int nsweeps, int row);
+hipError_t launch_face_copy(long long nface, hipStream_t st, const float* src, float* dst, long long cs,
+                            const BoxDev* boxes, const long long* face_prefix, int nboxes, const RowMask* rows);
 hipError_t launch_bc(long long nvox, hipStream_t st, float* q, long long cs, const BoxDev* boxes, const StepDyn* dyn,
                      const long long* vox_prefix, int nboxes, const RowMask* rows);
 hipError_t launch_aos_to_soa(hipStream_t st, const double* aos, float* q0, float* q1, long long cs, long long off,

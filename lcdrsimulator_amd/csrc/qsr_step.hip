@@ -127,10 +127,11 @@ __device__ __forceinline__ void philox10(unsigned c[4], unsigned k0, unsigned k1
             k0 += 0x9E3779B9u;
             k1 += 0xBB67AE85u;
         }
-        const unsigned lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
-        const unsigned lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
-        const unsigned n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
-        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        // one 32x32->64 product per multiplier (a single v_mad_u64_u32 each)
+        const unsigned long long p0 = (unsigned long long)0xD2511F53u * c[0];
+        const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c[2];
+        const unsigned n0 = unsigned(p1 >> 32) ^ c[1] ^ k0, n2 = unsigned(p0 >> 32) ^ c[3] ^ k1;
+        c[0] = n0; c[1] = unsigned(p1); c[2] = n2; c[3] = unsigned(p0);
     }
 }
 
@@ -864,55 +865,176 @@ __device__ __forceinline__ int box_of(long long g, const long long* prefix, int 
     return b;
 }
 
-// One Jacobi sweep of (I - alpha lap) Q_new = RHS (QSR.cu:786-849): box faces,
-// frozen strong shells and non-evolving voxels keep Q_old.  Voxels that never
-// change are skipped (both buffers hold them).  coef[b] = (alpha/dx^2,
-// alpha/dy^2, alpha/dz^2, 1 + 2 alpha sum 1/d^2).
-__global__ __launch_bounds__(256) void jacobi_kernel(const float* __restrict__ rhs, const float* __restrict__ qold,
-                                                     float* __restrict__ qnew, long long cs,
-                                                     const BoxDev* __restrict__ boxes,
-                                                     const long long* __restrict__ vox_prefix, int nboxes,
-                                                     const RowMask* __restrict__ rows,
-                                                     const float4* __restrict__ coef) {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= vox_prefix[nboxes]) return;
-    const int b = box_of(g, vox_prefix, nboxes);
-    const BoxDev B = boxes[b];
-    const long long v = g - vox_prefix[b];
+// One Jacobi sweep of (I - alpha lap) Q_new = RHS (QSR.cu:786-849) over the
+// step's tiles, streamed along i like the L1 stencil (x-register queue, one LDS
+// plane with halo 1).  Frozen strong shells and non-evolving voxels keep Q_old;
+// box faces are never tiles: face_copy_kernel makes both buffers hold the
+// step's input there before the sweeps.  coef[b] = (alpha/dx^2, alpha/dy^2,
+// alpha/dz^2, 1 / (1 + 2 alpha sum 1/d^2)).
+//
+// Converged sweeps are skipped without changing the result.  The sweeps
+// ping-pong between two buffers, so the output buffer of sweep s still holds the
+// iterate of sweep s-2 (for s = 1: the step's input, the iterate before sweep
+// 0).  Sweep s sets flags[row][s] unless every value it writes equals what its
+// output buffer held: then out(s) = out(s-2), the FP32 iteration has reached a
+// fixed point or a two-cycle (last-bit flips between neighbours), both buffers
+// already hold the two states every later sweep would alternate between, and
+// those sweeps return at once.  Sweep 0 always counts as changed.  Sweep s also
+// clears flags[row ^ 1][s] for the next step (rows alternate between steps).
+template <bool BUF>
+__global__ __launch_bounds__(TK* TJ) void jacobi_kernel(const float* __restrict__ rhs, const float* __restrict__ qin,
+                                                         float* __restrict__ qout, long long cs,
+                                                         const BoxDev* __restrict__ boxes,
+                                                         const int4* __restrict__ tiles,
+                                                         const RowMask* __restrict__ rows, int ch,
+                                                         const float4* __restrict__ coef,
+                                                         unsigned* __restrict__ flags, int sweep, int nsweeps,
+                                                         int row) {
+    constexpr int W = TK + 2, R = TJ + 2, PL = W * R, SLOT = 5 * PL;
+    __shared__ float lds[2 * SLOT];
+    unsigned* fl_cur = flags + row * nsweeps;
+    if (nsweeps > 1) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) flags[(row ^ 1) * nsweeps + sweep] = 0u;
+        if (sweep > 0 && fl_cur[sweep - 1] == 0u) return;  // the same decision in every block
+    }
+    const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;  // padding tile (uniform over the block)
+    const BoxDev B = boxes[t.x];
+    const float4 cf = coef[t.x];
+    const int tk = threadIdx.x & (TK - 1), tj = threadIdx.x / TK;
     const int Nx = B.Nx, Ny = B.Ny, Nz = B.Nz;
-    const int k = int(v % Nz), j = int((v / Nz) % Ny), i = int(v / ((long long)Ny * Nz));
     const bool per = (B.flags & F_PERIODIC) != 0;
+    int ib = t.y, ie = min(t.y + ch, Nx);
+    if (!per) {
+        ib = max(ib, 1);
+        ie = min(ie, Nx - 1);
+    }
+    if (ib >= ie) return;  // uniform over the block
+    const int j = t.z + tj, k = t.w + tk;
+    const bool out_ok = per ? (j < Ny && k < Nz) : (j >= 1 && j <= Ny - 2 && k >= 1 && k <= Nz - 2);
+    const int jc = wrap_or_clamp(j, Ny, per), kc = wrap_or_clamp(k, Nz, per);
+    const bool rowh = (tj == 0) || (tj == TJ - 1);
+    const bool colh = (tk == 0) || (tk == TK - 1);
+    const int jh = wrap_or_clamp(tj == 0 ? t.z - 1 : t.z + TJ, Ny, per);
+    const int kh = wrap_or_clamp(tk == 0 ? t.w - 1 : t.w + TK, Nz, per);
+    const long long plane = (long long)Ny * Nz;
+    const long long o_own = (long long)jc * Nz + kc, o_row = (long long)jh * Nz + kc, o_col = (long long)jc * Nz + kh;
+    const Field<BUF> F(qin, qout, cs, B.off);    // iterate in, iterate out
+    const Field<BUF> Fr(rhs, qout, cs, B.off);   // right-hand side
+    const Field<BUF> Fo(qout, qout, cs, B.off);  // what the output buffer holds (iterate two sweeps back)
+    const bool detect = nsweeps > 1 && sweep > 0;
+
+    auto ld5 = [&](const Field<BUF>& A, int p, long long o, float v[5]) {
+        const long long pb = (long long)p * plane;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) v[c] = A.ld(c, pb, o);
+    };
+    float qm[5], qc[5], qn[5], hr[5] = {}, hcol[5] = {}, hrn[5] = {}, hcoln[5] = {};
+    float rc[5] = {}, oc[5] = {};
+    ld5(F, wrap_or_clamp(ib - 1, Nx, per), o_own, qm);
+    ld5(F, ib, o_own, qc);
+    ld5(F, wrap_or_clamp(ib + 1, Nx, per), o_own, qn);
+    if (rowh) ld5(F, ib, o_row, hr);
+    if (colh) ld5(F, ib, o_col, hcol);
+    if (out_ok) {
+        ld5(Fr, ib, o_own, rc);
+        if (detect) ld5(Fo, ib, o_own, oc);
+    }
+    const int lrow = (tj + 1) * W + tk + 1;
+    const int hrow = (tj == 0 ? 0 : (TJ + 1)) * W + tk + 1;
+    const int hcolx = (tj + 1) * W + (tk == 0 ? 0 : TK + 1);
+    const bool pre_rows = out_ok && !per;
+    int4 mcur = make_int4(0, 0, 0, 0);
+    if (pre_rows) mcur = load_row4(rows, B, ib, j);
+    bool changed = false;
+    for (int p = ib; p < ie; ++p) {
+        // next plane's halo, row mask, RHS and old output, and own Q(p+2): a whole iteration ahead of use
+        const bool more = p + 1 < ie;
+        const int pn = wrap_or_clamp(p + 1, Nx, per);
+        int4 mnext = make_int4(0, 0, 0, 0);
+        float qf[5], rn[5] = {}, on[5] = {};
+        if (more) {
+            if (pre_rows) mnext = load_row4(rows, B, p + 1, j);
+            ld5(F, wrap_or_clamp(p + 2, Nx, per), o_own, qf);
+            if (rowh) ld5(F, pn, o_row, hrn);
+            if (colh) ld5(F, pn, o_col, hcoln);
+            if (out_ok) {
+                ld5(Fr, pn, o_own, rn);
+                if (detect) ld5(Fo, pn, o_own, on);
+            }
+        }
+        float* L = lds + (p & 1) * SLOT;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            L[c * PL + lrow] = qc[c];
+            if (rowh) L[c * PL + hrow] = hr[c];
+            if (colh) L[c * PL + hcolx] = hcol[c];
+        }
+        lds_barrier();
+        if (out_ok) {
+            bool evolve, shell;
+            decode_row4(mcur, B, k, evolve, shell);
+            const bool frozen = !evolve || ((B.flags & F_SKIPSHELL) && shell);
+            const long long inpl = (long long)j * Nz + k, pb = (long long)p * plane;
+            if (frozen) {
+                if (evolve || ((B.flags & F_BC) && shell)) {
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) F.st(c, pb, inpl, qc[c]);
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 5; ++c) {
+                    const float* Lc = L + c * PL;
+                    const float nv = (rc[c] + cf.x * (qn[c] + qm[c]) + cf.y * (Lc[lrow + W] + Lc[lrow - W]) +
+                                      cf.z * (Lc[lrow + 1] + Lc[lrow - 1])) *
+                                     cf.w;
+                    if (detect) changed |= __float_as_uint(nv) != __float_as_uint(oc[c]);
+                    F.st(c, pb, inpl, nv);
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            qm[c] = qc[c];
+            qc[c] = qn[c];
+            qn[c] = more ? qf[c] : qn[c];
+            hr[c] = hrn[c];
+            hcol[c] = hcoln[c];
+            rc[c] = rn[c];
+            oc[c] = on[c];
+        }
+        mcur = mnext;
+    }
+    if (nsweeps > 1) {
+        if (sweep == 0) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) fl_cur[0] = 1u;
+        } else {
+            const unsigned long long m = __ballot(changed);
+            if (m != 0ull && int(threadIdx.x & 63) == __ffsll((long long)m) - 1) fl_cur[sweep] = 1u;
+        }
+    }
+}
+
+// Box-face voxels that evolve or carry a strong shell: src -> dst, so both
+// buffers hold the step's input there (the semi-implicit sweeps keep faces).
+__global__ __launch_bounds__(256) void face_copy_kernel(const float* __restrict__ src, float* __restrict__ dst,
+                                                        long long cs, const BoxDev* __restrict__ boxes,
+                                                        const long long* __restrict__ face_prefix, int nboxes,
+                                                        const RowMask* __restrict__ rows) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= face_prefix[nboxes]) return;
+    int b = 0;
+    while (b + 1 < nboxes && face_prefix[b + 1] <= g) ++b;
+    const BoxDev B = boxes[b];
+    if (B.flags & F_PERIODIC) return;
+    int i, j, k;
+    face_decode(g - face_prefix[b], B.Nx, B.Ny, B.Nz, i, j, k);
     bool evolve, shell;
     decode_mask(rows, B, i, j, k, evolve, shell);
-    const bool face = !per && (i == 0 || j == 0 || k == 0 || i == Nx - 1 || j == Ny - 1 || k == Nz - 1);
-    const bool frozen = face || ((B.flags & F_SKIPSHELL) && shell) || !evolve;
-    const float4 cf = coef[b];
-    const float* qo = qold + B.off;
-    float* qn = qnew + B.off;
-    if (frozen || !(cf.w > 0.0f)) {
-        if (evolve || ((B.flags & F_BC) && shell)) {
+    if (!(evolve || ((B.flags & F_BC) && shell))) return;
+    const long long v = ((long long)i * B.Ny + j) * B.Nz + k;
 #pragma unroll
-            for (int c = 0; c < 5; ++c) qn[c * cs + v] = qo[c * cs + v];
-        }
-        return;
-    }
-    const long long plane = (long long)Ny * Nz;
-    auto at = [&](int ii, int jj, int kk) {
-        if (per) {
-            ii = (ii + Nx) % Nx;
-            jj = (jj + Ny) % Ny;
-            kk = (kk + Nz) % Nz;
-        }
-        return (long long)ii * plane + (long long)jj * Nz + kk;
-    };
-    const long long xp = at(i + 1, j, k), xm = at(i - 1, j, k), yp = at(i, j + 1, k), ym = at(i, j - 1, k);
-    const long long zp = at(i, j, k + 1), zm = at(i, j, k - 1);
-    const float* r = rhs + B.off;
-#pragma unroll
-    for (int c = 0; c < 5; ++c) {
-        const float* Q = qo + c * cs;
-        qn[c * cs + v] = (r[c * cs + v] + cf.x * (Q[xp] + Q[xm]) + cf.y * (Q[yp] + Q[ym]) + cf.z * (Q[zp] + Q[zm])) / cf.w;
-    }
+    for (int c = 0; c < 5; ++c) dst[c * cs + B.off + v] = src[c * cs + B.off + v];
 }
 
 // Strong anchoring re-imposed on the shell after the sweeps (QSR.cu:851-867)
@@ -1033,11 +1155,23 @@ hipError_t launch_faces(long long nface, hipStream_t st, const float* qin, float
     return hipGetLastError();
 }
 
-hipError_t launch_jacobi(long long nvox, hipStream_t st, const float* rhs, const float* qold, float* qnew, long long cs,
-                         const BoxDev* boxes, const long long* vox_prefix, int nboxes, const RowMask* rows,
-                         const float4* coef) {
-    hipLaunchKernelGGL(jacobi_kernel, dim3(unsigned((nvox + 255) / 256)), dim3(256), 0, st, rhs, qold, qnew, cs, boxes,
-                       vox_prefix, nboxes, rows, coef);
+hipError_t launch_jacobi(int ntiles, hipStream_t st, const float* rhs, const float* qold, float* qnew, long long cs,
+                         const BoxDev* boxes, const int4* tiles, const RowMask* rows, int ch, const float4* coef,
+                         unsigned* flags, int sweep, int nsweeps, int row) {
+    const bool buf = cs < (1LL << 32) / 20;
+    if (buf)
+        hipLaunchKernelGGL(jacobi_kernel<true>, dim3(ntiles), dim3(TK * TJ), 0, st, rhs, qold, qnew, cs, boxes, tiles,
+                           rows, ch, coef, flags, sweep, nsweeps, row);
+    else
+        hipLaunchKernelGGL(jacobi_kernel<false>, dim3(ntiles), dim3(TK * TJ), 0, st, rhs, qold, qnew, cs, boxes, tiles,
+                           rows, ch, coef, flags, sweep, nsweeps, row);
+    return hipGetLastError();
+}
+hipError_t launch_face_copy(long long nface, hipStream_t st, const float* src, float* dst, long long cs,
+                            const BoxDev* boxes, const long long* face_prefix, int nboxes, const RowMask* rows) {
+    if (nface <= 0) return hipSuccess;
+    hipLaunchKernelGGL(face_copy_kernel, dim3(unsigned((nface + 255) / 256)), dim3(256), 0, st, src, dst, cs, boxes,
+                       face_prefix, nboxes, rows);
     return hipGetLastError();
 }
 

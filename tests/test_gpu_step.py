@@ -161,3 +161,45 @@ def test_imex_step_matches_oracle(qsr, orc, name, J):
     err = np.abs(Qg - Qo).max()
     print(f"  max|Qgpu-Qorc| = {err:.3e}  (max|dQ| = {np.abs(Qo - Q0).max():.3e})")
     assert err <= 4 * STEP_TOL
+
+
+@pytest.mark.parametrize("scale", [100.0, 1e4])
+def test_imex_stiff_stabiliser_matches_oracle(qsr, orc, scale):
+    """A stiff stabiliser (alpha/dx^2 ~ 3e-2 and ~3) keeps the Jacobi iterate moving for many sweeps:
+    a converged-sweep skip taken too early would show against the oracle.  In FP32 a converged
+    iterate can sit in a last-bit limit cycle, so more sweeps agree to a few ulp, not bit for bit."""
+    p, od, Q0, S_shell = _case(qsr, orc, **CASES["l1_weak_custom"])
+    L_stab = p.L1 * scale
+    p.dt *= 3.0
+    runs = {}
+    for J in (25, 40):
+        with qsr.Solver(p) as s:
+            s.upload(Q0)
+            s.step_imex(2, L_stab, J)
+            runs[J] = s.download()
+    Qo = od.step_imex(Q0, S_shell, p.dt, L_stab, 25, steps=2)
+    err = np.abs(runs[25] - Qo).max()
+    print(f"  scale {scale}: max|Qgpu-Qorc| = {err:.3e}  (max|dQ| = {np.abs(Qo - Q0).max():.3e})  "
+          f"J25 vs J40 {np.abs(runs[25] - runs[40]).max():.3e}")
+    assert err <= 4 * STEP_TOL
+    if scale == 100.0:  # converged by 25 sweeps (contraction ~0.15 per sweep); at 1e4 it is not
+        assert np.abs(runs[25] - runs[40]).max() <= 4 * STEP_TOL
+
+
+def test_imex_converged_sweeps_are_exact(qsr, orc):
+    """At the reference stabiliser (L_stab = L1) the FP32 sweeps reach a fixed point or a last-bit
+    two-cycle within a few sweeps; later sweeps are skipped and must leave the result bit-identical to
+    running them (same parity of J), while the first sweeps must not be skipped (sweep 2 still changes
+    the field)."""
+    p, od, Q0, S_shell = _case(qsr, orc, **CASES["l2_l3_weak"])
+    p.dt *= 3.0
+    runs = {}
+    for J in (1, 2, 25, 41):
+        with qsr.Solver(p) as s:
+            s.upload(Q0)
+            s.step_imex(3, p.L1, J)
+            runs[J] = s.download()
+    assert not np.array_equal(runs[1], runs[2])
+    assert np.array_equal(runs[25], runs[41])
+    Qo = od.step_imex(Q0, S_shell, p.dt, p.L1, 25, steps=3)
+    assert np.abs(runs[25] - Qo).max() <= 4 * STEP_TOL
