@@ -169,6 +169,16 @@ int qsr_dt_estimate(double dx, double dy, double dz, double gamma, double a, dou
                     double T_star, int modechoice, double kappa, double L1, double L2, double L3,
                     double S_ref_used, double S0, double T_a, double T_b, int use_semi_implicit,
                     double L_stab, int jacobi_iters, double out[10]);
+/* Warm start / resume: read a Q-tensor text file in the format of
+ * saveQTensorToFile (QSR.cu:1818-1832; "i j k" + 9 tensor entries per row) into
+ * AoS double {Qxx,Qxy,Qxz,Qyy,Qyz}, reading it as QSRvis.py:2274-2304 does
+ * ('#' comments, off-diagonals averaged, rows outside the grid ignored, voxels
+ * without a row = 0).  No reference counterpart in QSR.cu, which only writes
+ * these files.  rows_read (may be NULL) receives the number of data rows. */
+int qsr_load_qtensor_file(const char* path, int Nx, int Ny, int Nz, double* aos5_out, size_t n_voxels,
+                          uint64_t* rows_read);
+/* Write AoS double Q in the saveQTensorToFile format (QSR.cu:1818-1832). */
+int qsr_save_qtensor_file(const char* path, const double* aos5, int Nx, int Ny, int Nz, size_t n_voxels);
 /* Seeded initial states in the reference draw order (QSR.cu:1841-1904):
  * mode 0 random director, 1 radial, 2 isotropic + N(0, noise_amplitude). */
 int qsr_init_field(const qsr_params* p, int mode, double S0, double noise_amplitude,

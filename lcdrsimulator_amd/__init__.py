@@ -5,4 +5,4 @@ the C ABI in include/qsr_amd.h); :mod:`lcdrsimulator_amd.qsr` is its ctypes
 binding and ``lib/QSR_hip`` the ``--config`` command-line driver.
 """
 from .qsr import (PROTOCOL_CONSTANT, PROTOCOL_RAMP, PROTOCOL_STEP, Params, QSRError, Solver, S_eq,  # noqa: F401
-                  device_count, dt_estimate, init_field, load)
+                  device_count, dt_estimate, init_field, load, load_qtensor, save_qtensor)

@@ -25,7 +25,7 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 DEV_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",
              "-Wno-unused-command-line-argument"]
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"--offload-arch={ARCH}", "-Wno-unused-command-line-argument"]
 
 DEVICE_SOURCES = ["qsr_step.hip", "qsr_obs.hip"]
 HOST_SOURCES = ["qsr_capi.cpp", "qsr_host.cpp"]

@@ -928,6 +928,27 @@ int qsr_dt_estimate(double dx, double dy, double dz, double gamma, double a, dou
     return QSR_OK;
 }
 
+int qsr_load_qtensor_file(const char* path, int Nx, int Ny, int Nz, double* aos5_out, size_t n_voxels,
+                          uint64_t* rows_read) {
+    if (!path || !aos5_out || Nx < 1 || Ny < 1 || Nz < 1) return fail(QSR_E_ARG, "bad argument");
+    if ((long long)n_voxels != (long long)Nx * Ny * Nz) return fail(QSR_E_ARG, "n_voxels mismatch");
+    std::vector<double> Q;
+    uint64_t rows = 0;
+    std::string err;
+    if (!qsrh::load_qtensor_text(path, Nx, Ny, Nz, Q, rows, err)) return fail(QSR_E_ARG, err);
+    std::memcpy(aos5_out, Q.data(), sizeof(double) * Q.size());
+    if (rows_read) *rows_read = rows;
+    return QSR_OK;
+}
+
+int qsr_save_qtensor_file(const char* path, const double* aos5, int Nx, int Ny, int Nz, size_t n_voxels) {
+    if (!path || !aos5 || Nx < 1 || Ny < 1 || Nz < 1) return fail(QSR_E_ARG, "bad argument");
+    if ((long long)n_voxels != (long long)Nx * Ny * Nz) return fail(QSR_E_ARG, "n_voxels mismatch");
+    std::string err;
+    if (!qsrh::save_qtensor_text(path, aos5, Nx, Ny, Nz, err)) return fail(QSR_E_ARG, err);
+    return QSR_OK;
+}
+
 int qsr_init_field(const qsr_params* p, int mode, double S0, double noise_amplitude, int64_t random_seed,
                    double* aos5_out, size_t n_voxels) {
     if (!p || !aos5_out) return fail(QSR_E_ARG, "null argument");

@@ -12,7 +12,8 @@
 // droplet_center_{x,y,z} (voxels), save_iteration_snapshots (sim_mode 1 writes
 // output/nematic_field_iter_*.dat at every log point when true),
 // early_stop_min_alignment_time (the 0.5*tau_align gate of QSR.cu:3193),
-// noise_sigma / noise_seed (Philox noise per step), step_graph_chunk.
+// noise_sigma / noise_seed (Philox noise per step), step_graph_chunk,
+// init_qtensor_file (warm start / resume from a Qtensor_output_*.dat file).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -115,18 +116,8 @@ void write_planes(const std::string& path, int Nx, const char* header, Fmt&& fmt
 }
 
 void save_q(const std::vector<double>& Q, int Nx, int Ny, int Nz, const std::string& path) {
-    write_planes(path, Nx, nullptr, [&](int i, std::string& out) {
-        char line[512];
-        out.reserve(size_t(Ny) * Nz * 140);
-        for (int j = 0; j < Ny; ++j)
-            for (int k = 0; k < Nz; ++k) {
-                const double* q = &Q[5 * (size_t(k) + size_t(Nz) * (size_t(j) + size_t(Ny) * i))];
-                const double zz = -q[0] - q[3];
-                const int n = std::snprintf(line, sizeof line, "%d %d %d %.6e %.6e %.6e %.6e %.6e %.6e %.6e %.6e %.6e\n",
-                                            i, j, k, q[0], q[1], q[2], q[1], q[3], q[4], q[2], q[4], zz);
-                out.append(line, size_t(n));
-            }
-    });
+    std::string err;
+    if (!qsrh::save_qtensor_text(path, Q.data(), Nx, Ny, Nz, err)) std::cerr << "[Output] " << err << std::endl;
 }
 
 void save_nematic(const std::vector<double>& nf, int Nx, int Ny, int Nz, const std::string& path) {
@@ -355,10 +346,26 @@ int main(int argc, char** argv) {
         run.p = pp;
         return check(qsr_create(&pp, 0, &run.ctx), "qsr_create");
     };
-    auto init_and_upload = [&](double noise_amp) -> int {
-        std::vector<double> Q;
+    // warm start / resume from a Q-tensor file (extension key; overrides init_mode)
+    const std::string warm = cfg.kv.count("init_qtensor_file") ? qsrh::trim(cfg.kv.at("init_qtensor_file")) : "";
+    auto initial_field = [&](double noise_amp, std::vector<double>& Q) -> int {
+        if (!warm.empty()) {
+            uint64_t rows = 0;
+            std::string err;
+            if (!qsrh::load_qtensor_text(warm, p.Nx, p.Ny, p.Nz, Q, rows, err)) {
+                std::cerr << "[Init] cannot warm start from " << warm << ": " << err << std::endl;
+                return 1;
+            }
+            std::cout << "[Init] Warm start from " << warm << " (" << rows << " rows)" << std::endl;
+            return 0;
+        }
         if (mode == 2) qsrh::init_isotropic(Q, geom, noise_amp, random_seed);
         else qsrh::init_director(Q, geom, S0, mode, random_seed);
+        return 0;
+    };
+    auto init_and_upload = [&](double noise_amp) -> int {
+        std::vector<double> Q;
+        if (initial_field(noise_amp, Q)) return 1;
         return check(qsr_upload_q(run.ctx, 0, Q.data(), n), "upload");
     };
     auto dt_window = [&](double Ta, double Tb, bool semi, double L_stab, int J) {
@@ -638,8 +645,7 @@ int main(int argc, char** argv) {
         }
         const double noise_amp = (mode == 2) ? cfg.get_d("noise_amplitude", 1e-3) : 0.0;
         std::vector<double> Q;
-        if (mode == 2) qsrh::init_isotropic(Q, geom, noise_amp, random_seed);
-        else qsrh::init_director(Q, geom, S0, mode, random_seed);
+        if (initial_field(noise_amp, Q)) return 1;
         const qsrh::DtWindow w = dt_window(T_start, T_end, false, 0.0, 0);
         print_dt(w);
         double dt = cfg.get_d("dt", w.dt_max, "Enter time step dt (s)");

@@ -3,12 +3,16 @@
 
 #include <algorithm>
 #include <cctype>
+#include <charconv>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <iostream>
 #include <limits>
 #include <random>
 #include <sstream>
+#include <thread>
 
 namespace qsrh {
 
@@ -372,6 +376,162 @@ void init_isotropic(std::vector<double>& Q, const Geometry& g, double amp, long 
                 if (!(sample(g, i, j, k).sd <= 0.0)) continue;
                 for (int c = 0; c < 5; ++c) Q[5 * id + c] = dist(eng);
             }
+}
+
+namespace {
+
+// Parses [b, e): whole lines.  Returns false on a malformed data row.
+bool parse_qtensor_lines(const char* b, const char* e, int Nx, int Ny, int Nz, double* Q, uint64_t& rows,
+                         std::string& err) {
+    const char* p = b;
+    while (p < e) {
+        const char* nl = static_cast<const char*>(std::memchr(p, '\n', size_t(e - p)));
+        const char* le = nl ? nl : e;
+        const char* q = p;
+        while (q < le && (*q == ' ' || *q == '\t' || *q == '\r')) ++q;
+        if (q < le && *q != '#') {
+            double v[12];
+            int nv = 0;
+            while (q < le && nv < 12) {
+                while (q < le && (*q == ' ' || *q == '\t' || *q == '\r')) ++q;
+                if (q >= le) break;
+                auto r = std::from_chars(q, le, v[nv]);
+                if (r.ec != std::errc()) {
+                    err = "malformed number in row: " + std::string(p, size_t(std::min<ptrdiff_t>(le - p, 120)));
+                    return false;
+                }
+                q = r.ptr;
+                ++nv;
+            }
+            if (nv < 12) {
+                err = "row with fewer than 12 columns: " + std::string(p, size_t(std::min<ptrdiff_t>(le - p, 120)));
+                return false;
+            }
+            const long i = long(v[0]), j = long(v[1]), k = long(v[2]);
+            if (i >= 0 && i < Nx && j >= 0 && j < Ny && k >= 0 && k < Nz) {
+                double* o = Q + 5 * (size_t(k) + size_t(Nz) * (size_t(j) + size_t(Ny) * size_t(i)));
+                o[0] = v[3];
+                o[1] = 0.5 * (v[4] + v[6]);
+                o[2] = 0.5 * (v[5] + v[9]);
+                o[3] = v[7];
+                o[4] = 0.5 * (v[8] + v[10]);
+            }
+            ++rows;
+        }
+        p = nl ? nl + 1 : e;
+    }
+    return true;
+}
+
+}  // namespace
+
+bool load_qtensor_text(const std::string& path, int Nx, int Ny, int Nz, std::vector<double>& Q, uint64_t& rows,
+                       std::string& err) {
+    rows = 0;
+    if (Nx < 1 || Ny < 1 || Nz < 1) {
+        err = "bad grid size";
+        return false;
+    }
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) {
+        err = "cannot open " + path;
+        return false;
+    }
+    Q.assign(size_t(5) * size_t(Nx) * size_t(Ny) * size_t(Nz), 0.0);
+    const unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    constexpr size_t kBlock = size_t(64) << 20;
+    std::vector<char> buf;
+    std::string carry;
+    bool ok = true;
+    for (;;) {
+        buf.assign(carry.begin(), carry.end());
+        const size_t have = buf.size();
+        buf.resize(have + kBlock);
+        const size_t got = std::fread(buf.data() + have, 1, kBlock, f);
+        buf.resize(have + got);
+        const bool last = got < kBlock;
+        size_t end = buf.size();
+        carry.clear();
+        if (!last) {
+            const void* nl = nullptr;
+            for (size_t x = buf.size(); x > 0 && !nl; --x)
+                if (buf[x - 1] == '\n') nl = &buf[x - 1];
+            if (!nl) {  // a single line longer than a block: keep reading
+                carry.assign(buf.begin(), buf.end());
+                continue;
+            }
+            end = size_t(static_cast<const char*>(nl) - buf.data()) + 1;
+            carry.assign(buf.begin() + long(end), buf.end());
+        }
+        // split [0, end) at line boundaries into nth parts
+        std::vector<size_t> cut{0};
+        for (unsigned t = 1; t < nth; ++t) {
+            size_t x = end * t / nth;
+            while (x > 0 && x < end && buf[x - 1] != '\n') ++x;
+            cut.push_back(std::max(cut.back(), x));
+        }
+        cut.push_back(end);
+        std::vector<uint64_t> nrow(nth, 0);
+        std::vector<std::string> errs(nth);
+        std::vector<char> good(nth, 1);
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nth; ++t)
+            th.emplace_back([&, t] {
+                good[t] = parse_qtensor_lines(buf.data() + cut[t], buf.data() + cut[t + 1], Nx, Ny, Nz, Q.data(),
+                                              nrow[t], errs[t]);
+            });
+        for (auto& x : th) x.join();
+        for (unsigned t = 0; t < nth; ++t) {
+            rows += nrow[t];
+            if (!good[t] && ok) {
+                ok = false;
+                err = errs[t];
+            }
+        }
+        if (!ok || last) break;
+    }
+    std::fclose(f);
+    if (ok && !carry.empty()) {
+        uint64_t n = 0;
+        ok = parse_qtensor_lines(carry.data(), carry.data() + carry.size(), Nx, Ny, Nz, Q.data(), n, err);
+        rows += n;
+    }
+    return ok;
+}
+
+bool save_qtensor_text(const std::string& path, const double* Q, int Nx, int Ny, int Nz, std::string& err) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) {
+        err = "cannot write " + path;
+        return false;
+    }
+    const unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    for (int i0 = 0; i0 < Nx; i0 += int(nth)) {
+        const int n = std::min(int(nth), Nx - i0);
+        std::vector<std::string> out(static_cast<size_t>(n));
+        std::vector<std::thread> th;
+        for (int t = 0; t < n; ++t)
+            th.emplace_back([&, t] {
+                const int i = i0 + t;
+                char line[512];
+                std::string& o = out[size_t(t)];
+                o.reserve(size_t(Ny) * size_t(Nz) * 140);
+                for (int j = 0; j < Ny; ++j)
+                    for (int k = 0; k < Nz; ++k) {
+                        const double* q = Q + 5 * (size_t(k) + size_t(Nz) * (size_t(j) + size_t(Ny) * size_t(i)));
+                        const double zz = -q[0] - q[3];
+                        const int m = std::snprintf(line, sizeof line,
+                                                    "%d %d %d %.6e %.6e %.6e %.6e %.6e %.6e %.6e %.6e %.6e\n", i, j,
+                                                    k, q[0], q[1], q[2], q[1], q[3], q[4], q[2], q[4], zz);
+                        o.append(line, size_t(m));
+                    }
+            });
+        for (auto& x : th) x.join();
+        for (auto& o : out) std::fwrite(o.data(), 1, o.size(), f);
+    }
+    const bool ok = std::fclose(f) == 0;
+    if (!ok) err = "write failed: " + path;
+    return ok;
 }
 
 std::string trim(const std::string& s) {

@@ -108,6 +108,19 @@ void init_director(std::vector<double>& Q, const Geometry& g, double S0, int mod
 void init_isotropic(std::vector<double>& Q, const Geometry& g, double amp, long long seed);
 
 // ---------------------------------------------------------------------------
+// Q-tensor text files, "i j k" + the full 3x3 tensor row-major (9 values), the
+// format saveQTensorToFile writes (QSR.cu:1818-1832).  Reading follows the
+// reference's own reader (QSRvis.py:2274-2304): '#' lines are comments, rows
+// need >= 12 columns, rows outside the grid are ignored, off-diagonals are the
+// mean of the two symmetric entries, Qzz is implied (traceless), voxels with no
+// row stay 0.  Parsed in parallel over blocks of lines.
+// ---------------------------------------------------------------------------
+bool load_qtensor_text(const std::string& path, int Nx, int Ny, int Nz, std::vector<double>& Q, uint64_t& rows,
+                       std::string& err);
+// Writes Q (AoS, 5 per voxel) in i-major order with "%d %d %d" + 9 x " %.6e".
+bool save_qtensor_text(const std::string& path, const double* Q, int Nx, int Ny, int Nz, std::string& err);
+
+// ---------------------------------------------------------------------------
 // key = value configuration (QSR.cu:1182-1297)
 // ---------------------------------------------------------------------------
 using KV = std::unordered_map<std::string, std::string>;

@@ -73,6 +73,7 @@ EXPORTS = (
     "qsr_upload_q", "qsr_download_q", "qsr_download_nematic", "qsr_step", "qsr_step_graph",
     "qsr_step_imex", "qsr_observables", "qsr_box_info_get", "qsr_set_iteration", "qsr_last_step_ms",
     "qsr_step_work", "qsr_profile_steps", "qsr_autotune", "qsr_set_tuning", "qsr_S_eq", "qsr_dt_estimate", "qsr_init_field",
+    "qsr_load_qtensor_file", "qsr_save_qtensor_file",
 )
 
 
@@ -107,6 +108,9 @@ def load():
     L.qsr_S_eq.argtypes = [C.c_double] * 5 + [C.c_int]
     L.qsr_dt_estimate.argtypes = [C.c_double] * 8 + [C.c_int] + [C.c_double] * 8 + [C.c_int, C.c_double, C.c_int, dp]
     L.qsr_init_field.argtypes = [C.POINTER(_Params), C.c_int, C.c_double, C.c_double, C.c_int64, dp, C.c_size_t]
+    L.qsr_load_qtensor_file.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, dp, C.c_size_t,
+                                        C.POINTER(C.c_uint64)]
+    L.qsr_save_qtensor_file.argtypes = [C.c_char_p, dp, C.c_int, C.c_int, C.c_int, C.c_size_t]
     L.qsr_device_count.argtypes = [C.POINTER(C.c_int)]
     _lib = L
     return L
@@ -304,3 +308,22 @@ def init_field(p: Params, mode: int, S0: float = 0.5, noise_amplitude: float = 1
     _check(load().qsr_init_field(C.byref(p.to_c()), mode, S0, noise_amplitude, seed,
                                  Q.ctypes.data_as(C.POINTER(C.c_double)), p.n))
     return Q
+
+
+def load_qtensor(path, dims):
+    """Read a Qtensor_output_*.dat file (QSR.cu:1818-1832 format) into an (Nx, Ny, Nz, 5) array,
+    as QSRvis.py:2274-2304 reads it; returns (Q, rows_read)."""
+    Nx, Ny, Nz = (int(x) for x in dims)
+    Q = np.zeros((Nx, Ny, Nz, 5))
+    rows = C.c_uint64()
+    _check(load().qsr_load_qtensor_file(os.fsencode(str(path)), Nx, Ny, Nz, Q.ctypes.data_as(C.POINTER(C.c_double)),
+                                        Nx * Ny * Nz, C.byref(rows)))
+    return Q, rows.value
+
+
+def save_qtensor(path, Q):
+    """Write an (Nx, Ny, Nz, 5) field in the Qtensor_output_*.dat format (QSR.cu:1818-1832)."""
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    Nx, Ny, Nz = Q.shape[:3]
+    _check(load().qsr_save_qtensor_file(os.fsencode(str(path)), Q.ctypes.data_as(C.POINTER(C.c_double)), Nx, Ny, Nz,
+                                        Nx * Ny * Nz))

@@ -221,3 +221,26 @@ def test_cli_temperature_sweep_matches_oracle(tmp_path):
         assert abs(summ[t, 2] - o["avg_S"]) < 2e-5
         Qf, _ = read_q(tmp_path / "output_temp_sweep" / f"T_{T:.6f}" / "Qtensor_output_final.dat", dims)
         assert np.abs(Qf - Q).max() < 4e-5
+
+
+@pytest.mark.gpu
+def test_cli_warm_start_resumes_run(tmp_path):
+    """init_qtensor_file: 15 steps, save, resume from the saved Qtensor file for 15 more steps ==
+    30 uninterrupted steps, up to the 7 significant digits of the file."""
+    cfg = dict(BASE, sim_mode=1, T=300.0, iterations=30, print_freq=10, L2=3e-12, S0=0.9,
+               save_iteration_snapshots="false")
+    runs = {}
+    for name, extra in (("full", {}), ("first", dict(iterations=15))):
+        d = tmp_path / name
+        d.mkdir()
+        out = run_cli(d, dict(cfg, **extra))
+        assert out.returncode == 0
+        runs[name] = d / "Qtensor_output_final.dat"
+    d = tmp_path / "resume"
+    d.mkdir()
+    out = run_cli(d, dict(cfg, iterations=15, init_qtensor_file=str(runs["first"])))
+    assert out.returncode == 0 and "[Init] Warm start from" in out.stdout
+    dims = (cfg["Nx"], cfg["Ny"], cfg["Nz"])
+    full, _ = read_q(runs["full"], dims)
+    resumed, _ = read_q(d / "Qtensor_output_final.dat", dims)
+    assert np.abs(full - resumed).max() < 2e-6
