@@ -36,7 +36,27 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-WORKLOADS = ("droplet_l1l2l3_256", "periodic_noise_256", "droplet_l1_64")
+WORKLOADS = ("droplet_l1l2l3_256", "periodic_noise_256", "droplet_l1_64", "ladder_64")
+
+
+def ladder_params(q, n_drops=64, seed=2024):
+    """Quench-rate ladder: droplet radius R = 16 + 32 i / 63 voxels in a (2R+8)^3 box, ramp 310 -> 300 K over
+    ramp_iters log-spaced 1e3..1e5 (8 rates x 8 seeds), a quarter with strong anchoring and the rest weak with
+    W log-uniform in 1e-5..1e-3 J/m^2, L1 = 6e-12 N, isotropic start with 1e-3 noise."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    rates = np.logspace(3, 5, 8)
+    w = q.dt_estimate(1e-8, 1e-8, 1e-8, 0.1, 0.044e6, 1.413e6, 1.153e6, 307.0, 1, 0.0, 6e-12, 0, 0, math.nan, 0.5,
+                      310.0, 300.0)
+    out = []
+    for i in range(n_drops):
+        R = int(round(16 + 32 * i / (n_drops - 1)))
+        N = 2 * R + 8
+        W = 0.0 if i % 4 == 0 else float(10 ** rng.uniform(-5, -3))
+        out.append(q.Params(Nx=N, Ny=N, Nz=N, dx=1e-8, dy=1e-8, dz=1e-8, T_star=307.0, kappa=0.0, L1=6e-12, W=W,
+                            gamma=0.1, radius_index=float(R), protocol=q.PROTOCOL_RAMP, T_high=310.0, T_low=300.0,
+                            ramp_iters=int(rates[i % 8]), seed=i // 8, dt=w["dt_max"]))
+    return out
 
 
 def workload_params(q, n=256, R=120.0, workload="droplet_l1l2l3_256"):
@@ -131,6 +151,68 @@ def describe(p, workload):
             f"weak anchoring W=1e-4 custom S=0.5, explicit Euler dt={p.dt:.5e}")
 
 
+def run_ladder(q, args):
+    """64 droplets in one batched context (one launch per step over all boxes) against the same droplets
+    stepped one context at a time; value = batched voxel-updates/s over all boxes."""
+    params = ladder_params(q)
+    fields = [q.init_field(p, 2, 0.5, 1e-3, 1000 + b) for b, p in enumerate(params)]
+    total = sum(p.n for p in params)
+    log(f"ladder: {len(params)} boxes, {total} voxels, dt={params[0].dt:.5e}")
+    with q.Solver(params) as s:
+        for b, Q in enumerate(fields):
+            s.upload(Q, box=b)
+        if args.tuning:
+            v, ch = (int(x) for x in args.tuning.split(","))
+            s.set_tuning(v, ch)
+            tune = {"variant": v, "planes_per_tile": ch, "fixed": True}
+        else:
+            tune = s.autotune(10)
+        log(f"tuning: {tune}")
+        advance = (lambda n: s.step_graph(n, args.graph)) if args.graph else s.step
+        advance(args.warmup)
+        t0 = time.perf_counter()
+        advance(args.steps)
+        wall = time.perf_counter() - t0
+        main_ms, face_ms = s.profile_steps(min(100, args.steps))
+        work = s.work()
+    log(f"batched: {1e3 * wall / args.steps:.4f} ms/step")
+    solvers = []
+    try:
+        for p, Q in zip(params, fields):
+            sv = q.Solver(p)
+            solvers.append(sv)
+            sv.upload(Q)
+            sv.autotune(5)
+            sv.step(args.warmup)
+        t0 = time.perf_counter()
+        for sv in solvers:
+            sv.step(args.steps)
+        wall_plain = time.perf_counter() - t0
+    finally:
+        for sv in solvers:
+            sv.close()
+    log(f"one context per droplet: {1e3 * wall_plain / args.steps:.4f} ms/step")
+    value = total * args.steps / wall
+    traffic_bytes = BYTES_PER_VOXEL * work["tile_voxels"]
+    achieved = traffic_bytes / (main_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC, "value": value, "unit": "voxel-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded isotropic start, reference init_mode 2)",
+        "config": {"workload": "quench-rate ladder: 64 droplets R=16..48 in (2R+8)^3 boxes, ramp 310->300 K over "
+                               "1e3..1e5 steps, strong/weak anchoring, L1=6e-12, one launch per step",
+                   "box_voxels": total, "tile_voxels": work["tile_voxels"], "active_voxels": work["active_voxels"],
+                   "graph_chunk": args.graph, "autotune": tune, "fused_stencil_ms": main_ms, "face_ms": face_ms,
+                   "plain_voxel_updates_per_s": total * args.steps / wall_plain,
+                   "batched_over_plain": wall_plain / wall},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +254,8 @@ def main():
     if q.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
+    if args.workload == "ladder_64":
+        return run_ladder(q, args)
     p = workload_params(q, args.size, args.radius, args.workload)
     log(f"box {p.dims} R={p.radius_index} dt={p.dt:.5e}")
     if p.periodic:

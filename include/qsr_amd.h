@@ -149,6 +149,11 @@ int qsr_last_step_ms(void* ctx, double* ms_per_step);
  * spare buffer only, so the state and iteration count do not change) and keeps
  * the fastest.  Results are bit-identical across choices. */
 int qsr_autotune(void* ctx, int trial_steps, int* best_variant, int* best_ch, double* best_ms);
+/* Hold (active = 0) or resume one box of a batch: a held box takes no part in
+ * later steps (its tiles leave the launch, its iteration count and field stay
+ * as they are), so a quench-rate ladder can retire finished droplets early.
+ * No reference counterpart (the reference runs one droplet per process). */
+int qsr_set_box_active(void* ctx, int box, int active);
 /* Set the stencil variant (0: 3 waves/SIMD, 1: 4 waves/SIMD register budget;
  * ignored for the L1 kernel) and planes per tile explicitly. */
 int qsr_set_tuning(void* ctx, int variant, int planes_per_tile);

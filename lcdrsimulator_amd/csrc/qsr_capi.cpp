@@ -46,6 +46,7 @@ struct Box {
     int64_t iter = 0;  // steps completed
     double last_T = 0, last_S = 0;
     bool uploaded = false;
+    bool active = true;  // inactive boxes are held: not stepped, iteration frozen
     double m = 0;  // dt / gamma
 
     double T_at(int64_t it) const {
@@ -91,7 +92,7 @@ struct Ctx {
     long long tile_voxels = 0, active_voxels = 0, box_voxels = 0;
     qsrk::RowMask* d_step_rows = nullptr;
     qsrk::RowMask* d_obs_rows = nullptr;
-    long long* d_face_prefix = nullptr;
+    int4* d_face_list = nullptr;  // (box, i, j, k) of the box-face voxels that can change
     long long nface = 0;
     qsrk::StepDyn* d_dyn = nullptr;
     qsrk::StepDyn* h_dyn = nullptr;  // pinned
@@ -120,7 +121,7 @@ struct Ctx {
         if (gexec) discard(hipGraphExecDestroy(gexec));
         if (graph) discard(hipGraphDestroy(graph));
         for (float* p : q) discard(hipFree(p));
-        for (void* p : {(void*)d_boxes, (void*)d_tiles, (void*)d_step_rows, (void*)d_obs_rows, (void*)d_face_prefix,
+        for (void* p : {(void*)d_boxes, (void*)d_tiles, (void*)d_step_rows, (void*)d_obs_rows, (void*)d_face_list,
                         (void*)d_dyn, (void*)d_part, (void*)d_out, (void*)d_nf_slice[0], (void*)d_nf_slice[1],
                         (void*)d_nf3, (void*)q_rhs, (void*)d_imex, (void*)d_jcoef, (void*)d_vox_prefix,
                         (void*)d_jflags})
@@ -227,6 +228,7 @@ int build_tiles(Ctx& c) {
     c.tile_voxels = 0;
     for (size_t bi = 0; bi < c.boxes.size(); ++bi) {
         const Box& b = c.boxes[bi];
+        if (!b.active) continue;
         for (int i0 = 0; i0 < b.p.Nx; i0 += c.ch)
             for (int j0 = 0; j0 < b.p.Ny; j0 += qsrk::TJ)
                 for (int k0 = 0; k0 < b.p.Nz; k0 += qsrk::TK)
@@ -256,6 +258,32 @@ int build_tiles(Ctx& c) {
     return QSR_OK;
 }
 
+// Box-face voxels that can change in a step (the face kernel's live set: evolving
+// and not a frozen strong shell, or a strong shell re-imposed), appended as
+// (box, i, j, k) in the face order i-faces, j-faces, k-faces.  Droplets that
+// keep clear of their box faces contribute none and the face pass is skipped.
+long long append_live_faces(const Box& b, int box, std::vector<int4>& list) {
+    const qsr_params& p = b.p;
+    const bool skipshell = p.W == 0.0, bc = !(p.W > 0.0);
+    const size_t before = list.size();
+    auto visit = [&](int i, int j, int k) {
+        const int16_t* m = &b.masks.step[(size_t(i) * size_t(p.Ny) + size_t(j)) * 8];
+        const bool evolve = k >= m[0] && k <= m[1];
+        const bool shell = (k >= m[2] && k <= m[3]) && !(k >= m[4] && k <= m[5]);
+        if ((evolve && !(skipshell && shell)) || (bc && shell)) list.push_back(make_int4(box, i, j, k));
+    };
+    for (int i : {0, p.Nx - 1})
+        for (int j = 0; j < p.Ny; ++j)
+            for (int k = 0; k < p.Nz; ++k) visit(i, j, k);
+    for (int j : {0, p.Ny - 1})
+        for (int i = 1; i < p.Nx - 1; ++i)
+            for (int k = 0; k < p.Nz; ++k) visit(i, j, k);
+    for (int k : {0, p.Nz - 1})
+        for (int i = 1; i < p.Nx - 1; ++i)
+            for (int j = 1; j < p.Ny - 1; ++j) visit(i, j, k);
+    return (long long)(list.size() - before);
+}
+
 Ctx* as_ctx(void* p) { return static_cast<Ctx*>(p); }
 
 int create_impl(const qsr_params* params, int n, int device, void** out) {
@@ -273,8 +301,8 @@ int create_impl(const qsr_params* params, int n, int device, void** out) {
     HIP_TRY(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&c->ev[0]));
     HIP_TRY(hipEventCreate(&c->ev[1]));
-    long long off = 0, rows_off = 0, nface = 0;
-    std::vector<long long> face_prefix{0};
+    long long off = 0, rows_off = 0;
+    std::vector<int4> face_list;
     std::vector<qsrk::RowMask> step_rows, obs_rows;
     c->boxes.resize(n);
     c->active_voxels = 0;
@@ -295,7 +323,7 @@ int create_impl(const qsr_params* params, int n, int device, void** out) {
             if (!qsrh::build_row_masks(b.g, p.W, b.masks, err)) return fail(QSR_E_GEOMETRY, err);
             b.thickness = qsrh::shell_thickness(b.g, b.masks.shell_count);
             b.area = qsrh::surface_area(b.g);
-            b.nface = 2LL * p.Ny * p.Nz + 2LL * (p.Nx - 2) * p.Nz + 2LL * (p.Nx - 2) * (p.Ny - 2);
+            b.nface = append_live_faces(b, i, face_list);
             c->active_voxels += b.masks.active_count;
         } else {
             b.masks.step.assign(size_t(p.Nx) * p.Ny * 8, 0);
@@ -304,8 +332,6 @@ int create_impl(const qsr_params* params, int n, int device, void** out) {
             c->active_voxels += b.nvox;
         }
         c->box_voxels += b.nvox;
-        nface += b.nface;
-        face_prefix.push_back(nface);
         const size_t rows = size_t(p.Nx) * p.Ny;
         for (size_t r = 0; r < rows; ++r) {
             qsrk::RowMask s{}, o{};
@@ -322,7 +348,7 @@ int create_impl(const qsr_params* params, int n, int device, void** out) {
         if (p.noise_sigma > 0.0) c->noise = true;
     }
     c->cs = std::max<long long>(off, 64);
-    c->nface = nface;
+    c->nface = (long long)face_list.size();
     for (int s = 0; s < 2; ++s) {
         HIP_TRY(hipMalloc(&c->q[s], sizeof(float) * 5 * c->cs));
         HIP_TRY(hipMemsetAsync(c->q[s], 0, sizeof(float) * 5 * c->cs, c->st));
@@ -342,8 +368,10 @@ int create_impl(const qsr_params* params, int n, int device, void** out) {
         HIP_TRY(hipMalloc(&c->d_vox_prefix, sizeof(long long) * vp.size()));
         HIP_TRY(hipMemcpy(c->d_vox_prefix, vp.data(), sizeof(long long) * vp.size(), hipMemcpyHostToDevice));
     }
-    HIP_TRY(hipMalloc(&c->d_face_prefix, sizeof(long long) * face_prefix.size()));
-    HIP_TRY(hipMemcpy(c->d_face_prefix, face_prefix.data(), sizeof(long long) * face_prefix.size(), hipMemcpyHostToDevice));
+    if (!face_list.empty()) {
+        HIP_TRY(hipMalloc(&c->d_face_list, sizeof(int4) * face_list.size()));
+        HIP_TRY(hipMemcpy(c->d_face_list, face_list.data(), sizeof(int4) * face_list.size(), hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipMalloc(&c->d_dyn, sizeof(qsrk::StepDyn) * kDynChunk * n));
     HIP_TRY(hipHostMalloc((void**)&c->h_dyn, sizeof(qsrk::StepDyn) * kDynChunk * n, hipHostMallocDefault));
     HIP_TRY(hipMalloc(&c->d_out, sizeof(double) * 32));
@@ -378,7 +406,7 @@ int enqueue_steps(Ctx& c, int nsteps) {
         if (c.ntiles > 0)
             HIP_TRY(qsrk::launch_step(c.full, c.variant, c.noise, c.ntiles, c.st, qin, qout, c.cs, c.d_boxes, c.d_tiles, dyn, nb,
                                       c.d_step_rows, c.ch, nullptr));
-        HIP_TRY(qsrk::launch_faces(c.nface, c.st, qin, qout, c.cs, c.d_boxes, dyn, c.d_face_prefix, nb, c.d_step_rows));
+        HIP_TRY(qsrk::launch_faces(c.nface, c.st, qin, qout, c.cs, c.d_boxes, dyn, c.d_face_list, c.d_step_rows));
         c.cur = 1 - c.cur;
     }
     return QSR_OK;
@@ -394,6 +422,7 @@ int fill_dyn(Ctx& c, int nsteps) {
 
 void advance_iters(Ctx& c, int nsteps) {
     for (Box& b : c.boxes) {
+        if (!b.active) continue;
         b.iter += nsteps;
         b.last_T = b.T_at(b.iter - 1);
         b.last_S = b.S_at(b.last_T);
@@ -575,7 +604,7 @@ int qsr_profile_steps(void* ctx, int nsteps, double* main_ms, double* face_ms) {
             HIP_TRY(qsrk::launch_step(c->full, c->variant, c->noise, c->ntiles, c->st, qin, qout, c->cs, c->d_boxes, c->d_tiles,
                                       dyn, nb, c->d_step_rows, c->ch, nullptr));
         HIP_TRY(hipEventRecord(ev[3 * s + 1], c->st));
-        HIP_TRY(qsrk::launch_faces(c->nface, c->st, qin, qout, c->cs, c->d_boxes, dyn, c->d_face_prefix, nb,
+        HIP_TRY(qsrk::launch_faces(c->nface, c->st, qin, qout, c->cs, c->d_boxes, dyn, c->d_face_list,
                                    c->d_step_rows));
         HIP_TRY(hipEventRecord(ev[3 * s + 2], c->st));
         c->cur = 1 - c->cur;
@@ -648,6 +677,35 @@ int qsr_autotune(void* ctx, int trial_steps, int* best_variant, int* best_ch, do
     return QSR_OK;
 }
 
+int qsr_set_box_active(void* ctx, int box, int active) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || box < 0 || box >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad argument");
+    Box& b = c->boxes[size_t(box)];
+    const bool on = active != 0;
+    if (b.active == on) return QSR_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    if (!on) {
+        // both ping-pong buffers hold the held state, whichever one later steps leave current
+        for (int comp = 0; comp < 5; ++comp) {
+            const size_t o = size_t(comp) * size_t(c->cs) + size_t(b.off);
+            HIP_TRY(hipMemcpyAsync(c->q[1 - c->cur] + o, c->q[c->cur] + o, sizeof(float) * size_t(b.nvox),
+                                   hipMemcpyDeviceToDevice, c->st));
+        }
+    }
+    b.active = on;
+    qsrk::BoxDev& d = c->h_boxes[size_t(box)];
+    d.flags = on ? (d.flags & ~qsrk::F_INACTIVE) : (d.flags | qsrk::F_INACTIVE);
+    HIP_TRY(hipMemcpyAsync(c->d_boxes + box, &d, sizeof(qsrk::BoxDev), hipMemcpyHostToDevice, c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    if (int rc = build_tiles(*c)) return rc;
+    if (c->gexec) discard(hipGraphExecDestroy(c->gexec));
+    if (c->graph) discard(hipGraphDestroy(c->graph));
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    return QSR_OK;
+}
+
 int qsr_set_tuning(void* ctx, int variant, int planes_per_tile) {
     Ctx* c = as_ctx(ctx);
     if (!c || variant < 0 || variant > 1 || planes_per_tile < 1 || planes_per_tile > 4096)
@@ -713,7 +771,7 @@ int qsr_step_imex(void* ctx, int nsteps, double L_stab, int jacobi_iters) {
                                           c->d_boxes, c->d_tiles, dyn, nb, c->d_step_rows, c->ch, c->d_imex));
             int src = c->cur;
             HIP_TRY(qsrk::launch_face_copy(c->nface, c->st, c->q[src], c->q[1 - src], c->cs, c->d_boxes,
-                                           c->d_face_prefix, nb, c->d_step_rows));
+                                           c->d_face_list));
             for (int it = 0; it < jacobi_iters; ++it) {
                 if (c->ntiles > 0)
                     HIP_TRY(qsrk::launch_jacobi(c->ntiles, c->st, c->q_rhs, c->q[src], c->q[1 - src], c->cs,

@@ -18,6 +18,7 @@ enum : int {
     F_CAP = 32,       // Frobenius-norm cap (QSR.cu:721-731)
     F_L2 = 64,
     F_L3 = 128,
+    F_INACTIVE = 256,  // box held (qsr_set_box_active): no tiles, faces skipped
 };
 
 // Static description of one box, FP32 coefficients pre-multiplied by the
@@ -87,16 +88,16 @@ namespace qsrk {
 hipError_t launch_step(bool full, int variant, bool noise, int ntiles, hipStream_t st, const float* qin, float* qout,
                        long long cs, const BoxDev* boxes, const int4* tiles, const StepDyn* dyn, int nboxes,
                        const RowMask* rows, int ch, const float4* imex);
-hipError_t launch_faces(long long nface, hipStream_t st, const float* qin, float* qout, long long cs,
-                        const BoxDev* boxes, const StepDyn* dyn, const long long* face_prefix, int nboxes,
-                        const RowMask* rows);
+// face_list: (box, i, j, k) of every box-face voxel that can change (built on the host)
+hipError_t launch_faces(long long nlive, hipStream_t st, const float* qin, float* qout, long long cs,
+                        const BoxDev* boxes, const StepDyn* dyn, const int4* face_list, const RowMask* rows);
 // One semi-implicit sweep over the step tiles.  flags: 2 x nsweeps words, zero
 // before the first step; row alternates between steps.
 hipError_t launch_jacobi(int ntiles, hipStream_t st, const float* rhs, const float* qold, float* qnew, long long cs,
                          const BoxDev* boxes, const int4* tiles, const RowMask* rows, int ch, const float4* coef,
                          unsigned* flags, int sweep, int nsweeps, int row);
-hipError_t launch_face_copy(long long nface, hipStream_t st, const float* src, float* dst, long long cs,
-                            const BoxDev* boxes, const long long* face_prefix, int nboxes, const RowMask* rows);
+hipError_t launch_face_copy(long long nlive, hipStream_t st, const float* src, float* dst, long long cs,
+                            const BoxDev* boxes, const int4* face_list);
 hipError_t launch_bc(long long nvox, hipStream_t st, float* q, long long cs, const BoxDev* boxes, const StepDyn* dyn,
                      const long long* vox_prefix, int nboxes, const RowMask* rows);
 hipError_t launch_aos_to_soa(hipStream_t st, const double* aos, float* q0, float* q1, long long cs, long long off,

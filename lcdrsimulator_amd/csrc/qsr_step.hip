@@ -759,46 +759,21 @@ struct FaceCtx {
     }
 };
 
-__device__ __forceinline__ void face_decode(long long f, int Nx, int Ny, int Nz, int& i, int& j, int& k) {
-    const long long a = (long long)Ny * Nz, b = (long long)(Nx - 2) * Nz, c = (long long)(Nx - 2) * (Ny - 2);
-    if (f < 2 * a) {
-        i = f < a ? 0 : Nx - 1;
-        const long long r = f % a;
-        j = int(r / Nz);
-        k = int(r % Nz);
-        return;
-    }
-    f -= 2 * a;
-    if (f < 2 * b) {
-        j = f < b ? 0 : Ny - 1;
-        const long long r = f % b;
-        i = 1 + int(r / Nz);
-        k = int(r % Nz);
-        return;
-    }
-    f -= 2 * b;
-    k = f < c ? 0 : Nz - 1;
-    const long long r = f % c;
-    i = 1 + int(r / (Ny - 2));
-    j = 1 + int(r % (Ny - 2));
-}
 
 }  // namespace
 
 __global__ __launch_bounds__(256) void step_face_kernel(const float* __restrict__ qin, float* __restrict__ qout,
                                                         long long cs, const BoxDev* __restrict__ boxes,
                                                         const StepDyn* __restrict__ dyn,
-                                                        const long long* __restrict__ face_prefix, int nboxes,
+                                                        const int4* __restrict__ face_list, long long nlive,
                                                         const RowMask* __restrict__ rows) {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= face_prefix[nboxes]) return;
-    int b = 0;
-    while (b + 1 < nboxes && face_prefix[b + 1] <= g) ++b;
+    if (g >= nlive) return;
+    const int4 e = face_list[g];  // (box, i, j, k) of a face voxel that can change
+    const int b = e.x, i = e.y, j = e.z, k = e.w;
     const BoxDev B = boxes[b];
-    if (B.flags & F_PERIODIC) return;
+    if (B.flags & (F_PERIODIC | F_INACTIVE)) return;
     const StepDyn D = dyn[b];
-    int i, j, k;
-    face_decode(g - face_prefix[b], B.Nx, B.Ny, B.Nz, i, j, k);
     bool evolve, shell;
     decode_mask(rows, B, i, j, k, evolve, shell);
     const bool live = (evolve && !((B.flags & F_SKIPSHELL) && shell)) || ((B.flags & F_BC) && shell);
@@ -1019,20 +994,13 @@ __global__ __launch_bounds__(TK* TJ) void jacobi_kernel(const float* __restrict_
 // buffers hold the step's input there (the semi-implicit sweeps keep faces).
 __global__ __launch_bounds__(256) void face_copy_kernel(const float* __restrict__ src, float* __restrict__ dst,
                                                         long long cs, const BoxDev* __restrict__ boxes,
-                                                        const long long* __restrict__ face_prefix, int nboxes,
-                                                        const RowMask* __restrict__ rows) {
+                                                        const int4* __restrict__ face_list, long long nlive) {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= face_prefix[nboxes]) return;
-    int b = 0;
-    while (b + 1 < nboxes && face_prefix[b + 1] <= g) ++b;
-    const BoxDev B = boxes[b];
-    if (B.flags & F_PERIODIC) return;
-    int i, j, k;
-    face_decode(g - face_prefix[b], B.Nx, B.Ny, B.Nz, i, j, k);
-    bool evolve, shell;
-    decode_mask(rows, B, i, j, k, evolve, shell);
-    if (!(evolve || ((B.flags & F_BC) && shell))) return;
-    const long long v = ((long long)i * B.Ny + j) * B.Nz + k;
+    if (g >= nlive) return;
+    const int4 e = face_list[g];
+    const BoxDev B = boxes[e.x];
+    if (B.flags & (F_PERIODIC | F_INACTIVE)) return;
+    const long long v = ((long long)e.y * B.Ny + e.z) * B.Nz + e.w;
 #pragma unroll
     for (int c = 0; c < 5; ++c) dst[c * cs + B.off + v] = src[c * cs + B.off + v];
 }
@@ -1146,12 +1114,11 @@ hipError_t launch_step(bool full, int variant, bool noise, int ntiles, hipStream
     return hipGetLastError();
 }
 
-hipError_t launch_faces(long long nface, hipStream_t st, const float* qin, float* qout, long long cs,
-                        const BoxDev* boxes, const StepDyn* dyn, const long long* face_prefix, int nboxes,
-                        const RowMask* rows) {
-    if (nface <= 0) return hipSuccess;
-    const dim3 grid(unsigned((nface + 255) / 256)), block(256);
-    hipLaunchKernelGGL(step_face_kernel, grid, block, 0, st, qin, qout, cs, boxes, dyn, face_prefix, nboxes, rows);
+hipError_t launch_faces(long long nlive, hipStream_t st, const float* qin, float* qout, long long cs,
+                        const BoxDev* boxes, const StepDyn* dyn, const int4* face_list, const RowMask* rows) {
+    if (nlive <= 0) return hipSuccess;
+    const dim3 grid(unsigned((nlive + 255) / 256)), block(256);
+    hipLaunchKernelGGL(step_face_kernel, grid, block, 0, st, qin, qout, cs, boxes, dyn, face_list, nlive, rows);
     return hipGetLastError();
 }
 
@@ -1167,11 +1134,11 @@ hipError_t launch_jacobi(int ntiles, hipStream_t st, const float* rhs, const flo
                            rows, ch, coef, flags, sweep, nsweeps, row);
     return hipGetLastError();
 }
-hipError_t launch_face_copy(long long nface, hipStream_t st, const float* src, float* dst, long long cs,
-                            const BoxDev* boxes, const long long* face_prefix, int nboxes, const RowMask* rows) {
-    if (nface <= 0) return hipSuccess;
-    hipLaunchKernelGGL(face_copy_kernel, dim3(unsigned((nface + 255) / 256)), dim3(256), 0, st, src, dst, cs, boxes,
-                       face_prefix, nboxes, rows);
+hipError_t launch_face_copy(long long nlive, hipStream_t st, const float* src, float* dst, long long cs,
+                            const BoxDev* boxes, const int4* face_list) {
+    if (nlive <= 0) return hipSuccess;
+    hipLaunchKernelGGL(face_copy_kernel, dim3(unsigned((nlive + 255) / 256)), dim3(256), 0, st, src, dst, cs, boxes,
+                       face_list, nlive);
     return hipGetLastError();
 }
 

@@ -73,7 +73,7 @@ EXPORTS = (
     "qsr_upload_q", "qsr_download_q", "qsr_download_nematic", "qsr_step", "qsr_step_graph",
     "qsr_step_imex", "qsr_observables", "qsr_box_info_get", "qsr_set_iteration", "qsr_last_step_ms",
     "qsr_step_work", "qsr_profile_steps", "qsr_autotune", "qsr_set_tuning", "qsr_S_eq", "qsr_dt_estimate", "qsr_init_field",
-    "qsr_load_qtensor_file", "qsr_save_qtensor_file",
+    "qsr_load_qtensor_file", "qsr_save_qtensor_file", "qsr_set_box_active",
 )
 
 
@@ -103,6 +103,7 @@ def load():
     L.qsr_profile_steps.argtypes = [vp, C.c_int, dp, dp]
     L.qsr_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), dp]
     L.qsr_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    L.qsr_set_box_active.argtypes = [vp, C.c_int, C.c_int]
     L.qsr_step_work.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.qsr_S_eq.restype = C.c_double
     L.qsr_S_eq.argtypes = [C.c_double] * 5 + [C.c_int]
@@ -279,6 +280,10 @@ class Solver:
 
     def set_tuning(self, variant: int, planes_per_tile: int):
         _check(load().qsr_set_tuning(self._h, int(variant), int(planes_per_tile)))
+
+    def set_active(self, box: int, active: bool):
+        """Hold (False) or resume (True) one box of a batch: held boxes are not stepped."""
+        _check(load().qsr_set_box_active(self._h, int(box), int(bool(active))))
 
     def work(self) -> dict:
         a, t, b = C.c_uint64(), C.c_uint64(), C.c_uint64()
