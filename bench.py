@@ -86,10 +86,10 @@ def workload_params(q, n=256, R=120.0, workload="droplet_l1l2l3_256"):
     return p
 
 
-def measured_traffic(workload, variant, planes):
-    """HBM bytes per fused-stencil launch from the committed counter summaries
-    (FETCH_SIZE and WRITE_SIZE passes, each scaled by a plain-copy calibration),
-    for the same workload and tuning; None if that combination was not profiled."""
+def profiled(tag, key, workload, variant, planes):
+    """A figure recorded in the committed counter summaries under profiles/ as a line
+    `<tag> <workload> variant=V planes=P <key>=X`, for the same workload and tuning;
+    (None, None) if that combination was not profiled."""
     pdir = os.path.join(ROOT, "profiles")
     if not os.path.isdir(pdir):
         return None, None
@@ -97,13 +97,19 @@ def measured_traffic(workload, variant, planes):
         if not fn.endswith(".md"):
             continue
         for line in open(os.path.join(pdir, fn)):
-            if not line.startswith("bench-traffic:"):
+            if not line.startswith(tag):
                 continue
             f = line.split()
             kv = dict(x.split("=", 1) for x in f[2:])
             if f[1] == workload and int(kv["variant"]) == variant and int(kv["planes"]) == planes:
-                return float(kv["bytes"]), "profiles/" + fn
+                return float(kv[key]), "profiles/" + fn
     return None, None
+
+
+def measured_traffic(workload, variant, planes):
+    """HBM bytes per fused-stencil launch (FETCH_SIZE and WRITE_SIZE passes, each scaled by a
+    plain-copy calibration)."""
+    return profiled("bench-traffic:", "bytes", workload, variant, planes)
 
 
 def cpu_baseline(p, budget_s=15.0):
@@ -298,6 +304,7 @@ def main():
     traffic_bytes = BYTES_PER_VOXEL * work["tile_voxels"]
     achieved = traffic_bytes / (main_ms * 1e-3) / 1e9
     hbm_bytes, hbm_src = measured_traffic(args.workload, int(tune["variant"]), int(tune["planes_per_tile"]))
+    valu_busy, _ = profiled("bench-valu:", "busy", args.workload, int(tune["variant"]), int(tune["planes_per_tile"]))
     cpu = None if (args.no_cpu or args.workload != WORKLOADS[0]) else cpu_baseline(p)
     out = {
         "metric": METRIC, "value": value, "unit": "voxel-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -310,7 +317,7 @@ def main():
                    "droplet_voxel_updates_per_s": work["active_voxels"] * args.steps / wall,
                    "fused_stencil_ms": main_ms, "face_ms": face_ms, "event_ms_per_step": ev_ms,
                    "final_total_F": obs["total"], "algorithmic_bytes_per_launch": traffic_bytes,
-                   "traffic_source": hbm_src},
+                   "traffic_source": hbm_src, "valu_busy_frac": valu_busy},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": hbm_bytes},
         "cpu_baseline": cpu,
