@@ -82,6 +82,7 @@ struct Ctx {
     float* q[2] = {nullptr, nullptr};
     int cur = 0;
     bool full = false, noise = false;
+    int elastic = 0;  // union of the boxes' anisotropic terms: 1 = L2, 2 = L3
     int ch = 32;
     int variant = 0;  // stencil kernel variant (waves/SIMD target), chosen by qsr_autotune
     std::vector<qsrk::BoxDev> h_boxes;
@@ -345,6 +346,8 @@ int create_impl(const qsr_params* params, int n, int device, void** out) {
         b.last_T = b.T_at(0);
         b.last_S = b.S_at(b.last_T);
         if (p.L2 != 0.0 || p.L3 != 0.0) c->full = true;
+        if (p.L2 != 0.0) c->elastic |= 1;
+        if (p.L3 != 0.0) c->elastic |= 2;
         if (p.noise_sigma > 0.0) c->noise = true;
     }
     c->cs = std::max<long long>(off, 64);
@@ -404,7 +407,7 @@ int enqueue_steps(Ctx& c, int nsteps) {
         float* qout = c.q[1 - c.cur];
         const qsrk::StepDyn* dyn = c.d_dyn + (size_t)s * nb;
         if (c.ntiles > 0)
-            HIP_TRY(qsrk::launch_step(c.full, c.variant, c.noise, c.ntiles, c.st, qin, qout, c.cs, c.d_boxes, c.d_tiles, dyn, nb,
+            HIP_TRY(qsrk::launch_step(c.elastic, c.variant, c.noise, c.ntiles, c.st, qin, qout, c.cs, c.d_boxes, c.d_tiles, dyn, nb,
                                       c.d_step_rows, c.ch, nullptr));
         HIP_TRY(qsrk::launch_faces(c.nface, c.st, qin, qout, c.cs, c.d_boxes, dyn, c.d_face_list, c.d_step_rows));
         c.cur = 1 - c.cur;
@@ -601,7 +604,7 @@ int qsr_profile_steps(void* ctx, int nsteps, double* main_ms, double* face_ms) {
         const qsrk::StepDyn* dyn = c->d_dyn + (size_t)s * nb;
         HIP_TRY(hipEventRecord(ev[3 * s], c->st));
         if (c->ntiles > 0)
-            HIP_TRY(qsrk::launch_step(c->full, c->variant, c->noise, c->ntiles, c->st, qin, qout, c->cs, c->d_boxes, c->d_tiles,
+            HIP_TRY(qsrk::launch_step(c->elastic, c->variant, c->noise, c->ntiles, c->st, qin, qout, c->cs, c->d_boxes, c->d_tiles,
                                       dyn, nb, c->d_step_rows, c->ch, nullptr));
         HIP_TRY(hipEventRecord(ev[3 * s + 1], c->st));
         HIP_TRY(qsrk::launch_faces(c->nface, c->st, qin, qout, c->cs, c->d_boxes, dyn, c->d_face_list,
@@ -648,7 +651,7 @@ int qsr_autotune(void* ctx, int trial_steps, int* best_variant, int* best_ch, do
             for (int s = -1; s < trial_steps; ++s) {
                 if (s == 0) HIP_TRY(hipEventRecord(c->ev[0], c->st));
                 if (c->ntiles > 0)
-                    HIP_TRY(qsrk::launch_step(c->full, c->variant, c->noise, c->ntiles, c->st, c->q[c->cur],
+                    HIP_TRY(qsrk::launch_step(c->elastic, c->variant, c->noise, c->ntiles, c->st, c->q[c->cur],
                                               c->q[1 - c->cur], c->cs, c->d_boxes, c->d_tiles, c->d_dyn, nb,
                                               c->d_step_rows, c->ch, nullptr));
             }
@@ -767,7 +770,7 @@ int qsr_step_imex(void* ctx, int nsteps, double L_stab, int jacobi_iters) {
         for (int s = 0; s < n; ++s) {
             const qsrk::StepDyn* dyn = c->d_dyn + (size_t)s * nb;
             if (c->ntiles > 0)
-                HIP_TRY(qsrk::launch_step(c->full, c->variant, false, c->ntiles, c->st, c->q[c->cur], c->q_rhs, c->cs,
+                HIP_TRY(qsrk::launch_step(c->elastic, c->variant, false, c->ntiles, c->st, c->q[c->cur], c->q_rhs, c->cs,
                                           c->d_boxes, c->d_tiles, dyn, nb, c->d_step_rows, c->ch, c->d_imex));
             int src = c->cur;
             HIP_TRY(qsrk::launch_face_copy(c->nface, c->st, c->q[src], c->q[1 - src], c->cs, c->d_boxes,

@@ -85,7 +85,9 @@ constexpr int NPLAQ = 3;     // periodic plaquette slots
 
 // Host launchers (defined in qsr_step.hip / qsr_obs.hip)
 namespace qsrk {
-hipError_t launch_step(bool full, int variant, bool noise, int ntiles, hipStream_t st, const float* qin, float* qout,
+// elastic: 0 = L1/kappa stencil; otherwise the anisotropic stencil with the terms
+// of the mask compiled in (1 = L2, 2 = L3, 3 = both)
+hipError_t launch_step(int elastic, int variant, bool noise, int ntiles, hipStream_t st, const float* qin, float* qout,
                        long long cs, const BoxDev* boxes, const int4* tiles, const StepDyn* dyn, int nboxes,
                        const RowMask* rows, int ch, const float4* imex);
 // face_list: (box, i, j, k) of every box-face voxel that can change (built on the host)

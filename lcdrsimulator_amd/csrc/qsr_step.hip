@@ -398,7 +398,10 @@ struct AnisoGeom {
 
 }  // namespace
 
-template <int TJv, int MINW, bool NOISE, bool BUF>
+// EL: elastic terms compiled in (1 = L2, 2 = L3, 3 = both), the union over the
+// context's boxes; a box without a term has a zero coefficient for it, which
+// adds exactly nothing, so the whole molecular field is one basic block.
+template <int TJv, int MINW, bool NOISE, bool BUF, int EL>
 __global__ __launch_bounds__(TK* TJv, MINW) void step_aniso_kernel(const float* __restrict__ qin, float* __restrict__ qout,
                                                              long long cs, const BoxDev* __restrict__ boxes,
                                                              const int4* __restrict__ tiles,
@@ -428,7 +431,7 @@ __global__ __launch_bounds__(TK* TJv, MINW) void step_aniso_kernel(const float* 
     const bool out_ok = per ? (j < Ny && k < Nz) : (j >= 1 && j <= Ny - 2 && k >= 1 && k <= Nz - 2);
     const long long plane = (long long)Ny * Nz;
     const Field<BUF> F(qin, qout, cs, B.off);
-    const bool hasL2 = (B.flags & F_L2) != 0, hasL3 = (B.flags & F_L3) != 0;
+    constexpr bool hasL2 = (EL & 1) != 0, hasL3 = (EL & 2) != 0;
     auto face = [&](int x, int n) { return !per && (x == 0 || x == n - 1); };
 
     // own column
@@ -631,7 +634,7 @@ __global__ __launch_bounds__(TK* TJv, MINW) void step_aniso_kernel(const float* 
                 bulk_field(q1, D.cA, B.cB, B.cC, h);
 #pragma unroll
                 for (int c = 0; c < 5; ++c) h[c] -= B.lx * sx_[c] + B.ly * syy1[c] + B.lz * szz1[c];
-                if (hasL2) {
+                if constexpr (hasL2) {
                     float dyp[3], dym[3], dzp[3], dzm[3];
 #pragma unroll
                     for (int a = 0; a < 3; ++a) {
@@ -651,7 +654,7 @@ __global__ __launch_bounds__(TK* TJv, MINW) void step_aniso_kernel(const float* 
                     h[2] -= B.mL2 * (dDz_dx + dDx_dz);
                     h[4] -= B.mL2 * (dDz_dy + dDy_dz);
                 }
-                if (hasL3) {
+                if constexpr (hasL3) {
                     // d[c][axis] at plane t (interior: central differences)
                     float d[6][3];
 #pragma unroll
@@ -1059,58 +1062,59 @@ QSR_INST_L1(true, false)
 QSR_INST_L1(false, true)
 QSR_INST_L1(true, true)
 #undef QSR_INST_L1
-#define QSR_INST_ANISO(MINW, NZ, BF)                                                                        \
-    template __global__ void step_aniso_kernel<TJ, MINW, NZ, BF>(const float*, float*, long long, const BoxDev*, \
-                                                                 const int4*, const StepDyn*, int, const RowMask*, int, \
-                                                                 const float4*);
-QSR_INST_ANISO(3, false, false)
-QSR_INST_ANISO(3, true, false)
-QSR_INST_ANISO(4, false, false)
-QSR_INST_ANISO(4, true, false)
-QSR_INST_ANISO(3, false, true)
-QSR_INST_ANISO(3, true, true)
-QSR_INST_ANISO(4, false, true)
-QSR_INST_ANISO(4, true, true)
-#undef QSR_INST_ANISO
 
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 namespace qsrk {
 
-hipError_t launch_step(bool full, int variant, bool noise, int ntiles, hipStream_t st, const float* qin, float* qout,
+namespace {
+
+template <int EL, int MINW, bool NZ, bool BF>
+void go_aniso(dim3 grid, dim3 block, hipStream_t st, const float* qin, float* qout, long long cs, const BoxDev* boxes,
+              const int4* tiles, const StepDyn* dyn, int nboxes, const RowMask* rows, int ch, const float4* imex) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(step_aniso_kernel<TJ, MINW, NZ, BF, EL>), grid, block, 0, st, qin, qout, cs,
+                       boxes, tiles, dyn, nboxes, rows, ch, imex);
+}
+
+template <int EL>
+void go_aniso_el(int variant, bool noise, bool buf, dim3 grid, dim3 block, hipStream_t st, const float* qin,
+                 float* qout, long long cs, const BoxDev* boxes, const int4* tiles, const StepDyn* dyn, int nboxes,
+                 const RowMask* rows, int ch, const float4* imex) {
+#define QSR_A(MW, NZ, BF) go_aniso<EL, MW, NZ, BF>(grid, block, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex)
+    if (variant == 1) {
+        if (buf) { if (noise) QSR_A(4, true, true); else QSR_A(4, false, true); }
+        else { if (noise) QSR_A(4, true, false); else QSR_A(4, false, false); }
+    } else {
+        if (buf) { if (noise) QSR_A(3, true, true); else QSR_A(3, false, true); }
+        else { if (noise) QSR_A(3, true, false); else QSR_A(3, false, false); }
+    }
+#undef QSR_A
+}
+
+}  // namespace
+
+hipError_t launch_step(int elastic, int variant, bool noise, int ntiles, hipStream_t st, const float* qin, float* qout,
                        long long cs, const BoxDev* boxes, const int4* tiles, const StepDyn* dyn, int nboxes,
                        const RowMask* rows, int ch, const float4* imex) {
     const dim3 grid(ntiles), block(TK * TJ);
-#define QSR_GO(...) hipLaunchKernelGGL(HIP_KERNEL_NAME(__VA_ARGS__), grid, block, 0, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex)
     // 32-bit buffer offsets reach every element when 5 planes x cs floats fit 4 GiB
     const bool buf = cs < (1LL << 32) / 20;
-    if (full) {
-        if (variant == 1) {
+    switch (elastic) {
+        case 1: go_aniso_el<1>(variant, noise, buf, grid, block, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex); break;
+        case 2: go_aniso_el<2>(variant, noise, buf, grid, block, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex); break;
+        case 3: go_aniso_el<3>(variant, noise, buf, grid, block, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex); break;
+        default:
+#define QSR_GO(...) hipLaunchKernelGGL(HIP_KERNEL_NAME(__VA_ARGS__), grid, block, 0, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex)
             if (buf) {
-                if (noise) QSR_GO(step_aniso_kernel<TJ, 4, true, true>);
-                else QSR_GO(step_aniso_kernel<TJ, 4, false, true>);
+                if (noise) QSR_GO(step_l1_kernel<true, true>);
+                else QSR_GO(step_l1_kernel<false, true>);
             } else {
-                if (noise) QSR_GO(step_aniso_kernel<TJ, 4, true, false>);
-                else QSR_GO(step_aniso_kernel<TJ, 4, false, false>);
+                if (noise) QSR_GO(step_l1_kernel<true, false>);
+                else QSR_GO(step_l1_kernel<false, false>);
             }
-        } else {
-            if (buf) {
-                if (noise) QSR_GO(step_aniso_kernel<TJ, 3, true, true>);
-                else QSR_GO(step_aniso_kernel<TJ, 3, false, true>);
-            } else {
-                if (noise) QSR_GO(step_aniso_kernel<TJ, 3, true, false>);
-                else QSR_GO(step_aniso_kernel<TJ, 3, false, false>);
-            }
-        }
-    } else if (buf) {
-        if (noise) QSR_GO(step_l1_kernel<true, true>);
-        else QSR_GO(step_l1_kernel<false, true>);
-    } else {
-        if (noise) QSR_GO(step_l1_kernel<true, false>);
-        else QSR_GO(step_l1_kernel<false, false>);
-    }
 #undef QSR_GO
+    }
     return hipGetLastError();
 }
 
