@@ -8,6 +8,8 @@
 // restatement.  Block partial sums use DPP wave reductions and are combined in
 // a fixed order, so results are run-to-run deterministic.
 #pragma clang fp contract(off)
+#include <algorithm>
+
 #include "qsr_kernels.hpp"
 
 using namespace qsrk;
@@ -83,7 +85,8 @@ __device__ void nematic(const double q5[5], bool kzm, double out[4]) {
     double nx = xx + xy + xz, ny = xy + yy + yz, nz = xz + yz - xx - yy;
     const double n0 = sqrt(nx * nx + ny * ny + nz * nz);
     if (n0 > 1e-12) {
-        nx /= n0; ny /= n0; nz /= n0;
+        const double inv = 1.0 / n0;  // one divide, three products
+        nx *= inv; ny *= inv; nz *= inv;
     } else {
         nx = 0.0; ny = 0.0; nz = 1.0;
     }
@@ -97,7 +100,8 @@ __device__ void nematic(const double q5[5], bool kzm, double out[4]) {
         const double az = xz * nx + yz * ny + zz * nz + alpha * nz;
         const double nrm = sqrt(ax * ax + ay * ay + az * az);
         if (nrm < 1e-13) break;
-        nx = ax / nrm; ny = ay / nrm; nz = az / nrm;
+        const double inv = 1.0 / nrm;
+        nx = ax * inv; ny = ay * inv; nz = az * inv;
     }
     const double lam = nx * (xx * nx + xy * ny + xz * nz) + ny * (xy * nx + yy * ny + yz * nz) +
                        nz * (xz * nx + yz * ny + zz * nz);
@@ -139,9 +143,10 @@ __global__ __launch_bounds__(256) void obs_droplet_kernel(const float* __restric
                                                           const RowMask* __restrict__ obs_rows,
                                                           double* __restrict__ partials) {
     const long long n = (long long)P.Nx * P.Ny * P.Nz;
-    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double acc[NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (v < n) {
+    // grid-stride: each thread accumulates its voxels in a fixed order (deterministic sums)
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
         const int k = int(v % P.Nz), j = int((v / P.Nz) % P.Ny), i = int(v / ((long long)P.Ny * P.Nz));
         const float* base = q + P.off;
         double Q[5];
@@ -157,7 +162,7 @@ __global__ __launch_bounds__(256) void obs_droplet_kernel(const float* __restric
             const double q2zz = xz * xz + yz * yz + zz * zz;
             const double tr3 = xx * q2xx + xy * q2yx + xz * q2zx + xy * q2xy + yy * q2yy + yz * q2zy + xz * q2xz +
                                yz * q2yz + zz * q2zz;
-            acc[0] = (P.c2 * tr2 - P.c3 * tr3 + P.c4 * tr2 * tr2) * dV;
+            acc[0] += (P.c2 * tr2 - P.c3 * tr3 + P.c4 * tr2 * tr2) * dV;
             const long long sx = (long long)P.Ny * P.Nz, sy = P.Nz;
             const double i2x = 1.0 / (2.0 * P.dx), i2y = 1.0 / (2.0 * P.dy), i2z = 1.0 / (2.0 * P.dz);
             double d[6][3];
@@ -191,7 +196,7 @@ __global__ __launch_bounds__(256) void obs_droplet_kernel(const float* __restric
                 }
                 fel = f1 + f2 + f3;
             }
-            acc[1] = fel * dV;
+            acc[1] += fel * dV;
         }
         const RowMask sm = step_rows[P.rows_off + (long long)i * P.Ny + j];
         const RowMask om = obs_rows[P.rows_off + (long long)i * P.Ny + j];
@@ -215,7 +220,7 @@ __global__ __launch_bounds__(256) void obs_droplet_kernel(const float* __restric
             const double a = xx - q0xx, b = xy - q0xy, c = xz - q0xz, e = yy - q0yy, f = yz - q0yz;
             const double zd = (-(xx + yy)) - q0zz;
             const double diff2 = a * a + e * e + zd * zd + 2.0 * (b * b + c * c + f * f);
-            acc[2] = 0.5 * P.W_over_delta * diff2 * dV;
+            acc[2] += 0.5 * P.W_over_delta * diff2 * dV;
         }
         const bool inA = in_iv(k, om.v[0], om.v[1]);
         const bool inRad = in_iv(k, om.v[2], om.v[3]) && !in_iv(k, om.v[4], om.v[5]);
@@ -223,16 +228,16 @@ __global__ __launch_bounds__(256) void obs_droplet_kernel(const float* __restric
             double nf[4];
             nematic(Q, false, nf);
             if (inA) {
-                acc[3] = nf[0];
-                acc[4] = 1.0;
-                if (isfinite(nf[0]) && nf[0] > 0.0) acc[7] = nf[0];
+                acc[3] += nf[0];
+                acc[4] += 1.0;
+                if (isfinite(nf[0]) && nf[0] > acc[7]) acc[7] = nf[0];
             }
             if (inRad) {
                 const double r = sqrt(x * x + y * y + z * z);
                 double rx = 1.0, ry = 0.0, rz = 0.0;
                 if (r > 1e-30) { rx = x / r; ry = y / r; rz = z / r; }
-                acc[5] = fabs(nf[1] * rx + nf[2] * ry + nf[3] * rz);
-                acc[6] = 1.0;
+                acc[5] += fabs(nf[1] * rx + nf[2] * ry + nf[3] * rz);
+                acc[6] += 1.0;
             }
         }
     }
@@ -249,9 +254,9 @@ __global__ __launch_bounds__(256) void obs_droplet_kernel(const float* __restric
 __global__ __launch_bounds__(256) void obs_periodic_kernel(const float* __restrict__ q, long long cs, ObsParams P,
                                                            double* __restrict__ nf_out, double* __restrict__ partials) {
     const long long n = (long long)P.Nx * P.Ny * P.Nz;
-    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double acc[NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (v < n) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
         const int k = int(v % P.Nz), j = int((v / P.Nz) % P.Ny), i = int(v / ((long long)P.Ny * P.Nz));
         const float* base = q + P.off;
         double Q[5];
@@ -267,7 +272,7 @@ __global__ __launch_bounds__(256) void obs_periodic_kernel(const float* __restri
         const double tr3 = xx * q2xx + xy * q2xy + xz * q2xz + xy * q2xy + yy * q2yy + yz * q2yz + xz * q2xz +
                            yz * q2yz + zz * q2zz;
         const double dV = P.dx * P.dy * P.dz;
-        acc[0] = (P.c2 * tr2 - P.c3 * tr3 + P.c4 * tr2 * tr2) * dV;
+        acc[0] += (P.c2 * tr2 - P.c3 * tr3 + P.c4 * tr2 * tr2) * dV;
         double d[6][3];
         for (int c = 0; c < 5; ++c) {
             const float* b = base + c * cs;
@@ -280,15 +285,15 @@ __global__ __launch_bounds__(256) void obs_periodic_kernel(const float* __restri
         for (int a = 0; a < 3; ++a) g2 += d[0][a] * d[0][a] + d[3][a] * d[3][a] + d[5][a] * d[5][a];
         for (int a = 0; a < 3; ++a) g2o += d[1][a] * d[1][a] + d[2][a] * d[2][a] + d[4][a] * d[4][a];
         g2 += 2.0 * g2o;
-        acc[1] = 0.5 * P.kappa * g2 * dV;
-        acc[2] = tr2 * dV;
-        acc[3] = g2 * dV;
+        acc[1] += 0.5 * P.kappa * g2 * dV;
+        acc[2] += tr2 * dV;
+        acc[3] += g2 * dV;
         double nf[4];
         nematic(Q, true, nf);
         for (int c = 0; c < 4; ++c) nf_out[4 * v + c] = nf[c];
-        acc[4] = nf[0];
-        acc[5] = 1.0;
-        acc[7] = nf[0] > 0.0 ? nf[0] : 0.0;
+        acc[4] += nf[0];
+        acc[5] += 1.0;
+        if (nf[0] > acc[7]) acc[7] = nf[0];
     }
     const bool is_max[NPART] = {false, false, false, false, false, false, false, true};
     block_reduce_store<NPART>(acc, is_max, partials + (long long)blockIdx.x * NPART);
@@ -319,9 +324,9 @@ __device__ int plaquette3(const double* a, const double* b, const double* c, con
 __global__ __launch_bounds__(256) void obs_plaquette3_kernel(const double* __restrict__ nf, ObsParams P, double thr,
                                                              double cutoff, double* __restrict__ partials) {
     const long long n = (long long)P.Nx * P.Ny * P.Nz;
-    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double acc[NPLAQ] = {0, 0, 0};
-    if (v < n) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
         const int k = int(v % P.Nz), j = int((v / P.Nz) % P.Ny), i = int(v / ((long long)P.Ny * P.Nz));
         const int ip = (i + 1) % P.Nx, jp = (j + 1) % P.Ny, kp = (k + 1) % P.Nz;
         auto at = [&](int a, int b, int c) { return nf + 4 * (((long long)a * P.Ny + b) * P.Nz + c); };
@@ -460,28 +465,31 @@ __global__ __launch_bounds__(256) void nematic_aos_kernel(const float* __restric
 namespace qsro {
 
 static unsigned blocks_for(long long n) { return unsigned((n + 255) / 256); }
+// grid of the volume passes (grid-stride loops): enough blocks to fill the chip, few partials
+static unsigned volume_blocks(long long n) { return std::min(blocks_for(n), 4096u); }
 
 hipError_t launch_obs_droplet(hipStream_t st, const float* q, long long cs, const ObsParams& P,
                               const RowMask* step_rows, const RowMask* obs_rows, double* partials, long long* nblk) {
     const long long n = (long long)P.Nx * P.Ny * P.Nz;
-    *nblk = blocks_for(n);
-    hipLaunchKernelGGL(obs_droplet_kernel, dim3(blocks_for(n)), dim3(256), 0, st, q, cs, P, step_rows, obs_rows, partials);
+    *nblk = volume_blocks(n);
+    hipLaunchKernelGGL(obs_droplet_kernel, dim3(volume_blocks(n)), dim3(256), 0, st, q, cs, P, step_rows, obs_rows,
+                       partials);
     return hipGetLastError();
 }
 
 hipError_t launch_obs_periodic(hipStream_t st, const float* q, long long cs, const ObsParams& P, double* nf,
                                double* partials, long long* nblk) {
     const long long n = (long long)P.Nx * P.Ny * P.Nz;
-    *nblk = blocks_for(n);
-    hipLaunchKernelGGL(obs_periodic_kernel, dim3(blocks_for(n)), dim3(256), 0, st, q, cs, P, nf, partials);
+    *nblk = volume_blocks(n);
+    hipLaunchKernelGGL(obs_periodic_kernel, dim3(volume_blocks(n)), dim3(256), 0, st, q, cs, P, nf, partials);
     return hipGetLastError();
 }
 
 hipError_t launch_obs_plaquette3(hipStream_t st, const double* nf, const ObsParams& P, double thr, double cutoff,
                                  double* partials, long long* nblk) {
     const long long n = (long long)P.Nx * P.Ny * P.Nz;
-    *nblk = blocks_for(n);
-    hipLaunchKernelGGL(obs_plaquette3_kernel, dim3(blocks_for(n)), dim3(256), 0, st, nf, P, thr, cutoff, partials);
+    *nblk = volume_blocks(n);
+    hipLaunchKernelGGL(obs_plaquette3_kernel, dim3(volume_blocks(n)), dim3(256), 0, st, nf, P, thr, cutoff, partials);
     return hipGetLastError();
 }
 
