@@ -154,8 +154,8 @@ int qsr_autotune(void* ctx, int trial_steps, int* best_variant, int* best_ch, do
  * as they are), so a quench-rate ladder can retire finished droplets early.
  * No reference counterpart (the reference runs one droplet per process). */
 int qsr_set_box_active(void* ctx, int box, int active);
-/* Set the stencil variant (0: 3 waves/SIMD, 1: 4 waves/SIMD register budget;
- * ignored for the L1 kernel) and planes per tile explicitly. */
+/* Set the stencil variant (tile shape: 0 = 64 k x 4 j rows, 1 = 32 k x 8 j rows)
+ * and planes per tile explicitly. */
 int qsr_set_tuning(void* ctx, int variant, int planes_per_tile);
 /* Instrumentation: run nsteps (<= 512) with hipEvents recorded on the
  * context stream around every launch; returns the average duration (ms) of the

@@ -84,7 +84,7 @@ struct Ctx {
     bool full = false, noise = false;
     int elastic = 0;  // union of the boxes' anisotropic terms: 1 = L2, 2 = L3
     int ch = 32;
-    int variant = 0;  // stencil kernel variant (waves/SIMD target), chosen by qsr_autotune
+    int variant = 0;  // tile shape (qsrk::tile_k / tile_j), chosen by qsr_autotune
     std::vector<qsrk::BoxDev> h_boxes;
     qsrk::BoxDev* d_boxes = nullptr;
     std::vector<int4> h_tiles;
@@ -202,12 +202,12 @@ qsrk::BoxDev make_boxdev(const Box& b) {
 }
 
 // Does any interior output voxel of the tile change in a step?
-bool tile_active(const Box& b, int i0, int j0, int k0, int ch) {
+bool tile_active(const Box& b, int i0, int j0, int k0, int ch, int tj, int tk) {
     const qsr_params& p = b.p;
     if (p.periodic) return true;
     const int ib = std::max(i0, 1), ie = std::min(i0 + ch, p.Nx - 1);
-    const int jb = std::max(j0, 1), je = std::min(j0 + qsrk::TJ, p.Ny - 1);
-    const int kb = std::max(k0, 1), ke = std::min(k0 + qsrk::TK, p.Nz - 1) - 1;
+    const int jb = std::max(j0, 1), je = std::min(j0 + tj, p.Ny - 1);
+    const int kb = std::max(k0, 1), ke = std::min(k0 + tk, p.Nz - 1) - 1;
     const bool bc = !(p.W > 0.0);
     for (int i = ib; i < ie; ++i)
         for (int j = jb; j < je; ++j) {
@@ -225,18 +225,20 @@ bool tile_active(const Box& b, int i0, int j0, int k0, int ch) {
 }
 
 int build_tiles(Ctx& c) {
+    // tile shape of the current variant (k extent x j rows)
+    const int tk = qsrk::tile_k(c.variant), tj = qsrk::tile_j(c.variant);
     std::vector<int4> nat;
     c.tile_voxels = 0;
     for (size_t bi = 0; bi < c.boxes.size(); ++bi) {
         const Box& b = c.boxes[bi];
         if (!b.active) continue;
         for (int i0 = 0; i0 < b.p.Nx; i0 += c.ch)
-            for (int j0 = 0; j0 < b.p.Ny; j0 += qsrk::TJ)
-                for (int k0 = 0; k0 < b.p.Nz; k0 += qsrk::TK)
-                    if (tile_active(b, i0, j0, k0, c.ch)) {
+            for (int j0 = 0; j0 < b.p.Ny; j0 += tj)
+                for (int k0 = 0; k0 < b.p.Nz; k0 += tk)
+                    if (tile_active(b, i0, j0, k0, c.ch, tj, tk)) {
                         nat.push_back(make_int4(int(bi), i0, j0, k0));
-                        c.tile_voxels += (long long)std::min(c.ch, b.p.Nx - i0) * std::min(qsrk::TJ, b.p.Ny - j0) *
-                                         std::min(qsrk::TK, b.p.Nz - k0);
+                        c.tile_voxels += (long long)std::min(c.ch, b.p.Nx - i0) * std::min(tj, b.p.Ny - j0) *
+                                         std::min(tk, b.p.Nz - k0);
                     }
     }
     // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs,
@@ -382,7 +384,7 @@ int create_impl(const qsr_params* params, int n, int device, void** out) {
     long long maxvox = 0;
     for (const Box& b : c->boxes) maxvox = std::max(maxvox, b.nvox);
     c->ch = 64;
-    while (c->ch > 8 && maxvox / ((long long)qsrk::TJ * qsrk::TK * c->ch) * (long long)n < 2048) c->ch /= 2;
+    while (c->ch > 8 && maxvox / ((long long)qsrk::TILE_THREADS * c->ch) * (long long)n < 2048) c->ch /= 2;
     const int rc = build_tiles(*c);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(c->st));
@@ -640,13 +642,13 @@ int qsr_autotune(void* ctx, int trial_steps, int* best_variant, int* best_ch, do
     for (const Box& b : c->boxes) maxNx = std::max(maxNx, b.p.Nx);
     double best = 1e300;
     int bv = c->variant, bch = c->ch;
-    const int nvar = c->full ? 2 : 1;
+    const int nvar = 2;  // tile shapes 64 x 4 and 32 x 8
     for (int ch : {8, 16, 32, 64}) {
         if (ch > 8 && ch / 2 >= maxNx) break;
         c->ch = ch;
-        if (int rc = build_tiles(*c)) return rc;
         for (int v = 0; v < nvar; ++v) {
             c->variant = v;
+            if (int rc = build_tiles(*c)) return rc;  // the tile table depends on the shape
             // trial launches read the current buffer and write the spare one: the state does not advance
             for (int s = -1; s < trial_steps; ++s) {
                 if (s == 0) HIP_TRY(hipEventRecord(c->ev[0], c->st));
@@ -715,7 +717,7 @@ int qsr_set_tuning(void* ctx, int variant, int planes_per_tile) {
         return fail(QSR_E_ARG, "variant must be 0/1 and planes_per_tile in [1, 4096]");
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->st));
-    c->variant = c->full ? variant : 0;
+    c->variant = variant;
     c->ch = planes_per_tile;
     if (int rc = build_tiles(*c)) return rc;
     if (c->gexec) discard(hipGraphExecDestroy(c->gexec));
@@ -777,7 +779,7 @@ int qsr_step_imex(void* ctx, int nsteps, double L_stab, int jacobi_iters) {
                                            c->d_face_list));
             for (int it = 0; it < jacobi_iters; ++it) {
                 if (c->ntiles > 0)
-                    HIP_TRY(qsrk::launch_jacobi(c->ntiles, c->st, c->q_rhs, c->q[src], c->q[1 - src], c->cs,
+                    HIP_TRY(qsrk::launch_jacobi(c->variant, c->ntiles, c->st, c->q_rhs, c->q[src], c->q[1 - src], c->cs,
                                                 c->d_boxes, c->d_tiles, c->d_step_rows, c->ch, c->d_jcoef,
                                                 c->d_jflags, it, jacobi_iters, c->jflag_row));
                 src = 1 - src;

@@ -54,8 +54,14 @@ struct alignas(16) RowMask {
     short v[8];
 };
 
-constexpr int TK = 64;  // k extent of a tile: one wavefront
-constexpr int TJ = 4;   // j rows per tile: one wavefront each
+// Tile shapes of the stencil and sweep kernels (256 threads, 4 waves per block),
+// chosen per context by the autotuner ("variant"):
+//   0: 64 k x 4 j rows, one wave per row (wider rows: fewer tiles along k)
+//   1: 32 k x 8 j rows, a wave covers two rows of 32 (smaller in-plane halo, and
+//      waves straddling a droplet surface carry fewer idle lanes)
+constexpr int TILE_THREADS = 256;
+constexpr int tile_k(int variant) { return variant == 1 ? 32 : 64; }
+constexpr int tile_j(int variant) { return variant == 1 ? 8 : 4; }
 
 }  // namespace qsrk
 
@@ -95,9 +101,9 @@ hipError_t launch_faces(long long nlive, hipStream_t st, const float* qin, float
                         const BoxDev* boxes, const StepDyn* dyn, const int4* face_list, const RowMask* rows);
 // One semi-implicit sweep over the step tiles.  flags: 2 x nsweeps words, zero
 // before the first step; row alternates between steps.
-hipError_t launch_jacobi(int ntiles, hipStream_t st, const float* rhs, const float* qold, float* qnew, long long cs,
-                         const BoxDev* boxes, const int4* tiles, const RowMask* rows, int ch, const float4* coef,
-                         unsigned* flags, int sweep, int nsweeps, int row);
+hipError_t launch_jacobi(int variant, int ntiles, hipStream_t st, const float* rhs, const float* qold, float* qnew,
+                         long long cs, const BoxDev* boxes, const int4* tiles, const RowMask* rows, int ch,
+                         const float4* coef, unsigned* flags, int sweep, int nsweeps, int row);
 hipError_t launch_face_copy(long long nlive, hipStream_t st, const float* src, float* dst, long long cs,
                             const BoxDev* boxes, const int4* face_list);
 hipError_t launch_bc(long long nvox, hipStream_t st, float* q, long long cs, const BoxDev* boxes, const StepDyn* dyn,

@@ -12,7 +12,7 @@
 // buffers; voxels that can never change are not written (both buffers hold
 // them from upload).
 //
-// Tiles: a block owns TJ=4 rows (one wavefront each, 64 consecutive k) of
+// Tiles: a block owns TJ rows x TK consecutive k (64 x 4 or 32 x 8) of
 // `ch` consecutive i-planes and streams along i.  Box faces of clamped boxes
 // are excluded here and done by step_face_kernel (one-sided differences).
 #include "qsr_kernels.hpp"
@@ -260,7 +260,7 @@ __device__ __forceinline__ void decode_row4(int4 m, const BoxDev& B, int k, bool
 // ---------------------------------------------------------------------------
 // L1 / kappa kernel: x-register queue + one LDS plane (halo 1), double-buffered.
 // ---------------------------------------------------------------------------
-template <bool NOISE, bool BUF>
+template <int TK, int TJ, bool NOISE, bool BUF>
 __global__ __launch_bounds__(TK* TJ) void step_l1_kernel(const float* __restrict__ qin, float* __restrict__ qout,
                                                           long long cs, const BoxDev* __restrict__ boxes,
                                                           const int4* __restrict__ tiles,
@@ -386,7 +386,7 @@ __global__ __launch_bounds__(TK* TJ) void step_l1_kernel(const float* __restrict
 // ---------------------------------------------------------------------------
 namespace {
 
-template <int TJv>
+template <int TK, int TJv>
 struct AnisoGeom {
     static constexpr int W2 = TK + 4, R2 = TJv + 4, P2 = W2 * R2, QSLOT = 5 * P2;
     static constexpr int W1 = TK + 2, R1 = TJv + 2, P1 = W1 * R1, DSLOT = 3 * P1;
@@ -401,14 +401,14 @@ struct AnisoGeom {
 // EL: elastic terms compiled in (1 = L2, 2 = L3, 3 = both), the union over the
 // context's boxes; a box without a term has a zero coefficient for it, which
 // adds exactly nothing, so the whole molecular field is one basic block.
-template <int TJv, int MINW, bool NOISE, bool BUF, int EL>
-__global__ __launch_bounds__(TK* TJv, MINW) void step_aniso_kernel(const float* __restrict__ qin, float* __restrict__ qout,
+template <int TK, int TJv, bool NOISE, bool BUF, int EL>
+__global__ __launch_bounds__(TK* TJv, 3) void step_aniso_kernel(const float* __restrict__ qin, float* __restrict__ qout,
                                                              long long cs, const BoxDev* __restrict__ boxes,
                                                              const int4* __restrict__ tiles,
                                                              const StepDyn* __restrict__ dyn, int nboxes,
                                                              const RowMask* __restrict__ rows, int ch,
                                                           const float4* __restrict__ imex_tab) {
-    using G = AnisoGeom<TJv>;
+    using G = AnisoGeom<TK, TJv>;
     constexpr int W2 = G::W2, P2 = G::P2, QSLOT = G::QSLOT, W1 = G::W1, P1 = G::P1, DSLOT = G::DSLOT;
     __shared__ float qs[2 * QSLOT];
     __shared__ float ds[2 * DSLOT];
@@ -859,7 +859,7 @@ __device__ __forceinline__ int box_of(long long g, const long long* prefix, int 
 // already hold the two states every later sweep would alternate between, and
 // those sweeps return at once.  Sweep 0 always counts as changed.  Sweep s also
 // clears flags[row ^ 1][s] for the next step (rows alternate between steps).
-template <bool BUF>
+template <int TK, int TJ, bool BUF>
 __global__ __launch_bounds__(TK* TJ) void jacobi_kernel(const float* __restrict__ rhs, const float* __restrict__ qin,
                                                          float* __restrict__ qout, long long cs,
                                                          const BoxDev* __restrict__ boxes,
@@ -1053,16 +1053,6 @@ __global__ void soa_to_aos_kernel(const float* __restrict__ q, double* __restric
     for (int c = 0; c < 5; ++c) aos[5 * v + c] = double(q[c * cs + off + v]);
 }
 
-// Explicit instantiations used by the context code.
-#define QSR_INST_L1(NZ, BF)                                                                          \
-    template __global__ void step_l1_kernel<NZ, BF>(const float*, float*, long long, const BoxDev*, const int4*, \
-                                                    const StepDyn*, int, const RowMask*, int, const float4*);
-QSR_INST_L1(false, false)
-QSR_INST_L1(true, false)
-QSR_INST_L1(false, true)
-QSR_INST_L1(true, true)
-#undef QSR_INST_L1
-
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
@@ -1070,26 +1060,43 @@ namespace qsrk {
 
 namespace {
 
-template <int EL, int MINW, bool NZ, bool BF>
+template <int V, int EL, bool NZ, bool BF>
 void go_aniso(dim3 grid, dim3 block, hipStream_t st, const float* qin, float* qout, long long cs, const BoxDev* boxes,
               const int4* tiles, const StepDyn* dyn, int nboxes, const RowMask* rows, int ch, const float4* imex) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(step_aniso_kernel<TJ, MINW, NZ, BF, EL>), grid, block, 0, st, qin, qout, cs,
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(step_aniso_kernel<tile_k(V), tile_j(V), NZ, BF, EL>), grid, block, 0, st, qin,
+                       qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex);
+}
+
+template <int V, bool NZ, bool BF>
+void go_l1(dim3 grid, dim3 block, hipStream_t st, const float* qin, float* qout, long long cs, const BoxDev* boxes,
+           const int4* tiles, const StepDyn* dyn, int nboxes, const RowMask* rows, int ch, const float4* imex) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(step_l1_kernel<tile_k(V), tile_j(V), NZ, BF>), grid, block, 0, st, qin, qout, cs,
                        boxes, tiles, dyn, nboxes, rows, ch, imex);
 }
 
-template <int EL>
-void go_aniso_el(int variant, bool noise, bool buf, dim3 grid, dim3 block, hipStream_t st, const float* qin,
-                 float* qout, long long cs, const BoxDev* boxes, const int4* tiles, const StepDyn* dyn, int nboxes,
-                 const RowMask* rows, int ch, const float4* imex) {
-#define QSR_A(MW, NZ, BF) go_aniso<EL, MW, NZ, BF>(grid, block, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex)
-    if (variant == 1) {
-        if (buf) { if (noise) QSR_A(4, true, true); else QSR_A(4, false, true); }
-        else { if (noise) QSR_A(4, true, false); else QSR_A(4, false, false); }
-    } else {
-        if (buf) { if (noise) QSR_A(3, true, true); else QSR_A(3, false, true); }
-        else { if (noise) QSR_A(3, true, false); else QSR_A(3, false, false); }
+template <int V>
+void go_step(int elastic, bool noise, bool buf, dim3 grid, dim3 block, hipStream_t st, const float* qin, float* qout,
+             long long cs, const BoxDev* boxes, const int4* tiles, const StepDyn* dyn, int nboxes, const RowMask* rows,
+             int ch, const float4* imex) {
+#define QSR_ARGS grid, block, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex
+#define QSR_A(EL) \
+    (buf ? (noise ? go_aniso<V, EL, true, true>(QSR_ARGS) : go_aniso<V, EL, false, true>(QSR_ARGS)) \
+         : (noise ? go_aniso<V, EL, true, false>(QSR_ARGS) : go_aniso<V, EL, false, false>(QSR_ARGS)))
+    switch (elastic) {
+        case 1: QSR_A(1); break;
+        case 2: QSR_A(2); break;
+        case 3: QSR_A(3); break;
+        default:
+            if (buf) {
+                if (noise) go_l1<V, true, true>(QSR_ARGS);
+                else go_l1<V, false, true>(QSR_ARGS);
+            } else {
+                if (noise) go_l1<V, true, false>(QSR_ARGS);
+                else go_l1<V, false, false>(QSR_ARGS);
+            }
     }
 #undef QSR_A
+#undef QSR_ARGS
 }
 
 }  // namespace
@@ -1097,24 +1104,13 @@ void go_aniso_el(int variant, bool noise, bool buf, dim3 grid, dim3 block, hipSt
 hipError_t launch_step(int elastic, int variant, bool noise, int ntiles, hipStream_t st, const float* qin, float* qout,
                        long long cs, const BoxDev* boxes, const int4* tiles, const StepDyn* dyn, int nboxes,
                        const RowMask* rows, int ch, const float4* imex) {
-    const dim3 grid(ntiles), block(TK * TJ);
+    const dim3 grid(ntiles), block(TILE_THREADS);
     // 32-bit buffer offsets reach every element when 5 planes x cs floats fit 4 GiB
     const bool buf = cs < (1LL << 32) / 20;
-    switch (elastic) {
-        case 1: go_aniso_el<1>(variant, noise, buf, grid, block, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex); break;
-        case 2: go_aniso_el<2>(variant, noise, buf, grid, block, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex); break;
-        case 3: go_aniso_el<3>(variant, noise, buf, grid, block, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex); break;
-        default:
-#define QSR_GO(...) hipLaunchKernelGGL(HIP_KERNEL_NAME(__VA_ARGS__), grid, block, 0, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex)
-            if (buf) {
-                if (noise) QSR_GO(step_l1_kernel<true, true>);
-                else QSR_GO(step_l1_kernel<false, true>);
-            } else {
-                if (noise) QSR_GO(step_l1_kernel<true, false>);
-                else QSR_GO(step_l1_kernel<false, false>);
-            }
-#undef QSR_GO
-    }
+    if (variant == 1)
+        go_step<1>(elastic, noise, buf, grid, block, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex);
+    else
+        go_step<0>(elastic, noise, buf, grid, block, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex);
     return hipGetLastError();
 }
 
@@ -1126,16 +1122,21 @@ hipError_t launch_faces(long long nlive, hipStream_t st, const float* qin, float
     return hipGetLastError();
 }
 
-hipError_t launch_jacobi(int ntiles, hipStream_t st, const float* rhs, const float* qold, float* qnew, long long cs,
-                         const BoxDev* boxes, const int4* tiles, const RowMask* rows, int ch, const float4* coef,
-                         unsigned* flags, int sweep, int nsweeps, int row) {
+hipError_t launch_jacobi(int variant, int ntiles, hipStream_t st, const float* rhs, const float* qold, float* qnew,
+                         long long cs, const BoxDev* boxes, const int4* tiles, const RowMask* rows, int ch,
+                         const float4* coef, unsigned* flags, int sweep, int nsweeps, int row) {
     const bool buf = cs < (1LL << 32) / 20;
-    if (buf)
-        hipLaunchKernelGGL(jacobi_kernel<true>, dim3(ntiles), dim3(TK * TJ), 0, st, rhs, qold, qnew, cs, boxes, tiles,
-                           rows, ch, coef, flags, sweep, nsweeps, row);
-    else
-        hipLaunchKernelGGL(jacobi_kernel<false>, dim3(ntiles), dim3(TK * TJ), 0, st, rhs, qold, qnew, cs, boxes, tiles,
-                           rows, ch, coef, flags, sweep, nsweeps, row);
+#define QSR_J(V, BF)                                                                                              \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(jacobi_kernel<tile_k(V), tile_j(V), BF>), dim3(ntiles), dim3(TILE_THREADS), 0, \
+                       st, rhs, qold, qnew, cs, boxes, tiles, rows, ch, coef, flags, sweep, nsweeps, row)
+    if (variant == 1) {
+        if (buf) QSR_J(1, true);
+        else QSR_J(1, false);
+    } else {
+        if (buf) QSR_J(0, true);
+        else QSR_J(0, false);
+    }
+#undef QSR_J
     return hipGetLastError();
 }
 hipError_t launch_face_copy(long long nlive, hipStream_t st, const float* src, float* dst, long long cs,

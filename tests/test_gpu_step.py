@@ -229,3 +229,24 @@ def test_held_box_is_frozen_and_others_continue(qsr, orc):
             print(f"  box {t}: {n} steps, err {err:.3e}")
             assert err <= 4 * STEP_TOL
         assert s.info(1)["iteration"] == 3 and s.info(2)["iteration"] == 10
+
+
+@pytest.mark.parametrize("name", ["l2_l3_weak", "aniso_spacing_l2", "l1_weak_custom", "isotropic_init_quench_T"])
+def test_tile_shapes_agree(qsr, orc, name):
+    """Both tile shapes (64 k x 4 rows, 32 k x 8 rows) and several plane counts give the same field,
+    matching the oracle."""
+    p, od, Q0, S_shell = _case(qsr, orc, **CASES[name])
+    Qo = od.step(Q0, S_shell, p.dt, steps=3)
+    got = {}
+    for variant, ch in ((0, 8), (0, 32), (1, 8), (1, 16)):
+        with qsr.Solver(p) as s:
+            s.set_tuning(variant, ch)
+            s.upload(Q0)
+            s.step(3)
+            got[(variant, ch)] = s.download()
+        err = np.abs(got[(variant, ch)] - Qo).max()
+        print(f"  variant {variant} ch {ch}: err {err:.3e}")
+        assert err <= 4 * STEP_TOL
+    ref = got[(0, 8)]
+    for key, Q in got.items():
+        assert np.abs(Q - ref).max() <= 1e-6, key
