@@ -189,6 +189,63 @@ int qsr_save_qtensor_file(const char* path, const double* aos5, int Nx, int Ny, 
 int qsr_init_field(const qsr_params* p, int mode, double S0, double noise_amplitude,
                    int64_t random_seed, double* aos5_out, size_t n_voxels);
 
+/* ---------------------------------------------------------------------------
+ * Learned warm-start step.  No counterpart in the reference: QSR.cu only has
+ * the plain relaxation loop (QSR.cu:3126-3476), which stays the fallback and
+ * the judge of every state.  A three-layer 3x3x3 convolution network (5 -> 32
+ * -> 32 -> 5 channels, GELU, 36,357 parameters) predicts dQ = Q(k+N) - Q(k)
+ * from Q(k); it runs on BF16 matrix cores with FP32 accumulation, and is
+ * trained with its own backward and Adam kernels on pairs of fields the plain
+ * solver produced.  Parameters are one flat FP32 array:
+ * W1[27][5][32] b1[32] W2[27][32][32] b2[32] W3[27][32][5] b3[5], tap
+ * t = (di+1)*9 + (dj+1)*3 + (dk+1) for the neighbour (i+di, j+dj, k+dk),
+ * y[v,co] = b[co] + sum_{t,ci} W[t][ci][co] x[v+off(t), ci] (zero outside).
+ * ------------------------------------------------------------------------- */
+enum {
+    QSR_SURR_CHECKED = 0, /* keep the step only if F and the residual do not rise */
+    QSR_SURR_FORCE = 1,   /* keep the step */
+    QSR_SURR_PROPOSE = 2  /* measure, then restore the previous field */
+};
+typedef struct qsr_surr_result {
+    double F_before, F_after;     /* total free energy (J) before / with the step */
+    double res_before, res_after; /* |Q' - Q|_2 of one explicit Euler step over the voxels a step can change */
+    int32_t accepted;             /* 1 if the context now holds the stepped field */
+    double ms_network;            /* device time of the network pass */
+} qsr_surr_result;
+
+int qsr_surr_num_params(size_t* n);
+int qsr_surr_create(int device, void** net);
+int qsr_surr_destroy(void* net);
+/* uniform(-1/sqrt(fan_in), +) weights and biases, last layer x0.1, from seed */
+int qsr_surr_init_params(void* net, uint64_t seed);
+/* set resets the Adam state */
+int qsr_surr_set_params(void* net, const float* params, size_t n);
+int qsr_surr_get_params(void* net, float* params, size_t n);
+int qsr_surr_save(void* net, const char* path);
+int qsr_surr_load(void* net, const char* path);
+/* One pass over a host batch: x5, target5 (may be NULL = 0) and pred5 are
+ * (nbatch, X, Y, Z, 5) FP32, mask (may be NULL = 1) is (nbatch, X, Y, Z).
+ * loss = sum mask * |pred - target|^2 / (5 nbatch X Y Z); grad receives its
+ * gradient (flat layout).  loss / grad need Z % 8 == 0.  Outputs may be NULL. */
+int qsr_surr_eval(void* net, int nbatch, int X, int Y, int Z, const float* x5, const float* target5,
+                  const float* mask, double* loss, float* pred5, float* grad);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected) with a given gradient. */
+int qsr_surr_adam_step(void* net, const float* grad, double lr);
+/* Training data: snapshot the current field of a context's box (and which of
+ * its voxels a step can change) into device memory; returns its index. */
+int qsr_surr_dataset_add(void* net, void* ctx, int box, int* snapshot);
+int qsr_surr_dataset_size(void* net, int* n);
+int qsr_surr_dataset_clear(void* net);
+/* iters Adam steps on batches of `batch` random crop^3 windows (crop % 8 == 0)
+ * of the pairs (input snapshot, target snapshot) pairs[2p], pairs[2p+1]; the
+ * target is their difference.  A crop face inside its box leaves its 3-voxel
+ * band unscored.  loss_per_iter (may be NULL) gets the masked mean square error. */
+int qsr_surr_train(void* net, const int* pairs, int npairs, int iters, int batch, int crop, double lr,
+                   uint64_t seed, double* loss_per_iter);
+/* Propose Q + dQ for every voxel a step can change in a single-box context;
+ * mode QSR_SURR_*.  The iteration counter is not advanced. */
+int qsr_surr_apply(void* net, void* ctx, int mode, qsr_surr_result* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,5 +4,5 @@ The compute path is libqsr_amd.so (hand-written HIP kernels for gfx950 behind
 the C ABI in include/qsr_amd.h); :mod:`lcdrsimulator_amd.qsr` is its ctypes
 binding and ``lib/QSR_hip`` the ``--config`` command-line driver.
 """
-from .qsr import (PROTOCOL_CONSTANT, PROTOCOL_RAMP, PROTOCOL_STEP, Params, QSRError, Solver, S_eq,  # noqa: F401
-                  device_count, dt_estimate, init_field, load, load_qtensor, save_qtensor)
+from .qsr import (PROTOCOL_CONSTANT, PROTOCOL_RAMP, PROTOCOL_STEP, SURR_CHECKED, SURR_FORCE, SURR_PROPOSE, Params,  # noqa: F401
+                  QSRError, Solver, S_eq, Surrogate, device_count, dt_estimate, init_field, load, load_qtensor, save_qtensor)

@@ -27,8 +27,8 @@ DEV_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC",
              "-Wno-unused-command-line-argument"]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"--offload-arch={ARCH}", "-Wno-unused-command-line-argument"]
 
-DEVICE_SOURCES = ["qsr_step.hip", "qsr_obs.hip"]
-HOST_SOURCES = ["qsr_capi.cpp", "qsr_host.cpp"]
+DEVICE_SOURCES = ["qsr_step.hip", "qsr_obs.hip", "qsr_surr.hip"]
+HOST_SOURCES = ["qsr_capi.cpp", "qsr_host.cpp", "qsr_surrogate.cpp"]
 
 
 def _run(cmd):

@@ -12,8 +12,10 @@
 #include <vector>
 
 #include "../../include/qsr_amd.h"
+#include "qsr_ctx.hpp"
 #include "qsr_host.hpp"
 #include "qsr_kernels.hpp"
+#include "qsr_surr.hpp"
 
 namespace {
 
@@ -117,6 +119,7 @@ struct Ctx {
     int jflag_n = 0, jflag_row = 0;
     long long* d_vox_prefix = nullptr;
     long long nvox_total = 0;
+    float4* d_rhs0 = nullptr;  // per box (0, 0, 0, 1): the RHS mode with no stabiliser (residuals)
 
     ~Ctx() {
         if (gexec) discard(hipGraphExecDestroy(gexec));
@@ -125,7 +128,7 @@ struct Ctx {
         for (void* p : {(void*)d_boxes, (void*)d_tiles, (void*)d_step_rows, (void*)d_obs_rows, (void*)d_face_list,
                         (void*)d_dyn, (void*)d_part, (void*)d_out, (void*)d_nf_slice[0], (void*)d_nf_slice[1],
                         (void*)d_nf3, (void*)q_rhs, (void*)d_imex, (void*)d_jcoef, (void*)d_vox_prefix,
-                        (void*)d_jflags})
+                        (void*)d_jflags, (void*)d_rhs0})
             discard(hipFree(p));
         if (h_dyn) discard(hipHostFree(h_dyn));
         for (hipEvent_t e : ev)
@@ -736,8 +739,8 @@ int qsr_step_imex(void* ctx, int nsteps, double L_stab, int jacobi_iters) {
     HIP_TRY(hipSetDevice(c->device));
     if (nsteps == 0) return QSR_OK;
     const int nb = int(c->boxes.size());
-    if (!c->q_rhs) {
-        HIP_TRY(hipMalloc(&c->q_rhs, sizeof(float) * 5 * c->cs));
+    if (!c->q_rhs) HIP_TRY(hipMalloc(&c->q_rhs, sizeof(float) * 5 * c->cs));  // also made by euler_residual
+    if (!c->d_imex) {
         HIP_TRY(hipMalloc(&c->d_imex, sizeof(float4) * nb));
         HIP_TRY(hipMalloc(&c->d_jcoef, sizeof(float4) * nb));
     }
@@ -1026,3 +1029,67 @@ int qsr_init_field(const qsr_params* p, int mode, double S0, double noise_amplit
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Internal access for the learned warm-start step (qsr_ctx.hpp)
+// ---------------------------------------------------------------------------
+namespace qsrc {
+
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+int field_view(void* ctx, int box, FieldView* out) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !out || box < 0 || box >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad context or box");
+    FieldView v;
+    v.device = c->device;
+    v.st = c->st;
+    v.cur = c->q[c->cur];
+    v.other = c->q[1 - c->cur];
+    v.cs = c->cs;
+    v.nvox = c->boxes[box].nvox;
+    v.box = c->h_boxes[box];
+    v.rows = c->d_step_rows;
+    v.nboxes = int(c->boxes.size());
+    v.uploaded = c->boxes[box].uploaded;
+    *out = v;
+    return QSR_OK;
+}
+
+int commit_other(void* ctx) {
+    Ctx* c = as_ctx(ctx);
+    if (!c) return fail(QSR_E_ARG, "bad context");
+    if (c->boxes.size() != 1) return fail(QSR_E_STATE, "commit_other needs a single-box context");
+    c->cur = 1 - c->cur;
+    return QSR_OK;
+}
+
+int euler_residual(void* ctx, int box, double* res) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !res || box < 0 || box >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad context or box");
+    if (int rc = check_uploaded(*c)) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    const int nb = int(c->boxes.size());
+    if (!c->q_rhs) HIP_TRY(hipMalloc(&c->q_rhs, sizeof(float) * 5 * c->cs));
+    if (!c->d_rhs0) {
+        std::vector<float4> z(nb, make_float4(0.f, 0.f, 0.f, 1.f));
+        HIP_TRY(hipMalloc(&c->d_rhs0, sizeof(float4) * nb));
+        HIP_TRY(hipMemcpy(c->d_rhs0, z.data(), sizeof(float4) * nb, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipStreamSynchronize(c->st));
+    if (int rc = fill_dyn(*c, 1)) return rc;
+    if (c->ntiles > 0)
+        HIP_TRY(qsrk::launch_step(c->elastic, c->variant, false, c->ntiles, c->st, c->q[c->cur], c->q_rhs, c->cs,
+                                  c->d_boxes, c->d_tiles, c->d_dyn, nb, c->d_step_rows, c->ch, c->d_rhs0));
+    if (int rc = ensure_partials(*c, 4096)) return rc;
+    long long nblk = 0;
+    HIP_TRY(qsrs::launch_residual(c->st, c->q[c->cur], c->q_rhs, c->cs, c->h_boxes[box], c->d_step_rows, c->d_part,
+                                  &nblk));
+    HIP_TRY(qsrs::launch_sum_slots(c->st, c->d_part, nblk, 1, c->d_out));
+    double r = 0.0;
+    HIP_TRY(hipMemcpyAsync(&r, c->d_out, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    *res = std::sqrt(r);
+    return QSR_OK;
+}
+
+}  // namespace qsrc

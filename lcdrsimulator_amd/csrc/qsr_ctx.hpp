@@ -1,0 +1,39 @@
+// qsr_ctx.hpp -- what other parts of the library may see of a solver context
+// (defined in qsr_capi.cpp): its device field, one box's description and an
+// explicit-Euler residual that leaves the field untouched.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "qsr_kernels.hpp"
+
+namespace qsrc {
+
+struct FieldView {
+    int device = 0;
+    hipStream_t st = nullptr;
+    const float* cur = nullptr;  // the current field (five SoA planes of cs floats)
+    float* other = nullptr;      // the other ping-pong buffer
+    long long cs = 0, nvox = 0;
+    qsrk::BoxDev box{};
+    const qsrk::RowMask* rows = nullptr;  // step row masks (packed over all boxes)
+    int nboxes = 0;
+    bool uploaded = false;
+};
+
+// Fails with QSR_E_ARG for a bad context or box index.
+int field_view(void* ctx, int box, FieldView* out);
+// Makes the other buffer current.  Only for single-box contexts, after every
+// voxel a step can change has been written into it (the rest is identical in
+// both buffers).
+int commit_other(void* ctx);
+// sqrt of the sum over the box's changeable interior voxels of |Q' - Q|^2,
+// Q' = one explicit Euler step without noise of the current field (the
+// semi-implicit right-hand side with no stabiliser); the field is unchanged.
+int euler_residual(void* ctx, int box, double* res);
+// Sets the calling thread's qsr_last_error() message and returns code.
+int set_error(int code, const std::string& msg);
+
+}  // namespace qsrc

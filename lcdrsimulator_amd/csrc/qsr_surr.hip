@@ -1,0 +1,664 @@
+// qsr_surr.hip -- kernels of the learned warm-start step: BF16 implicit-GEMM
+// 3x3x3 convolutions on the gfx950 matrix cores (v_mfma_f32_16x16x32_bf16,
+// FP32 accumulation), their input and weight gradients, the masked loss, Adam
+// and the packing of parameters into MFMA operand fragments.  The network and
+// the layouts are described in qsr_surr.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "qsr_surr.hpp"
+
+namespace {
+
+using qsrk::BoxDev;
+using qsrk::RowMask;
+using namespace qsrs;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2/pi)
+constexpr float kGeluA = 0.044715f;
+
+__device__ __forceinline__ float gelu_f(float z) {
+    return 0.5f * z * (1.0f + tanhf(kGeluC * (z + kGeluA * z * z * z)));
+}
+// d/dz of gelu_f
+__device__ __forceinline__ float gelu_d(float z) {
+    const float z2 = z * z;
+    const float t = tanhf(kGeluC * (z + kGeluA * z2 * z));
+    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * kGeluA * z2);
+}
+
+// Round to nearest even (the hardware conversion), as raw bits.
+__device__ __forceinline__ uint16_t bf16_bits(float x) {
+    return __builtin_bit_cast(uint16_t, static_cast<__bf16>(x));
+}
+
+__device__ __forceinline__ f32x4 mfma(u32x4 a, u32x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
+}
+
+__device__ __forceinline__ bool changeable(const BoxDev& B, const RowMask* rows, int i, int j, int k) {
+    if (B.flags & qsrk::F_PERIODIC) return true;
+    const RowMask m = rows[B.rows_off + (long long)i * B.Ny + j];
+    const bool evolve = k >= m.v[0] && k <= m.v[1];
+    const bool shell = (k >= m.v[2] && k <= m.v[3]) && !(k >= m.v[4] && k <= m.v[5]);
+    return evolve && !(shell && (B.flags & (qsrk::F_SKIPSHELL | qsrk::F_BC)));
+}
+
+// ---------------------------------------------------------------------------
+// Convolution (forward, and the input gradient as a forward convolution with
+// the taps flipped and the channel roles swapped).  Each wave computes S
+// segments of 16 voxels along k for all 16*NB output channels:
+//   C[co][voxel] += Wfrag[co][K] * act[K][voxel], K = (tap, ci) in steps of 32.
+// The weight fragments of the layer are staged in LDS once per block; the
+// activation operand of lane l is the 16-byte run of 8 channels of one tap at
+// voxel l%16 of the segment (zero outside the box).
+// ---------------------------------------------------------------------------
+template <int CP, int NB, int EPI>
+__global__ __launch_bounds__(256) void conv_kernel(ConvArgs a) {
+    constexpr int KS = ksteps(CP);
+    constexpr int S = 4;
+    extern __shared__ u32x4 wl[];
+    const u32x4* wf = reinterpret_cast<const u32x4*>(a.frag);
+    for (int f = threadIdx.x; f < NB * KS * 64; f += 256) wl[f] = wf[f];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, col = lane & 15;
+    const long long YZ = (long long)a.Y * a.Z;
+    const long long stride = (long long)gridDim.x * 4 * S;
+    for (long long base = ((long long)blockIdx.x * 4 + wave) * S; base < a.nseg; base += stride) {
+        long long vox[S];
+        int vi[S], vj[S], vk[S];
+        bool ok[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const long long seg = base + s;
+            const long long row = seg < a.nseg ? seg / a.segs_row : 0;
+            const int k = int(seg - row * a.segs_row) * 16 + col;
+            ok[s] = seg < a.nseg && k < a.Z;
+            vj[s] = int(row % a.Y);
+            vi[s] = int((row / a.Y) % a.X);
+            vk[s] = k;
+            vox[s] = row * a.Z + k;
+        }
+        f32x4 acc[S][NB];
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) acc[s][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            // tap and first channel of this lane's 8 K values
+            const int t = CP == 32 ? ks : ks * 4 + g;
+            const int c0 = CP == 32 ? g * 8 : 0;
+            const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
+            const long long shift = di * YZ + dj * (long long)a.Z + dk;
+            u32x4 w[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) w[nb] = wl[(nb * KS + ks) * 64 + lane];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const bool inb = ok[s] && t < TAPS && unsigned(vi[s] + di) < unsigned(a.X) &&
+                                 unsigned(vj[s] + dj) < unsigned(a.Y) && unsigned(vk[s] + dk) < unsigned(a.Z);
+                u32x4 b = {0u, 0u, 0u, 0u};
+                if (inb) b = *reinterpret_cast<const u32x4*>(a.in + (vox[s] + shift) * CP + c0);
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) acc[s][nb] = mfma(w[nb], b, acc[s][nb]);
+            }
+        }
+
+        // lane holds output channels nb*16 + g*4 + r (r = 0..3) of voxel vox[s]
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (!ok[s]) continue;
+            const long long v = vox[s];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                const int co = nb * 16 + g * 4;
+                const f32x4 c = acc[s][nb];
+                if constexpr (EPI == EPI_HIDDEN) {
+                    const f32x4 bz = *reinterpret_cast<const f32x4*>(a.bias + co);
+                    const f32x4 z = c + bz;
+                    if (a.z_out) *reinterpret_cast<f32x4*>(a.z_out + v * 32 + co) = z;
+                    uint16_t h[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) h[r] = bf16_bits(gelu_f(z[r]));
+                    const uint2 pk = make_uint2(unsigned(h[0]) | (unsigned(h[1]) << 16),
+                                                unsigned(h[2]) | (unsigned(h[3]) << 16));
+                    *reinterpret_cast<uint2*>(a.a_out + v * 32 + co) = pk;
+                    if (a.a_cm) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) a.a_cm[(long long)(co + r) * a.V + v] = h[r];
+                    }
+                } else if constexpr (EPI == EPI_PRED) {
+                    if (co < 8) {
+                        const f32x4 bz = *reinterpret_cast<const f32x4*>(a.bias + co);
+                        *reinterpret_cast<f32x4*>(a.pred + v * 8 + co) = c + bz;
+                    }
+                } else if constexpr (EPI == EPI_APPLY) {
+                    if (co < 8 && changeable(a.box, a.rows, vi[s], vj[s], vk[s])) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int ch = co + r;
+                            if (ch < CH_Q) {
+                                const long long idx = (long long)ch * a.cs + a.box.off + v;
+                                a.q_new[idx] = a.q_old[idx] + (c[r] + a.bias[ch]);
+                            }
+                        }
+                    }
+                } else {  // EPI_BWD
+                    const f32x4 z = *reinterpret_cast<const f32x4*>(a.z_in + v * 32 + co);
+                    uint16_t h[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) h[r] = bf16_bits(c[r] * gelu_d(z[r]));
+                    const uint2 pk = make_uint2(unsigned(h[0]) | (unsigned(h[1]) << 16),
+                                                unsigned(h[2]) | (unsigned(h[3]) << 16));
+                    *reinterpret_cast<uint2*>(a.a_out + v * 32 + co) = pk;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) a.a_cm[(long long)(co + r) * a.V + v] = h[r];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: dW[t][ci][co] = sum_v act[v + off(t), ci] * dy[v, co] and
+// db[co] = sum_v dy[v, co], as a GEMM with M = (tap, ci) rows (plus one row of
+// ones for the bias), N = output channels, K = voxels in chunks of 32.  Lane l
+// holds 8 consecutive voxels (along k, Z % 8 == 0) of one row or channel:
+// both operands are read from the channel-major copies.  A tap's k shift of
+// +-1 element is rebuilt from aligned loads with v_alignbyte.  Each block
+// accumulates over a grid-stride set of chunks and writes its partial sums.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ u32x4 load_run(const uint16_t* rowp, int k0, int dk, int Z) {
+    const u32x4 cur = *reinterpret_cast<const u32x4*>(rowp + k0);
+    if (dk == 0) return cur;
+    u32x4 o;
+    if (dk < 0) {
+        const unsigned prev = k0 > 0 ? *reinterpret_cast<const unsigned*>(rowp + k0 - 2) : 0u;
+        o[0] = __builtin_amdgcn_alignbyte(cur[0], prev, 2);
+        o[1] = __builtin_amdgcn_alignbyte(cur[1], cur[0], 2);
+        o[2] = __builtin_amdgcn_alignbyte(cur[2], cur[1], 2);
+        o[3] = __builtin_amdgcn_alignbyte(cur[3], cur[2], 2);
+    } else {
+        const unsigned next = k0 + 8 < Z ? *reinterpret_cast<const unsigned*>(rowp + k0 + 8) : 0u;
+        o[0] = __builtin_amdgcn_alignbyte(cur[1], cur[0], 2);
+        o[1] = __builtin_amdgcn_alignbyte(cur[2], cur[1], 2);
+        o[2] = __builtin_amdgcn_alignbyte(cur[3], cur[2], 2);
+        o[3] = __builtin_amdgcn_alignbyte(next, cur[3], 2);
+    }
+    return o;
+}
+
+template <int CP, int NB>
+__global__ __launch_bounds__(256) void wgrad_kernel(const uint16_t* __restrict__ act_cm,
+                                                    const uint16_t* __restrict__ dy_cm, int X, int Y, int Z,
+                                                    long long V, float* __restrict__ part) {
+    constexpr int ROWS = TAPS * CP;
+    constexpr int MBT = (ROWS + 1 + 15) / 16;
+    constexpr int MBW = (MBT + 3) / 4;
+    constexpr int COP = NB == 1 ? 8 : 32;  // output-gradient channels stored
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, col = lane & 15;
+    f32x4 acc[MBW][NB];
+#pragma unroll
+    for (int m = 0; m < MBW; ++m)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[m][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const long long nchunk = (V + 31) / 32;
+    for (long long chk = blockIdx.x; chk < nchunk; chk += gridDim.x) {
+        const long long v0 = chk * 32 + g * 8;
+        const bool run_ok = v0 < V;
+        const long long row = run_ok ? v0 / Z : 0;
+        const int k0 = int(v0 - row * Z);
+        const int j = int(row % Y), i = int((row / Y) % X);
+        u32x4 bfr[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const int co = nb * 16 + col;
+            bfr[nb] = u32x4{0u, 0u, 0u, 0u};
+            if (run_ok && co < COP) bfr[nb] = *reinterpret_cast<const u32x4*>(dy_cm + (long long)co * V + v0);
+        }
+#pragma unroll
+        for (int m = 0; m < MBW; ++m) {
+            const int mb = wave + 4 * m;
+            if (mb >= MBT) break;
+            const int R = mb * 16 + col;
+            u32x4 afr = {0u, 0u, 0u, 0u};
+            if (run_ok) {
+                if (R < ROWS) {
+                    const int t = R / CP, ci = R % CP;
+                    const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
+                    if (unsigned(i + di) < unsigned(X) && unsigned(j + dj) < unsigned(Y)) {
+                        const long long nrow = row + (long long)di * Y + dj;
+                        afr = load_run(act_cm + (long long)ci * V + nrow * Z, k0, dk, Z);
+                    }
+                } else if (R == ROWS) {
+                    afr = u32x4{0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u};  // BF16 1.0
+                }
+            }
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) acc[m][nb] = mfma(afr, bfr[nb], acc[m][nb]);
+        }
+    }
+    // lane holds rows mb*16 + g*4 + r, output channel nb*16 + col
+#pragma unroll
+    for (int m = 0; m < MBW; ++m) {
+        const int mb = wave + 4 * m;
+        if (mb >= MBT) break;
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long long R = mb * 16 + g * 4 + r;
+                part[((long long)blockIdx.x * MBT * 16 + R) * (NB * 16) + nb * 16 + col] = acc[m][nb][r];
+            }
+    }
+}
+
+// Fixed-order sum of the per-block partials into the flat gradient.
+__global__ __launch_bounds__(256) void wgrad_reduce_kernel(const float* __restrict__ part, int nblk, int cp, int mbt,
+                                                           int ncol, int cin, int cout, long long w_off,
+                                                           long long b_off, float* __restrict__ grad) {
+    const int rows = TAPS * cp;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)(rows + 1) * cout;
+    if (idx >= total) return;
+    const int R = int(idx / cout), co = int(idx % cout);
+    long long dst;
+    if (R < rows) {
+        const int t = R / cp, ci = R % cp;
+        if (ci >= cin) return;
+        dst = w_off + ((long long)t * cin + ci) * cout + co;
+    } else {
+        dst = b_off + co;
+    }
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += part[((long long)b * mbt * 16 + R) * ncol + co];
+    grad[dst] = float(s);
+}
+
+// ---------------------------------------------------------------------------
+// Parameter packing: FP32 master parameters -> BF16 fragments and padded biases.
+// Forward fragment (cp, nb): value W[t][ci][co], K = t*cp + ci, co = nb*16 + l%16.
+// Input-gradient fragment of a layer (cp = its padded output count): the
+// convolution of dy with W'[t][co][ci] = W[26 - t][ci][co].
+// ---------------------------------------------------------------------------
+struct FragSpec {
+    uint16_t* dst;
+    long long w_off;
+    int cp, nb, cin, cout, transpose;
+};
+
+__device__ float frag_value(const float* P, const FragSpec& s, long long e) {
+    const int KS = ksteps(s.cp);
+    const int el = int(e & 7), l = int((e >> 3) & 63);
+    const long long fi = e >> 9;  // nb * KS + ks
+    const int ks = int(fi % KS), nb = int(fi / KS);
+    const int K = ks * 32 + (l >> 4) * 8 + el;
+    const int t = K / s.cp, kc = K % s.cp;
+    const int oc = nb * 16 + (l & 15);
+    if (t >= TAPS) return 0.0f;
+    if (!s.transpose) {
+        if (kc >= s.cin || oc >= s.cout) return 0.0f;
+        return P[s.w_off + ((long long)t * s.cin + kc) * s.cout + oc];
+    }
+    // input gradient: kc runs over the layer's outputs, oc over its inputs
+    if (kc >= s.cout || oc >= s.cin) return 0.0f;
+    return P[s.w_off + ((long long)(TAPS - 1 - t) * s.cin + oc) * s.cout + kc];
+}
+
+__global__ __launch_bounds__(256) void pack_frags_kernel(const float* __restrict__ P, Frags f) {
+    const FragSpec spec[5] = {
+        {f.f1, OFF_W1, 8, 2, CH_Q, CH_H, 0},   {f.f2, OFF_W2, 32, 2, CH_H, CH_H, 0},
+        {f.f3, OFF_W3, 32, 1, CH_H, CH_Q, 0},  {f.g3, OFF_W3, 8, 2, CH_H, CH_Q, 1},
+        {f.g2, OFF_W2, 32, 2, CH_H, CH_H, 1},
+    };
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int s = 0; s < 5; ++s) {
+        const long long n = frag_elems(spec[s].cp, spec[s].nb);
+        if (e < n) {
+            spec[s].dst[e] = bf16_bits(frag_value(P, spec[s], e));
+            return;
+        }
+        e -= n;
+    }
+    if (e < 32) {
+        f.b1[e] = P[OFF_B1 + e];
+        return;
+    }
+    e -= 32;
+    if (e < 32) {
+        f.b2[e] = P[OFF_B2 + e];
+        return;
+    }
+    e -= 32;
+    if (e < 16) f.b3[e] = e < CH_Q ? P[OFF_B3 + e] : 0.0f;
+}
+
+// Adam (Kingma & Ba 2015), bias corrections folded into lr_t and inv_sqrt_bc2:
+// p -= lr_t * m / (sqrt(v) * inv_sqrt_bc2 + eps),  lr_t = lr / (1 - b1^t),
+// inv_sqrt_bc2 = 1 / sqrt(1 - b2^t).
+__global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
+                                                   float* __restrict__ m, float* __restrict__ v, long long n,
+                                                   float lr_t, float b1, float b2, float eps, float inv_sqrt_bc2) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr_t * mi / (sqrtf(vi) * inv_sqrt_bc2 + eps);
+}
+
+// ---------------------------------------------------------------------------
+// Batch assembly
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void store_x(const Work& w, long long v, const float q[5]) {
+    uint16_t h[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) h[c] = c < CH_Q ? bf16_bits(q[c]) : uint16_t(0);
+    u32x4 pk;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) pk[c] = unsigned(h[2 * c]) | (unsigned(h[2 * c + 1]) << 16);
+    *reinterpret_cast<u32x4*>(w.x + v * 8) = pk;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) w.x_cm[(long long)c * w.V + v] = h[c];
+}
+
+__global__ __launch_bounds__(256) void crop_kernel(const Snap* __restrict__ snaps, const CropJob* __restrict__ jobs,
+                                                   int C, Work w) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= w.V) return;
+    const long long C3 = (long long)C * C * C;
+    const int b = int(v / C3);
+    const long long loc = v - b * C3;
+    const int i = int(loc / ((long long)C * C)), j = int((loc / C) % C), k = int(loc % C);
+    const CropJob J = jobs[b];
+    const Snap si = snaps[J.s_in], so = snaps[J.s_out];
+    const long long src = ((long long)(J.ox + i) * si.Y + (J.oy + j)) * si.Z + (J.oz + k);
+    float qi[5], d[8];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        qi[c] = si.q[c * si.nvox + src];
+        d[c] = so.q[c * so.nvox + src] - qi[c];
+    }
+    d[5] = d[6] = d[7] = 0.0f;
+    store_x(w, v, qi);
+    *reinterpret_cast<f32x4*>(w.tgt + v * 8) = f32x4{d[0], d[1], d[2], d[3]};
+    *reinterpret_cast<f32x4*>(w.tgt + v * 8 + 4) = f32x4{d[4], d[5], d[6], d[7]};
+    // a crop face that is not a box face sees zero padding instead of the true
+    // neighbours: the 3-voxel band (the network's reach) behind it is not scored
+    const int m3 = 3;
+    const bool inner = (i >= m3 || J.ox == 0) && (i < C - m3 || J.ox + C == si.X) && (j >= m3 || J.oy == 0) &&
+                       (j < C - m3 || J.oy + C == si.Y) && (k >= m3 || J.oz == 0) && (k < C - m3 || J.oz + C == si.Z);
+    w.mask[v] = (inner && si.mask[src]) ? 1.0f : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void pack_batch_kernel(const float* __restrict__ x5, const float* __restrict__ t5,
+                                                         const float* __restrict__ mk, Work w) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= w.V) return;
+    float q[5], d[8] = {};
+#pragma unroll
+    for (int c = 0; c < 5; ++c) q[c] = x5[v * 5 + c];
+    store_x(w, v, q);
+    if (t5) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) d[c] = t5[v * 5 + c];
+    }
+    *reinterpret_cast<f32x4*>(w.tgt + v * 8) = f32x4{d[0], d[1], d[2], d[3]};
+    *reinterpret_cast<f32x4*>(w.tgt + v * 8 + 4) = f32x4{d[4], d[5], d[6], d[7]};
+    w.mask[v] = mk ? mk[v] : 1.0f;
+}
+
+__global__ __launch_bounds__(256) void pack_field_kernel(const float* __restrict__ q, long long cs, long long off,
+                                                         long long V, uint16_t* __restrict__ x) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    uint16_t h[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) h[c] = c < CH_Q ? bf16_bits(q[(long long)c * cs + off + v]) : uint16_t(0);
+    u32x4 pk;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) pk[c] = unsigned(h[2 * c]) | (unsigned(h[2 * c + 1]) << 16);
+    *reinterpret_cast<u32x4*>(x + v * 8) = pk;
+}
+
+__global__ __launch_bounds__(256) void unpack_pred_kernel(const float* __restrict__ pred, long long V,
+                                                          float* __restrict__ out5) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) out5[v * 5 + c] = pred[v * 8 + c];
+}
+
+// ---------------------------------------------------------------------------
+// Reductions (FP64, fixed order: per-block partials, then one finishing block)
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ void block_sum_store(double (&v)[N], double* dst) {
+    __shared__ double red[N][256];
+#pragma unroll
+    for (int s = 0; s < N; ++s) red[s][threadIdx.x] = v[s];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+#pragma unroll
+            for (int s = 0; s < N; ++s) red[s][threadIdx.x] += red[s][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int s = 0; s < N; ++s) dst[s] = red[s][0];
+    }
+}
+
+__global__ __launch_bounds__(256) void loss_kernel(Work w, float scale) {
+    double acc[2] = {0.0, 0.0};
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < w.V;
+         v += (long long)gridDim.x * blockDim.x) {
+        const float mk = w.mask[v];
+        uint16_t h[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const float d = c < CH_Q ? w.pred[v * 8 + c] - w.tgt[v * 8 + c] : 0.0f;
+            acc[0] += double(mk) * double(d) * double(d);
+            h[c] = bf16_bits(scale * mk * d);
+        }
+        acc[1] += mk;
+        u32x4 pk;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) pk[c] = unsigned(h[2 * c]) | (unsigned(h[2 * c + 1]) << 16);
+        *reinterpret_cast<u32x4*>(w.dy + v * 8) = pk;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) w.dy_cm[(long long)c * w.V + v] = h[c];
+    }
+    block_sum_store<2>(acc, w.red + 2 * blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void sum_slots_kernel(const double* __restrict__ partials, long long nblk,
+                                                        int nslot, double* __restrict__ out) {
+    __shared__ double red[256];
+    for (int s = 0; s < nslot; ++s) {
+        double a = 0.0;
+        for (long long b = threadIdx.x; b < nblk; b += 256) a += partials[b * nslot + s];
+        red[threadIdx.x] = a;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[s] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void mask_bytes_kernel(long long nvox, BoxDev B, const RowMask* __restrict__ rows,
+                                                         uint8_t* __restrict__ out) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    const int k = int(v % B.Nz), j = int((v / B.Nz) % B.Ny), i = int(v / ((long long)B.Nz * B.Ny));
+    out[v] = changeable(B, rows, i, j, k) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void residual_kernel(const float* __restrict__ q, const float* __restrict__ rhs,
+                                                       long long cs, BoxDev B, const RowMask* __restrict__ rows,
+                                                       double* __restrict__ partials) {
+    const long long nvox = (long long)B.Nx * B.Ny * B.Nz;
+    double acc[1] = {0.0};
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
+         v += (long long)gridDim.x * blockDim.x) {
+        const int k = int(v % B.Nz), j = int((v / B.Nz) % B.Ny), i = int(v / ((long long)B.Nz * B.Ny));
+        const bool face = i == 0 || j == 0 || k == 0 || i == B.Nx - 1 || j == B.Ny - 1 || k == B.Nz - 1;
+        if (face || !changeable(B, rows, i, j, k)) continue;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            const long long idx = (long long)c * cs + B.off + v;
+            const double d = double(rhs[idx]) - double(q[idx]);
+            acc[0] += d * d;
+        }
+    }
+    block_sum_store<1>(acc, partials + blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void copy_box_kernel(const float* __restrict__ q, long long cs, long long off,
+                                                       long long nvox, float* __restrict__ dst) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) dst[(long long)c * nvox + v] = q[(long long)c * cs + off + v];
+}
+
+unsigned blocks_for(long long n) { return unsigned((n + 255) / 256); }
+unsigned reduce_blocks(long long n) { return std::max(1u, std::min(blocks_for(n), 2048u)); }
+
+template <int CP, int NB, int EPI>
+hipError_t go_conv(hipStream_t st, const ConvArgs& a) {
+    constexpr int S = 4;
+    const size_t lds = size_t(NB) * ksteps(CP) * 64 * 16;
+    const long long groups = (a.nseg + 4 * S - 1) / (4 * S);
+    const unsigned grid = unsigned(std::max(1LL, std::min(groups, 2048LL)));
+    hipLaunchKernelGGL((conv_kernel<CP, NB, EPI>), dim3(grid), dim3(256), lds, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+namespace qsrs {
+
+hipError_t launch_conv(int cp, int nb, int epi, hipStream_t st, const ConvArgs& a) {
+    if (cp == 8 && nb == 2 && epi == EPI_HIDDEN) return go_conv<8, 2, EPI_HIDDEN>(st, a);
+    if (cp == 32 && nb == 2 && epi == EPI_HIDDEN) return go_conv<32, 2, EPI_HIDDEN>(st, a);
+    if (cp == 32 && nb == 1 && epi == EPI_PRED) return go_conv<32, 1, EPI_PRED>(st, a);
+    if (cp == 32 && nb == 1 && epi == EPI_APPLY) return go_conv<32, 1, EPI_APPLY>(st, a);
+    if (cp == 8 && nb == 2 && epi == EPI_BWD) return go_conv<8, 2, EPI_BWD>(st, a);
+    if (cp == 32 && nb == 2 && epi == EPI_BWD) return go_conv<32, 2, EPI_BWD>(st, a);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_wgrad(int cp, int nb, hipStream_t st, const uint16_t* act_cm, const uint16_t* dz_cm, int X, int Y,
+                        int Z, long long V, float* part) {
+    if (Z % 8 != 0) return hipErrorInvalidValue;
+    if (cp == 8 && nb == 2)
+        hipLaunchKernelGGL((wgrad_kernel<8, 2>), dim3(WG_BLOCKS), dim3(256), 0, st, act_cm, dz_cm, X, Y, Z, V, part);
+    else if (cp == 32 && nb == 2)
+        hipLaunchKernelGGL((wgrad_kernel<32, 2>), dim3(WG_BLOCKS), dim3(256), 0, st, act_cm, dz_cm, X, Y, Z, V, part);
+    else if (cp == 32 && nb == 1)
+        hipLaunchKernelGGL((wgrad_kernel<32, 1>), dim3(WG_BLOCKS), dim3(256), 0, st, act_cm, dz_cm, X, Y, Z, V, part);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_wgrad_reduce(int cp, int nb, int cin_real, int cout_real, long long w_off, long long b_off,
+                               hipStream_t st, const float* part, float* grad) {
+    const int mbt = (TAPS * cp + 1 + 15) / 16;
+    const long long total = (long long)(TAPS * cp + 1) * cout_real;
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(blocks_for(total)), dim3(256), 0, st, part, WG_BLOCKS, cp, mbt,
+                       nb * 16, cin_real, cout_real, w_off, b_off, grad);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_frags(hipStream_t st, const float* params, Frags f) {
+    const long long n = frag_elems(8, 2) + frag_elems(32, 2) + frag_elems(32, 1) + frag_elems(8, 2) +
+                        frag_elems(32, 2) + 32 + 32 + 16;
+    hipLaunchKernelGGL(pack_frags_kernel, dim3(blocks_for(n)), dim3(256), 0, st, params, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(hipStream_t st, float* params, const float* grad, float* m, float* v, long long n, float lr_t,
+                       float b1, float b2, float eps, float inv_sqrt_bc2) {
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n)), dim3(256), 0, st, params, grad, m, v, n, lr_t, b1, b2, eps,
+                       inv_sqrt_bc2);
+    return hipGetLastError();
+}
+
+hipError_t launch_crop(hipStream_t st, const Snap* snaps, const CropJob* jobs, int nbatch, int C, const Work& w) {
+    (void)nbatch;
+    hipLaunchKernelGGL(crop_kernel, dim3(blocks_for(w.V)), dim3(256), 0, st, snaps, jobs, C, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_batch(hipStream_t st, const float* x5, const float* t5, const float* m, long long V,
+                             const Work& w) {
+    (void)V;
+    hipLaunchKernelGGL(pack_batch_kernel, dim3(blocks_for(w.V)), dim3(256), 0, st, x5, t5, m, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_field(hipStream_t st, const float* q, long long cs, long long off, long long V, uint16_t* x) {
+    hipLaunchKernelGGL(pack_field_kernel, dim3(blocks_for(V)), dim3(256), 0, st, q, cs, off, V, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack_pred(hipStream_t st, const float* pred, long long V, float* out5) {
+    hipLaunchKernelGGL(unpack_pred_kernel, dim3(blocks_for(V)), dim3(256), 0, st, pred, V, out5);
+    return hipGetLastError();
+}
+
+hipError_t launch_loss(hipStream_t st, const Work& w, float scale, long long* nblk) {
+    const unsigned g = reduce_blocks(w.V);
+    *nblk = g;
+    hipLaunchKernelGGL(loss_kernel, dim3(g), dim3(256), 0, st, w, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_slots(hipStream_t st, const double* partials, long long nblk, int nslot, double* out) {
+    hipLaunchKernelGGL(sum_slots_kernel, dim3(1), dim3(256), 0, st, partials, nblk, nslot, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mask_bytes(hipStream_t st, long long nvox, const qsrk::BoxDev& box, const qsrk::RowMask* rows,
+                             uint8_t* out) {
+    hipLaunchKernelGGL(mask_bytes_kernel, dim3(blocks_for(nvox)), dim3(256), 0, st, nvox, box, rows, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(hipStream_t st, const float* q, const float* rhs, long long cs, const qsrk::BoxDev& box,
+                           const qsrk::RowMask* rows, double* partials, long long* nblk) {
+    const long long nvox = (long long)box.Nx * box.Ny * box.Nz;
+    const unsigned g = reduce_blocks(nvox);
+    *nblk = g;
+    hipLaunchKernelGGL(residual_kernel, dim3(g), dim3(256), 0, st, q, rhs, cs, box, rows, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_box(hipStream_t st, const float* q, long long cs, long long off, long long nvox, float* dst) {
+    hipLaunchKernelGGL(copy_box_kernel, dim3(blocks_for(nvox)), dim3(256), 0, st, q, cs, off, nvox, dst);
+    return hipGetLastError();
+}
+
+}  // namespace qsrs

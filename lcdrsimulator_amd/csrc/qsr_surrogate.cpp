@@ -1,0 +1,630 @@
+// qsr_surrogate.cpp -- the C ABI of the learned warm-start step
+// (include/qsr_amd.h, qsr_surr_*): parameters, training on (Q_k, Q_k+N)
+// pairs that the plain solver produces, and the accepted-or-rejected
+// application to a solver context.  The kernels are in qsr_surr.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/qsr_amd.h"
+#include "qsr_ctx.hpp"
+#include "qsr_surr.hpp"
+
+namespace {
+
+using namespace qsrs;
+
+int fail(int code, const std::string& msg) { return qsrc::set_error(code, msg); }
+
+#define HIP_TRY(expr)                                                                                \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess) return fail(QSR_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+inline void discard(hipError_t) {}
+
+// splitmix64 (Steele, Lea & Flood 2014): crop sampling and initial weights
+struct SplitMix {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return double(next() >> 11) * (1.0 / 9007199254740992.0); }
+    int below(int n) { return int(next() % uint64_t(n)); }
+};
+
+struct SnapH {
+    float* q = nullptr;
+    uint8_t* mask = nullptr;
+    int X = 0, Y = 0, Z = 0;
+    long long nvox = 0;
+};
+
+struct Net {
+    int device = 0;
+    hipStream_t st = nullptr;
+    float *P = nullptr, *G = nullptr, *M = nullptr, *Vm = nullptr;
+    Frags fr{};
+    void* frag_mem = nullptr;
+    long long t = 0;  // Adam steps taken
+    bool has_params = false;
+    // training / evaluation workspace
+    Work w{};
+    long long w_cap = 0;
+    void* w_mem = nullptr;
+    float* stage = nullptr;  // host batch upload (x5, t5, mask)
+    long long stage_cap = 0;
+    double* d_out = nullptr;  // 2 doubles per training iteration (loss partial sums)
+    long long out_cap = 0;
+    // application workspace
+    uint16_t *ax = nullptr, *aa1 = nullptr, *aa2 = nullptr;
+    long long a_cap = 0;
+    // dataset
+    std::vector<SnapH> snaps;
+    Snap* d_snaps = nullptr;
+    size_t d_snaps_n = 0;
+    CropJob* d_jobs = nullptr;
+    long long jobs_cap = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+
+    ~Net() {
+        for (SnapH& s : snaps) {
+            discard(hipFree(s.q));
+            discard(hipFree(s.mask));
+        }
+        for (void* p : {(void*)P, (void*)G, (void*)M, (void*)Vm, frag_mem, w_mem, (void*)stage, (void*)d_out,
+                        (void*)ax, (void*)aa1, (void*)aa2, (void*)d_snaps, (void*)d_jobs})
+            discard(hipFree(p));
+        for (hipEvent_t e : ev)
+            if (e) discard(hipEventDestroy(e));
+        if (st) discard(hipStreamDestroy(st));
+    }
+};
+
+Net* as_net(void* p) { return static_cast<Net*>(p); }
+
+int ensure_work(Net& n, long long V) {
+    if (V <= n.w_cap) {
+        n.w.V = V;
+        return QSR_OK;
+    }
+    discard(hipFree(n.w_mem));
+    n.w_mem = nullptr;
+    n.w_cap = 0;
+    // element sizes per voxel, 256-byte aligned sub-buffers
+    const long long b16_8 = V * 8 * 2, b16_32 = V * 32 * 2, f32_8 = V * 8 * 4, f32_32 = V * 32 * 4, f32_1 = V * 4;
+    auto al = [](long long b) { return (b + 255) / 256 * 256; };
+    const long long sizes[] = {b16_8,  b16_8,  f32_32, f32_32, b16_32, b16_32, b16_32, b16_32, f32_8, f32_8,
+                               f32_1,  b16_8,  b16_8,  b16_32, b16_32, b16_32, b16_32, part_floats() * 4,
+                               2048 * 2 * 8};
+    long long total = 0;
+    for (long long s : sizes) total += al(s);
+    HIP_TRY(hipMalloc(&n.w_mem, total));
+    char* p = static_cast<char*>(n.w_mem);
+    void* ptrs[19];
+    for (int i = 0; i < 19; ++i) {
+        ptrs[i] = p;
+        p += al(sizes[i]);
+    }
+    Work& w = n.w;
+    w.x = (uint16_t*)ptrs[0];
+    w.x_cm = (uint16_t*)ptrs[1];
+    w.z1 = (float*)ptrs[2];
+    w.z2 = (float*)ptrs[3];
+    w.a1 = (uint16_t*)ptrs[4];
+    w.a1_cm = (uint16_t*)ptrs[5];
+    w.a2 = (uint16_t*)ptrs[6];
+    w.a2_cm = (uint16_t*)ptrs[7];
+    w.pred = (float*)ptrs[8];
+    w.tgt = (float*)ptrs[9];
+    w.mask = (float*)ptrs[10];
+    w.dy = (uint16_t*)ptrs[11];
+    w.dy_cm = (uint16_t*)ptrs[12];
+    w.dz2 = (uint16_t*)ptrs[13];
+    w.dz2_cm = (uint16_t*)ptrs[14];
+    w.dz1 = (uint16_t*)ptrs[15];
+    w.dz1_cm = (uint16_t*)ptrs[16];
+    w.part = (float*)ptrs[17];
+    w.red = (double*)ptrs[18];
+    w.V = V;
+    n.w_cap = V;
+    return QSR_OK;
+}
+
+int ensure_out(Net& n, long long doubles) {
+    if (doubles <= n.out_cap) return QSR_OK;
+    discard(hipFree(n.d_out));
+    n.d_out = nullptr;
+    HIP_TRY(hipMalloc(&n.d_out, sizeof(double) * doubles));
+    n.out_cap = doubles;
+    return QSR_OK;
+}
+
+ConvArgs conv_dims(int nb, int X, int Y, int Z) {
+    ConvArgs a{};
+    a.n = nb;
+    a.X = X;
+    a.Y = Y;
+    a.Z = Z;
+    a.V = (long long)nb * X * Y * Z;
+    a.segs_row = (Z + 15) / 16;
+    a.nseg = (long long)nb * X * Y * a.segs_row;
+    return a;
+}
+
+// Forward over the workspace batch; with `train` the pre-activations and the
+// channel-major copies the backward pass needs are kept.
+int forward(Net& n, int nb, int X, int Y, int Z, bool train) {
+    Work& w = n.w;
+    ConvArgs a = conv_dims(nb, X, Y, Z);
+    a.in = w.x;
+    a.frag = n.fr.f1;
+    a.bias = n.fr.b1;
+    a.z_out = train ? w.z1 : nullptr;
+    a.a_out = w.a1;
+    a.a_cm = train ? w.a1_cm : nullptr;
+    HIP_TRY(launch_conv(8, 2, EPI_HIDDEN, n.st, a));
+    a.in = w.a1;
+    a.frag = n.fr.f2;
+    a.bias = n.fr.b2;
+    a.z_out = train ? w.z2 : nullptr;
+    a.a_out = w.a2;
+    a.a_cm = train ? w.a2_cm : nullptr;
+    HIP_TRY(launch_conv(32, 2, EPI_HIDDEN, n.st, a));
+    a.in = w.a2;
+    a.frag = n.fr.f3;
+    a.bias = n.fr.b3;
+    a.z_out = nullptr;
+    a.a_out = nullptr;
+    a.a_cm = nullptr;
+    a.pred = w.pred;
+    HIP_TRY(launch_conv(32, 1, EPI_PRED, n.st, a));
+    return QSR_OK;
+}
+
+// Loss partial sums into out[0..1] (sum m (y-t)^2, sum m) and dy.
+int loss(Net& n, double* out) {
+    long long nblk = 0;
+    const float scale = float(2.0 / (5.0 * double(n.w.V)));
+    HIP_TRY(launch_loss(n.st, n.w, scale, &nblk));
+    HIP_TRY(launch_sum_slots(n.st, n.w.red, nblk, 2, out));
+    return QSR_OK;
+}
+
+// Backward from dy into the flat gradient G.
+int backward(Net& n, int nb, int X, int Y, int Z) {
+    Work& w = n.w;
+    const long long V = w.V;
+    HIP_TRY(launch_wgrad(32, 1, n.st, w.a2_cm, w.dy_cm, X, Y, Z, V, w.part));
+    HIP_TRY(launch_wgrad_reduce(32, 1, CH_H, CH_Q, OFF_W3, OFF_B3, n.st, w.part, n.G));
+    ConvArgs a = conv_dims(nb, X, Y, Z);
+    a.in = w.dy;
+    a.frag = n.fr.g3;
+    a.z_in = w.z2;
+    a.a_out = w.dz2;
+    a.a_cm = w.dz2_cm;
+    HIP_TRY(launch_conv(8, 2, EPI_BWD, n.st, a));
+    HIP_TRY(launch_wgrad(32, 2, n.st, w.a1_cm, w.dz2_cm, X, Y, Z, V, w.part));
+    HIP_TRY(launch_wgrad_reduce(32, 2, CH_H, CH_H, OFF_W2, OFF_B2, n.st, w.part, n.G));
+    a.in = w.dz2;
+    a.frag = n.fr.g2;
+    a.z_in = w.z1;
+    a.a_out = w.dz1;
+    a.a_cm = w.dz1_cm;
+    HIP_TRY(launch_conv(32, 2, EPI_BWD, n.st, a));
+    HIP_TRY(launch_wgrad(8, 2, n.st, w.x_cm, w.dz1_cm, X, Y, Z, V, w.part));
+    HIP_TRY(launch_wgrad_reduce(8, 2, CH_Q, CH_H, OFF_W1, OFF_B1, n.st, w.part, n.G));
+    return QSR_OK;
+}
+
+int adam(Net& n, double lr) {
+    constexpr double b1 = 0.9, b2 = 0.999, eps = 1e-8;
+    n.t += 1;
+    const double bc1 = 1.0 - std::pow(b1, double(n.t)), bc2 = 1.0 - std::pow(b2, double(n.t));
+    HIP_TRY(launch_adam(n.st, n.P, n.G, n.M, n.Vm, NPARAM, float(lr / bc1), float(b1), float(b2), float(eps),
+                        float(1.0 / std::sqrt(bc2))));
+    HIP_TRY(launch_pack_frags(n.st, n.P, n.fr));
+    return QSR_OK;
+}
+
+int check_net(Net* n) {
+    if (!n) return fail(QSR_E_ARG, "null network");
+    if (!n->has_params) return fail(QSR_E_STATE, "network has no parameters (init, set or load them first)");
+    return QSR_OK;
+}
+
+int upload_params(Net& n, const float* host) {
+    HIP_TRY(hipSetDevice(n.device));
+    HIP_TRY(hipMemcpyAsync(n.P, host, sizeof(float) * NPARAM, hipMemcpyHostToDevice, n.st));
+    HIP_TRY(hipMemsetAsync(n.M, 0, sizeof(float) * NPARAM, n.st));
+    HIP_TRY(hipMemsetAsync(n.Vm, 0, sizeof(float) * NPARAM, n.st));
+    HIP_TRY(launch_pack_frags(n.st, n.P, n.fr));
+    HIP_TRY(hipStreamSynchronize(n.st));
+    n.t = 0;
+    n.has_params = true;
+    return QSR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsr_surr_num_params(size_t* n) {
+    if (!n) return fail(QSR_E_ARG, "null argument");
+    *n = size_t(NPARAM);
+    return QSR_OK;
+}
+
+int qsr_surr_create(int device, void** out) {
+    if (!out) return fail(QSR_E_ARG, "null argument");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(QSR_E_ARG, "bad device " + std::to_string(device));
+    HIP_TRY(hipSetDevice(device));
+    auto n = std::make_unique<Net>();
+    n->device = device;
+    HIP_TRY(hipStreamCreateWithFlags(&n->st, hipStreamNonBlocking));
+    for (hipEvent_t& e : n->ev) HIP_TRY(hipEventCreate(&e));
+    for (float** p : {&n->P, &n->G, &n->M, &n->Vm}) HIP_TRY(hipMalloc(p, sizeof(float) * NPARAM));
+    HIP_TRY(hipMemsetAsync(n->G, 0, sizeof(float) * NPARAM, n->st));
+    const long long fe[5] = {frag_elems(8, 2), frag_elems(32, 2), frag_elems(32, 1), frag_elems(8, 2),
+                             frag_elems(32, 2)};
+    long long bytes = 0;
+    for (long long e : fe) bytes += e * 2;
+    bytes += (32 + 32 + 16) * 4;
+    HIP_TRY(hipMalloc(&n->frag_mem, bytes));
+    char* p = static_cast<char*>(n->frag_mem);
+    uint16_t** fp[5] = {&n->fr.f1, &n->fr.f2, &n->fr.f3, &n->fr.g3, &n->fr.g2};
+    for (int i = 0; i < 5; ++i) {
+        *fp[i] = reinterpret_cast<uint16_t*>(p);
+        p += fe[i] * 2;
+    }
+    n->fr.b1 = reinterpret_cast<float*>(p);
+    n->fr.b2 = n->fr.b1 + 32;
+    n->fr.b3 = n->fr.b2 + 32;
+    HIP_TRY(hipStreamSynchronize(n->st));
+    *out = n.release();
+    return QSR_OK;
+}
+
+int qsr_surr_destroy(void* net) {
+    Net* n = as_net(net);
+    if (!n) return QSR_OK;
+    discard(hipSetDevice(n->device));
+    discard(hipStreamSynchronize(n->st));
+    delete n;
+    return QSR_OK;
+}
+
+int qsr_surr_init_params(void* net, uint64_t seed) {
+    Net* n = as_net(net);
+    if (!n) return fail(QSR_E_ARG, "null network");
+    // uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) for weights and biases; the last
+    // layer scaled by 0.1 so the untrained step starts close to "no change"
+    std::vector<float> P(NPARAM);
+    SplitMix r{seed};
+    auto fill = [&](long long off, long long cnt, int fan_in, double scale) {
+        const double a = scale / std::sqrt(double(fan_in));
+        for (long long i = 0; i < cnt; ++i) P[off + i] = float((2.0 * r.uniform() - 1.0) * a);
+    };
+    fill(OFF_W1, OFF_B1 - OFF_W1, TAPS * CH_Q, 1.0);
+    fill(OFF_B1, CH_H, TAPS * CH_Q, 1.0);
+    fill(OFF_W2, OFF_B2 - OFF_W2, TAPS * CH_H, 1.0);
+    fill(OFF_B2, CH_H, TAPS * CH_H, 1.0);
+    fill(OFF_W3, OFF_B3 - OFF_W3, TAPS * CH_H, 0.1);
+    fill(OFF_B3, CH_Q, TAPS * CH_H, 0.0);
+    return upload_params(*n, P.data());
+}
+
+int qsr_surr_set_params(void* net, const float* params, size_t count) {
+    Net* n = as_net(net);
+    if (!n || !params) return fail(QSR_E_ARG, "null argument");
+    if ((long long)count != NPARAM) return fail(QSR_E_ARG, "expected " + std::to_string(NPARAM) + " parameters");
+    for (size_t i = 0; i < count; ++i)
+        if (!std::isfinite(params[i])) return fail(QSR_E_ARG, "non-finite parameter " + std::to_string(i));
+    return upload_params(*n, params);
+}
+
+int qsr_surr_get_params(void* net, float* params, size_t count) {
+    Net* n = as_net(net);
+    if (int rc = check_net(n)) return rc;
+    if (!params || (long long)count != NPARAM) return fail(QSR_E_ARG, "bad buffer");
+    HIP_TRY(hipSetDevice(n->device));
+    HIP_TRY(hipMemcpyAsync(params, n->P, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, n->st));
+    HIP_TRY(hipStreamSynchronize(n->st));
+    return QSR_OK;
+}
+
+int qsr_surr_save(void* net, const char* path) {
+    Net* n = as_net(net);
+    if (int rc = check_net(n)) return rc;
+    if (!path) return fail(QSR_E_ARG, "null path");
+    std::vector<float> P(NPARAM);
+    if (int rc = qsr_surr_get_params(net, P.data(), P.size())) return rc;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(QSR_E_ARG, std::string("cannot write ") + path);
+    const char magic[8] = {'Q', 'S', 'R', 'S', 'U', 'R', 'R', '1'};
+    const int64_t cnt = NPARAM;
+    bool ok = std::fwrite(magic, 1, 8, f) == 8 && std::fwrite(&cnt, sizeof cnt, 1, f) == 1 &&
+              std::fwrite(P.data(), sizeof(float), P.size(), f) == P.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return fail(QSR_E_ARG, std::string("short write to ") + path);
+    return QSR_OK;
+}
+
+int qsr_surr_load(void* net, const char* path) {
+    Net* n = as_net(net);
+    if (!n || !path) return fail(QSR_E_ARG, "null argument");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(QSR_E_ARG, std::string("cannot read ") + path);
+    char magic[8] = {};
+    int64_t cnt = 0;
+    std::vector<float> P(NPARAM);
+    const bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "QSRSURR1", 8) == 0 &&
+                    std::fread(&cnt, sizeof cnt, 1, f) == 1 && cnt == NPARAM &&
+                    std::fread(P.data(), sizeof(float), P.size(), f) == P.size();
+    std::fclose(f);
+    if (!ok) return fail(QSR_E_ARG, std::string("not a network file of this layout: ") + path);
+    return qsr_surr_set_params(net, P.data(), P.size());
+}
+
+int qsr_surr_eval(void* net, int nbatch, int X, int Y, int Z, const float* x5, const float* target5,
+                  const float* mask, double* loss_out, float* pred5, float* grad) {
+    Net* n = as_net(net);
+    if (int rc = check_net(n)) return rc;
+    if (nbatch < 1 || X < 1 || Y < 1 || Z < 1 || !x5) return fail(QSR_E_ARG, "bad batch");
+    if ((loss_out || grad) && Z % 8 != 0) return fail(QSR_E_ARG, "Z must be a multiple of 8 for the loss");
+    HIP_TRY(hipSetDevice(n->device));
+    const long long V = (long long)nbatch * X * Y * Z;
+    if (int rc = ensure_work(*n, V)) return rc;
+    if (int rc = ensure_out(*n, 2)) return rc;
+    const long long need = V * 11;
+    if (need > n->stage_cap) {
+        discard(hipFree(n->stage));
+        n->stage = nullptr;
+        n->stage_cap = 0;
+        HIP_TRY(hipMalloc(&n->stage, sizeof(float) * need));
+        n->stage_cap = need;
+    }
+    float* dx = n->stage;
+    float* dt = n->stage + V * 5;
+    float* dm = n->stage + V * 10;
+    HIP_TRY(hipMemcpyAsync(dx, x5, sizeof(float) * V * 5, hipMemcpyHostToDevice, n->st));
+    if (target5) HIP_TRY(hipMemcpyAsync(dt, target5, sizeof(float) * V * 5, hipMemcpyHostToDevice, n->st));
+    if (mask) HIP_TRY(hipMemcpyAsync(dm, mask, sizeof(float) * V, hipMemcpyHostToDevice, n->st));
+    HIP_TRY(launch_pack_batch(n->st, dx, target5 ? dt : nullptr, mask ? dm : nullptr, V, n->w));
+    if (int rc = forward(*n, nbatch, X, Y, Z, grad != nullptr)) return rc;
+    if (loss_out || grad) {
+        if (int rc = loss(*n, n->d_out)) return rc;
+        if (grad) {
+            if (int rc = backward(*n, nbatch, X, Y, Z)) return rc;
+            HIP_TRY(hipMemcpyAsync(grad, n->G, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, n->st));
+        }
+        double s[2] = {0, 0};
+        HIP_TRY(hipMemcpyAsync(s, n->d_out, sizeof s, hipMemcpyDeviceToHost, n->st));
+        HIP_TRY(hipStreamSynchronize(n->st));
+        if (loss_out) *loss_out = s[0] / (5.0 * double(V));
+    }
+    if (pred5) {
+        HIP_TRY(launch_unpack_pred(n->st, n->w.pred, V, dx));
+        HIP_TRY(hipMemcpyAsync(pred5, dx, sizeof(float) * V * 5, hipMemcpyDeviceToHost, n->st));
+    }
+    HIP_TRY(hipStreamSynchronize(n->st));
+    return QSR_OK;
+}
+
+int qsr_surr_adam_step(void* net, const float* grad, double lr) {
+    Net* n = as_net(net);
+    if (int rc = check_net(n)) return rc;
+    if (!grad || !(lr > 0.0) || !std::isfinite(lr)) return fail(QSR_E_ARG, "bad argument");
+    HIP_TRY(hipSetDevice(n->device));
+    HIP_TRY(hipMemcpyAsync(n->G, grad, sizeof(float) * NPARAM, hipMemcpyHostToDevice, n->st));
+    if (int rc = adam(*n, lr)) return rc;
+    HIP_TRY(hipStreamSynchronize(n->st));
+    return QSR_OK;
+}
+
+int qsr_surr_dataset_add(void* net, void* ctx, int box, int* snapshot) {
+    Net* n = as_net(net);
+    if (!n) return fail(QSR_E_ARG, "null network");
+    qsrc::FieldView v;
+    if (int rc = qsrc::field_view(ctx, box, &v)) return rc;
+    if (!v.uploaded) return fail(QSR_E_STATE, "box has no field");
+    if (v.device != n->device) return fail(QSR_E_ARG, "network and context are on different devices");
+    HIP_TRY(hipSetDevice(n->device));
+    SnapH s;
+    s.X = v.box.Nx;
+    s.Y = v.box.Ny;
+    s.Z = v.box.Nz;
+    s.nvox = v.nvox;
+    HIP_TRY(hipMalloc(&s.q, sizeof(float) * 5 * s.nvox));
+    if (hipMalloc(&s.mask, s.nvox) != hipSuccess) {
+        discard(hipFree(s.q));
+        return fail(QSR_E_HIP, "out of device memory for the snapshot");
+    }
+    n->snaps.push_back(s);
+    // on the context's stream, ordered after its steps
+    HIP_TRY(launch_copy_box(v.st, v.cur, v.cs, v.box.off, s.nvox, s.q));
+    HIP_TRY(launch_mask_bytes(v.st, s.nvox, v.box, v.rows, s.mask));
+    HIP_TRY(hipStreamSynchronize(v.st));
+    if (snapshot) *snapshot = int(n->snaps.size()) - 1;
+    return QSR_OK;
+}
+
+int qsr_surr_dataset_size(void* net, int* count) {
+    Net* n = as_net(net);
+    if (!n || !count) return fail(QSR_E_ARG, "null argument");
+    *count = int(n->snaps.size());
+    return QSR_OK;
+}
+
+int qsr_surr_dataset_clear(void* net) {
+    Net* n = as_net(net);
+    if (!n) return fail(QSR_E_ARG, "null network");
+    HIP_TRY(hipSetDevice(n->device));
+    HIP_TRY(hipStreamSynchronize(n->st));
+    for (SnapH& s : n->snaps) {
+        discard(hipFree(s.q));
+        discard(hipFree(s.mask));
+    }
+    n->snaps.clear();
+    return QSR_OK;
+}
+
+int qsr_surr_train(void* net, const int* pairs, int npairs, int iters, int batch, int crop, double lr, uint64_t seed,
+                   double* loss_per_iter) {
+    Net* n = as_net(net);
+    if (int rc = check_net(n)) return rc;
+    if (!pairs || npairs < 1 || iters < 1 || batch < 1 || !(lr > 0.0) || !std::isfinite(lr))
+        return fail(QSR_E_ARG, "bad argument");
+    if (crop < 8 || crop % 8 != 0) return fail(QSR_E_ARG, "crop must be a positive multiple of 8");
+    const int ns = int(n->snaps.size());
+    for (int p = 0; p < npairs; ++p) {
+        const int a = pairs[2 * p], b = pairs[2 * p + 1];
+        if (a < 0 || a >= ns || b < 0 || b >= ns) return fail(QSR_E_ARG, "pair " + std::to_string(p) + ": bad snapshot");
+        const SnapH &sa = n->snaps[a], &sb = n->snaps[b];
+        if (sa.X != sb.X || sa.Y != sb.Y || sa.Z != sb.Z)
+            return fail(QSR_E_ARG, "pair " + std::to_string(p) + ": snapshots differ in size");
+        if (sa.X < crop || sa.Y < crop || sa.Z < crop)
+            return fail(QSR_E_ARG, "pair " + std::to_string(p) + ": box smaller than the crop");
+    }
+    HIP_TRY(hipSetDevice(n->device));
+    const long long V = (long long)batch * crop * crop * crop;
+    if (int rc = ensure_work(*n, V)) return rc;
+    if (int rc = ensure_out(*n, 2LL * iters)) return rc;
+    if (n->d_snaps_n != n->snaps.size()) {
+        discard(hipFree(n->d_snaps));
+        n->d_snaps = nullptr;
+        std::vector<Snap> hs(n->snaps.size());
+        for (size_t i = 0; i < hs.size(); ++i) {
+            const SnapH& s = n->snaps[i];
+            hs[i] = Snap{s.q, s.mask, s.X, s.Y, s.Z, 0, s.nvox};
+        }
+        HIP_TRY(hipMalloc(&n->d_snaps, sizeof(Snap) * hs.size()));
+        HIP_TRY(hipMemcpy(n->d_snaps, hs.data(), sizeof(Snap) * hs.size(), hipMemcpyHostToDevice));
+        n->d_snaps_n = hs.size();
+    }
+    const long long njobs = (long long)iters * batch;
+    if (njobs > n->jobs_cap) {
+        discard(hipFree(n->d_jobs));
+        n->d_jobs = nullptr;
+        n->jobs_cap = 0;
+        HIP_TRY(hipMalloc(&n->d_jobs, sizeof(CropJob) * njobs));
+        n->jobs_cap = njobs;
+    }
+    std::vector<CropJob> jobs(njobs);
+    SplitMix r{seed};
+    for (long long j = 0; j < njobs; ++j) {
+        const int p = r.below(npairs);
+        const SnapH& s = n->snaps[pairs[2 * p]];
+        jobs[j] = CropJob{pairs[2 * p], pairs[2 * p + 1], r.below(s.X - crop + 1), r.below(s.Y - crop + 1),
+                          r.below(s.Z - crop + 1), 0};
+    }
+    HIP_TRY(hipStreamSynchronize(n->st));
+    HIP_TRY(hipMemcpy(n->d_jobs, jobs.data(), sizeof(CropJob) * njobs, hipMemcpyHostToDevice));
+    for (int it = 0; it < iters; ++it) {
+        HIP_TRY(launch_crop(n->st, n->d_snaps, n->d_jobs + (long long)it * batch, batch, crop, n->w));
+        if (int rc = forward(*n, batch, crop, crop, crop, true)) return rc;
+        if (int rc = loss(*n, n->d_out + 2LL * it)) return rc;
+        if (int rc = backward(*n, batch, crop, crop, crop)) return rc;
+        if (int rc = adam(*n, lr)) return rc;
+    }
+    std::vector<double> h(2 * size_t(iters));
+    HIP_TRY(hipMemcpyAsync(h.data(), n->d_out, sizeof(double) * h.size(), hipMemcpyDeviceToHost, n->st));
+    HIP_TRY(hipStreamSynchronize(n->st));
+    if (loss_per_iter)
+        for (int it = 0; it < iters; ++it)
+            loss_per_iter[it] = h[2 * it + 1] > 0 ? h[2 * it] / (5.0 * h[2 * it + 1]) : 0.0;
+    return QSR_OK;
+}
+
+int qsr_surr_apply(void* net, void* ctx, int mode, qsr_surr_result* out) {
+    Net* n = as_net(net);
+    if (int rc = check_net(n)) return rc;
+    if (!out || mode < QSR_SURR_CHECKED || mode > QSR_SURR_PROPOSE) return fail(QSR_E_ARG, "bad argument");
+    qsrc::FieldView v;
+    if (int rc = qsrc::field_view(ctx, 0, &v)) return rc;
+    if (v.nboxes != 1) return fail(QSR_E_ARG, "the learned step applies to single-box contexts");
+    if (!v.uploaded) return fail(QSR_E_STATE, "box has no field");
+    if (v.device != n->device) return fail(QSR_E_ARG, "network and context are on different devices");
+    HIP_TRY(hipSetDevice(n->device));
+    HIP_TRY(hipStreamSynchronize(n->st));  // parameters last written on the network's stream
+    qsr_surr_result r{};
+    const qsr_obs_opts oo = {-1, 0.1, 0.25, -1, -1.0, 0, 0};
+    qsr_obs o{};
+    if (int rc = qsr_observables(ctx, 0, &oo, &o)) return rc;
+    r.F_before = o.total;
+    if (int rc = qsrc::euler_residual(ctx, 0, &r.res_before)) return rc;
+
+    const long long V = v.nvox;
+    if (V > n->a_cap) {
+        for (uint16_t** p : {&n->ax, &n->aa1, &n->aa2}) {
+            discard(hipFree(*p));
+            *p = nullptr;
+        }
+        n->a_cap = 0;
+        HIP_TRY(hipMalloc(&n->ax, sizeof(uint16_t) * 8 * V));
+        HIP_TRY(hipMalloc(&n->aa1, sizeof(uint16_t) * 32 * V));
+        HIP_TRY(hipMalloc(&n->aa2, sizeof(uint16_t) * 32 * V));
+        n->a_cap = V;
+    }
+    // the network runs on the context's stream, after its steps
+    const hipStream_t st = v.st;
+    ConvArgs a = conv_dims(1, v.box.Nx, v.box.Ny, v.box.Nz);
+    HIP_TRY(hipEventRecord(n->ev[0], st));
+    HIP_TRY(launch_pack_field(st, v.cur, v.cs, v.box.off, V, n->ax));
+    a.in = n->ax;
+    a.frag = n->fr.f1;
+    a.bias = n->fr.b1;
+    a.a_out = n->aa1;
+    HIP_TRY(launch_conv(8, 2, EPI_HIDDEN, st, a));
+    a.in = n->aa1;
+    a.frag = n->fr.f2;
+    a.bias = n->fr.b2;
+    a.a_out = n->aa2;
+    HIP_TRY(launch_conv(32, 2, EPI_HIDDEN, st, a));
+    a.in = n->aa2;
+    a.frag = n->fr.f3;
+    a.bias = n->fr.b3;
+    a.a_out = nullptr;
+    a.q_old = v.cur;
+    a.q_new = v.other;
+    a.cs = v.cs;
+    a.box = v.box;
+    a.rows = v.rows;
+    HIP_TRY(launch_conv(32, 1, EPI_APPLY, st, a));
+    HIP_TRY(hipEventRecord(n->ev[1], st));
+    HIP_TRY(hipEventSynchronize(n->ev[1]));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, n->ev[0], n->ev[1]));
+    r.ms_network = ms;
+
+    if (int rc = qsrc::commit_other(ctx)) return rc;
+    if (int rc = qsr_observables(ctx, 0, &oo, &o)) return rc;
+    r.F_after = o.total;
+    if (int rc = qsrc::euler_residual(ctx, 0, &r.res_after)) return rc;
+    bool keep = false;
+    if (mode == QSR_SURR_FORCE) keep = true;
+    else if (mode == QSR_SURR_CHECKED)
+        keep = std::isfinite(r.F_after) && std::isfinite(r.res_after) && r.F_after <= r.F_before &&
+               r.res_after <= r.res_before;
+    if (!keep) {
+        // the previous field is intact in the other buffer
+        if (int rc = qsrc::commit_other(ctx)) return rc;
+    }
+    r.accepted = keep ? 1 : 0;
+    *out = r;
+    return QSR_OK;
+}
+
+}  // extern "C"

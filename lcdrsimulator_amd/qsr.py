@@ -74,7 +74,17 @@ EXPORTS = (
     "qsr_step_imex", "qsr_observables", "qsr_box_info_get", "qsr_set_iteration", "qsr_last_step_ms",
     "qsr_step_work", "qsr_profile_steps", "qsr_autotune", "qsr_set_tuning", "qsr_S_eq", "qsr_dt_estimate", "qsr_init_field",
     "qsr_load_qtensor_file", "qsr_save_qtensor_file", "qsr_set_box_active",
+    "qsr_surr_num_params", "qsr_surr_create", "qsr_surr_destroy", "qsr_surr_init_params", "qsr_surr_set_params",
+    "qsr_surr_get_params", "qsr_surr_save", "qsr_surr_load", "qsr_surr_eval", "qsr_surr_adam_step",
+    "qsr_surr_dataset_add", "qsr_surr_dataset_size", "qsr_surr_dataset_clear", "qsr_surr_train", "qsr_surr_apply",
 )
+
+SURR_CHECKED, SURR_FORCE, SURR_PROPOSE = 0, 1, 2
+
+
+class _SurrResult(C.Structure):
+    _fields_ = [("F_before", C.c_double), ("F_after", C.c_double), ("res_before", C.c_double),
+                ("res_after", C.c_double), ("accepted", C.c_int32), ("ms_network", C.c_double)]
 
 
 def load():
@@ -113,6 +123,23 @@ def load():
                                         C.POINTER(C.c_uint64)]
     L.qsr_save_qtensor_file.argtypes = [C.c_char_p, dp, C.c_int, C.c_int, C.c_int, C.c_size_t]
     L.qsr_device_count.argtypes = [C.POINTER(C.c_int)]
+    fp = C.POINTER(C.c_float)
+    L.qsr_surr_num_params.argtypes = [C.POINTER(C.c_size_t)]
+    L.qsr_surr_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.qsr_surr_destroy.argtypes = [vp]
+    L.qsr_surr_init_params.argtypes = [vp, C.c_uint64]
+    L.qsr_surr_set_params.argtypes = [vp, fp, C.c_size_t]
+    L.qsr_surr_get_params.argtypes = [vp, fp, C.c_size_t]
+    L.qsr_surr_save.argtypes = [vp, C.c_char_p]
+    L.qsr_surr_load.argtypes = [vp, C.c_char_p]
+    L.qsr_surr_eval.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, fp, fp, fp, dp, fp, fp]
+    L.qsr_surr_adam_step.argtypes = [vp, fp, C.c_double]
+    L.qsr_surr_dataset_add.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
+    L.qsr_surr_dataset_size.argtypes = [vp, C.POINTER(C.c_int)]
+    L.qsr_surr_dataset_clear.argtypes = [vp]
+    L.qsr_surr_train.argtypes = [vp, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                 C.c_uint64, dp]
+    L.qsr_surr_apply.argtypes = [vp, vp, C.c_int, C.POINTER(_SurrResult)]
     _lib = L
     return L
 
@@ -332,3 +359,114 @@ def save_qtensor(path, Q):
     Nx, Ny, Nz = Q.shape[:3]
     _check(load().qsr_save_qtensor_file(os.fsencode(str(path)), Q.ctypes.data_as(C.POINTER(C.c_double)), Nx, Ny, Nz,
                                         Nx * Ny * Nz))
+
+
+def _fptr(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Surrogate:
+    """The learned warm-start step: a 5 -> 32 -> 32 -> 5 channel 3x3x3 convolution
+    network on the GPU (BF16 matrix cores, FP32 accumulation) that predicts
+    dQ = Q(k+N) - Q(k) from Q(k).  Trained with its own backward and Adam kernels
+    on field pairs a Solver produced (dataset_add / train); apply() proposes the
+    step to a single-box Solver and keeps it only if neither the free energy nor
+    the explicit-Euler residual rises (SURR_CHECKED)."""
+
+    def __init__(self, device: int = 0, seed: int | None = 1):
+        L = load()
+        h = C.c_void_p()
+        _check(L.qsr_surr_create(device, C.byref(h)))
+        self._h = h
+        n = C.c_size_t()
+        _check(L.qsr_surr_num_params(C.byref(n)))
+        self.nparams = n.value
+        if seed is not None:
+            _check(L.qsr_surr_init_params(self._h, int(seed)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().qsr_surr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.empty(self.nparams, dtype=np.float32)
+        _check(load().qsr_surr_get_params(self._h, _fptr(out), self.nparams))
+        return out
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32).ravel()
+        _check(load().qsr_surr_set_params(self._h, _fptr(p), p.size))
+
+    def save(self, path):
+        _check(load().qsr_surr_save(self._h, os.fsencode(str(path))))
+
+    def load(self, path):
+        _check(load().qsr_surr_load(self._h, os.fsencode(str(path))))
+
+    def eval(self, x5, target5=None, mask=None, want_pred=True, want_grad=False):
+        """x5, target5: (n, X, Y, Z, 5); mask: (n, X, Y, Z).  Returns (loss, pred5, grad)."""
+        x5 = np.ascontiguousarray(x5, dtype=np.float32)
+        nb, X, Y, Z, c5 = x5.shape
+        if c5 != 5:
+            raise ValueError("last axis must hold the 5 Q components")
+        t5 = None if target5 is None else np.ascontiguousarray(target5, dtype=np.float32)
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.float32)
+        if t5 is not None and t5.shape != x5.shape:
+            raise ValueError("target shape mismatch")
+        if m is not None and m.shape != x5.shape[:4]:
+            raise ValueError("mask shape mismatch")
+        pred = np.empty_like(x5) if want_pred else None
+        grad = np.empty(self.nparams, dtype=np.float32) if want_grad else None
+        want_loss = t5 is not None or want_grad
+        loss = C.c_double(0.0)
+        _check(load().qsr_surr_eval(self._h, nb, X, Y, Z, _fptr(x5), _fptr(t5), _fptr(m),
+                                    C.byref(loss) if want_loss else None, _fptr(pred), _fptr(grad)))
+        return (loss.value if want_loss else None), pred, grad
+
+    def adam_step(self, grad, lr: float):
+        g = np.ascontiguousarray(grad, dtype=np.float32).ravel()
+        if g.size != self.nparams:
+            raise ValueError("gradient size mismatch")
+        _check(load().qsr_surr_adam_step(self._h, _fptr(g), float(lr)))
+
+    def dataset_add(self, solver: "Solver", box: int = 0) -> int:
+        i = C.c_int()
+        _check(load().qsr_surr_dataset_add(self._h, solver._h, box, C.byref(i)))
+        return i.value
+
+    def dataset_size(self) -> int:
+        i = C.c_int()
+        _check(load().qsr_surr_dataset_size(self._h, C.byref(i)))
+        return i.value
+
+    def dataset_clear(self):
+        _check(load().qsr_surr_dataset_clear(self._h))
+
+    def train(self, pairs, iters: int, batch: int = 4, crop: int = 32, lr: float = 1e-3, seed: int = 7) -> np.ndarray:
+        """Adam on random crops of (input, target) snapshot pairs; returns the per-iteration loss."""
+        pr = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+        hist = np.zeros(int(iters))
+        _check(load().qsr_surr_train(self._h, pr.ctypes.data_as(C.POINTER(C.c_int)), pr.shape[0], int(iters),
+                                     int(batch), int(crop), float(lr), int(seed),
+                                     hist.ctypes.data_as(C.POINTER(C.c_double))))
+        return hist
+
+    def apply(self, solver: "Solver", mode: int = SURR_CHECKED) -> dict:
+        r = _SurrResult()
+        _check(load().qsr_surr_apply(self._h, solver._h, int(mode), C.byref(r)))
+        return {f: getattr(r, f) for f, _ in _SurrResult._fields_}

@@ -1,0 +1,174 @@
+"""GPU parity of the learned warm-start step against the FP64 oracle
+(oracle/surrogate.py) evaluated with the same BF16 rounding points, and the
+behaviour of its accept / reject rule on a solver context.
+
+Tolerances: the GPU accumulates in FP32 (the oracle in FP64) and an FP32 GELU
+can round to a different BF16 neighbour than the FP64 one; both effects are
+far below these bounds:
+  forward   max |y_gpu - y_oracle| <= 2e-3 * max |y_oracle|
+  gradient  |g_gpu - g_oracle|_2 <= 5e-3 * |g_oracle|_2 per parameter block
+  loss      relative 1e-4;  Adam (FP32 state) 1e-6 absolute on the parameters
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import surrogate as S
+
+pytestmark = pytest.mark.gpu
+
+FWD_TOL = 2e-3
+GRAD_TOL = 5e-3
+BLOCKS = (("W1", S.OFF_W1, S.OFF_B1), ("b1", S.OFF_B1, S.OFF_W2), ("W2", S.OFF_W2, S.OFF_B2),
+          ("b2", S.OFF_B2, S.OFF_W3), ("W3", S.OFF_W3, S.OFF_B3), ("b3", S.OFF_B3, S.NPARAM))
+
+
+def _batch(seed, shape, scale=0.3):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(scale=scale, size=shape + (5,)).astype(np.float32)
+    t = rng.normal(scale=0.05, size=shape + (5,)).astype(np.float32)
+    m = (rng.random(shape) < 0.8).astype(np.float32)
+    return x, t, m
+
+
+@pytest.mark.parametrize("shape", [(1, 8, 8, 16), (2, 6, 7, 24), (1, 5, 9, 13), (1, 3, 4, 40), (3, 4, 4, 8)])
+def test_forward_matches_oracle(qsr, shape):
+    print(f"[surrogate forward] shape={shape}", flush=True)
+    p = S.random_params(11, scale=2.0).astype(np.float32)
+    x, _, _ = _batch(1, shape)
+    with qsr.Surrogate(seed=None) as net:
+        net.params = p
+        _, pred, _ = net.eval(x)
+    ref = S.forward(p.astype(np.float64), x.astype(np.float64), emulate=True)["y"]
+    err, scale = np.abs(pred - ref).max(), np.abs(ref).max()
+    print(f"  max err {err:.3e} of max |y| {scale:.3e}")
+    assert err <= FWD_TOL * scale
+
+
+@pytest.mark.parametrize("shape", [(1, 8, 8, 16), (2, 5, 6, 24), (1, 4, 4, 8), (1, 7, 3, 32)])
+def test_gradient_matches_oracle(qsr, shape):
+    print(f"[surrogate gradient] shape={shape}", flush=True)
+    p = S.random_params(12, scale=2.0).astype(np.float32)
+    x, t, m = _batch(2, shape)
+    with qsr.Surrogate(seed=None) as net:
+        net.params = p
+        loss, _, g = net.eval(x, t, m, want_pred=False, want_grad=True)
+    lo, go, _ = S.loss_and_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64),
+                                m.astype(np.float64), emulate=True)
+    print(f"  loss gpu {loss:.6e} oracle {lo:.6e}")
+    assert abs(loss - lo) <= 1e-4 * lo
+    for name, a, b in BLOCKS:
+        rel = np.linalg.norm(g[a:b] - go[a:b]) / np.linalg.norm(go[a:b])
+        print(f"  {name}: rel err {rel:.3e}")
+        assert rel <= GRAD_TOL, name
+
+
+def test_zero_mask_gives_zero_gradient(qsr):
+    p = S.random_params(14).astype(np.float32)
+    x, t, m = _batch(3, (1, 6, 6, 8))
+    with qsr.Surrogate(seed=None) as net:
+        net.params = p
+        loss, _, g = net.eval(x, t, np.zeros_like(m), want_pred=False, want_grad=True)
+    assert loss == 0.0 and not np.any(g)
+
+
+def test_adam_steps_match_oracle_and_repack(qsr):
+    rng = np.random.default_rng(13)
+    p = S.random_params(13).astype(np.float32)
+    po, m, v = p.astype(np.float64), np.zeros(S.NPARAM), np.zeros(S.NPARAM)
+    x, _, _ = _batch(4, (1, 6, 6, 16))
+    with qsr.Surrogate(seed=None) as net:
+        net.params = p
+        for t in range(1, 6):
+            g = (rng.normal(size=S.NPARAM) * (1e-2 if t % 2 else 1.0)).astype(np.float32)
+            net.adam_step(g, 1e-3)
+            po, m, v = S.adam(po, g.astype(np.float64), m, v, t, 1e-3)
+        got = net.params
+        _, pred, _ = net.eval(x)
+    assert np.abs(got - po).max() <= 1e-6
+    ref = S.forward(got.astype(np.float64), x.astype(np.float64), emulate=True)["y"]
+    assert np.abs(pred - ref).max() <= FWD_TOL * np.abs(ref).max()
+
+
+def test_params_roundtrip_through_file(qsr, tmp_path):
+    with qsr.Surrogate(seed=5) as a, qsr.Surrogate(seed=None) as b:
+        a.save(tmp_path / "net.bin")
+        b.load(tmp_path / "net.bin")
+        np.testing.assert_array_equal(a.params, b.params)
+        with pytest.raises(qsr.QSRError):
+            (tmp_path / "bad.bin").write_bytes(b"not a network")
+            b.load(tmp_path / "bad.bin")
+
+
+def _droplet(qsr, n=32, R=12.0, W=1e-4, L=(6e-12, 0.0, 0.0)):
+    p = qsr.Params(Nx=n, Ny=n, Nz=n, T_star=307.0, T_high=300.0, kappa=0.0, L1=L[0], L2=L[1], L3=L[2], W=W,
+                   radius_index=R, boundary_custom=1, boundary_S=0.5)
+    w = qsr.dt_estimate(p.dx, p.dy, p.dz, p.gamma, p.a, p.b, p.c, p.T_star, 1, 0.0, *L, math.nan, 0.5, 300.0, 300.0)
+    p.dt = w["dt_max"]
+    return p
+
+
+def test_apply_force_writes_q_plus_prediction_on_changeable_voxels(qsr):
+    p = _droplet(qsr)
+    Q0 = qsr.init_field(p, 0, 0.5, seed=3).astype(np.float32).astype(np.float64)
+    with qsr.Solver(p) as s, qsr.Surrogate(seed=9) as net:
+        s.upload(Q0)
+        s.step(1)
+        changeable = np.any(s.download() != Q0, axis=-1)
+        s.upload(Q0)
+        r = net.apply(s, qsr.SURR_FORCE)
+        Q1 = s.download()
+        _, pred, _ = net.eval(Q0[None].astype(np.float32))
+    assert r["accepted"] == 1
+    expect = np.where(changeable[..., None], (Q0.astype(np.float32) + pred[0]).astype(np.float64), Q0)
+    np.testing.assert_array_equal(Q1, expect)
+    print(f"  {changeable.sum()} changeable voxels, network {r['ms_network']:.3f} ms")
+
+
+def test_apply_checked_rejects_a_bad_step_and_restores(qsr):
+    p = _droplet(qsr)
+    Q0 = qsr.init_field(p, 0, 0.5, seed=4).astype(np.float32).astype(np.float64)
+    with qsr.Solver(p) as s, qsr.Surrogate(seed=None) as net:
+        net.params = S.random_params(21, scale=30.0).astype(np.float32)  # a large random step
+        s.upload(Q0)
+        s.step(20)
+        Qa = s.download()
+        F = s.observables()["total"]
+        r = net.apply(s, qsr.SURR_CHECKED)
+        assert r["accepted"] == 0
+        assert r["F_after"] > r["F_before"] or r["res_after"] > r["res_before"]
+        assert abs(r["F_before"] - F) <= 1e-12 * abs(F)
+        np.testing.assert_array_equal(s.download(), Qa)
+        r2 = net.apply(s, qsr.SURR_PROPOSE)
+        assert r2["accepted"] == 0
+        np.testing.assert_array_equal(s.download(), Qa)
+        s.step(5)  # the context keeps stepping from the restored field
+        assert np.isfinite(s.download()).all()
+
+
+def test_apply_needs_a_single_box_context(qsr):
+    p = _droplet(qsr, n=24, R=8.0)
+    with qsr.Solver([p, p]) as s, qsr.Surrogate(seed=1) as net:
+        for b in range(2):
+            s.upload(qsr.init_field(p, 0, 0.5, seed=b), box=b)
+        with pytest.raises(qsr.QSRError):
+            net.apply(s)
+        assert net.dataset_add(s, box=1) == 0  # snapshots can come from any box
+
+
+def test_training_reduces_loss(qsr):
+    p = _droplet(qsr)
+    with qsr.Solver(p) as s, qsr.Surrogate(seed=3) as net:
+        pairs = []
+        for seed in (1, 2):
+            s.upload(qsr.init_field(p, 0, 0.5, seed=seed))
+            ids = [net.dataset_add(s)]
+            for _ in range(3):
+                s.step(50)
+                ids.append(net.dataset_add(s))
+            pairs += [(ids[i], ids[i + 1]) for i in range(3)]
+        hist = net.train(pairs, iters=200, batch=4, crop=16, lr=2e-3, seed=5)
+    print(f"  loss first {hist[:5].mean():.3e} last {hist[-20:].mean():.3e}")
+    assert np.all(np.isfinite(hist))
+    assert hist[-20:].mean() < 0.5 * hist[:5].mean()
