@@ -71,8 +71,8 @@ struct Net {
     long long a_cap = 0;
     // dataset
     std::vector<SnapH> snaps;
-    Snap* d_snaps = nullptr;
-    size_t d_snaps_n = 0;
+    Snap* d_snaps = nullptr;  // device copy of the snapshot table
+    bool d_snaps_stale = true;  // set whenever the host table changes (add, clear)
     CropJob* d_jobs = nullptr;
     long long jobs_cap = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -453,6 +453,7 @@ int qsr_surr_dataset_add(void* net, void* ctx, int box, int* snapshot) {
         return fail(QSR_E_HIP, "out of device memory for the snapshot");
     }
     n->snaps.push_back(s);
+    n->d_snaps_stale = true;
     // on the context's stream, ordered after its steps
     HIP_TRY(launch_copy_box(v.st, v.cur, v.cs, v.box.off, s.nvox, s.q));
     HIP_TRY(launch_mask_bytes(v.st, s.nvox, v.box, v.rows, s.mask));
@@ -478,6 +479,11 @@ int qsr_surr_dataset_clear(void* net) {
         discard(hipFree(s.mask));
     }
     n->snaps.clear();
+    // the device table points at the buffers just freed: drop it, so training
+    // rebuilds it even when the next dataset has the same number of snapshots
+    discard(hipFree(n->d_snaps));
+    n->d_snaps = nullptr;
+    n->d_snaps_stale = true;
     return QSR_OK;
 }
 
@@ -502,7 +508,7 @@ int qsr_surr_train(void* net, const int* pairs, int npairs, int iters, int batch
     const long long V = (long long)batch * crop * crop * crop;
     if (int rc = ensure_work(*n, V)) return rc;
     if (int rc = ensure_out(*n, 2LL * iters)) return rc;
-    if (n->d_snaps_n != n->snaps.size()) {
+    if (n->d_snaps_stale) {
         discard(hipFree(n->d_snaps));
         n->d_snaps = nullptr;
         std::vector<Snap> hs(n->snaps.size());
@@ -512,7 +518,7 @@ int qsr_surr_train(void* net, const int* pairs, int npairs, int iters, int batch
         }
         HIP_TRY(hipMalloc(&n->d_snaps, sizeof(Snap) * hs.size()));
         HIP_TRY(hipMemcpy(n->d_snaps, hs.data(), sizeof(Snap) * hs.size(), hipMemcpyHostToDevice));
-        n->d_snaps_n = hs.size();
+        n->d_snaps_stale = false;
     }
     const long long njobs = (long long)iters * batch;
     if (njobs > n->jobs_cap) {

@@ -172,3 +172,29 @@ def test_training_reduces_loss(qsr):
     print(f"  loss first {hist[:5].mean():.3e} last {hist[-20:].mean():.3e}")
     assert np.all(np.isfinite(hist))
     assert hist[-20:].mean() < 0.5 * hist[:5].mean()
+
+
+def test_training_after_clear_uses_the_new_snapshots(qsr):
+    """A dataset cleared and refilled with the same number of snapshots (of another box size)
+    must be the one training reads: identical input/target pairs under a zero network give a
+    zero loss at every iteration, where the previous pairs (500 steps apart) would not."""
+    with qsr.Surrogate(seed=None) as net:
+        net.params = np.zeros(S.NPARAM, dtype=np.float32)
+        p = _droplet(qsr, n=40, R=16.0)
+        with qsr.Solver(p) as s:
+            s.upload(qsr.init_field(p, 0, 0.5, seed=8))
+            a = net.dataset_add(s)
+            s.step(500)
+            b = net.dataset_add(s)
+        first = net.train([(a, b)], iters=2, batch=2, crop=16, lr=1e-6, seed=1)
+        assert np.any(first > 0)
+        net.dataset_clear()
+        p = _droplet(qsr, n=24, R=8.0)
+        with qsr.Solver(p) as s:
+            s.upload(qsr.init_field(p, 0, 0.5, seed=9))
+            c = net.dataset_add(s)
+            d = net.dataset_add(s)
+        assert (c, d) == (0, 1)
+        net.params = np.zeros(S.NPARAM, dtype=np.float32)  # undo the first training (and its Adam state)
+        again = net.train([(c, d)], iters=3, batch=2, crop=16, lr=1e-6, seed=2)
+        np.testing.assert_array_equal(again, 0.0)
