@@ -36,7 +36,9 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-WORKLOADS = ("droplet_l1l2l3_256", "periodic_noise_256", "droplet_l1_64", "ladder_64")
+WORKLOADS = ("droplet_l1l2l3_256", "periodic_noise_256", "droplet_l1_64", "ladder_64", "surrogate_192")
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+NET_FLOP_PER_VOXEL = 72576.0  # 2 * 27 * (5*32 + 32*32 + 32*5) multiply-adds of the 5-32-32-5 network
 
 
 def ladder_params(q, n_drops=64, seed=2024):
@@ -253,6 +255,49 @@ def run_ladder(q, args):
     return 0
 
 
+def run_surrogate(q, args):
+    """Learned warm start (192^3, R = 88, T = 300 K, W = 1e-4, L1/L2/L3): train the network on plain-solver
+    trajectories, then relax the same start plainly and warm-started; value = plain time / warm time to
+    |dF/F| < 1e-6 between checks 100 steps apart (radial start; the random-director start is reported too)."""
+    from lcdrsimulator_amd import warmstart as ws
+    trajs = ws.training_set(args.surr_traj, T=300.0 if args.surr_matched else None,
+                            W=1e-4 if args.surr_matched else None)
+    with q.Surrogate(seed=3) as net:
+        tr = ws.train(net, trajs, iters_per_group=args.surr_iters, log=log)
+        p = ws.droplet_params(88.0, 300.0, 1e-4, N=192)
+        res = {m: ws.compare(p, net, m, max_steps=200000, log=log) for m in (1, 0)}
+        with q.Solver(p) as s:
+            s.upload(q.init_field(p, 1, 0.5, 0.0, 12345))
+            ms = [net.apply(s, q.SURR_PROPOSE)["ms_network"] for _ in range(5)]
+    net_ms = float(np.median(ms))
+    achieved = NET_FLOP_PER_VOXEL * p.n / (net_ms * 1e-3) / 1e12
+    r1 = res[1]
+    summary = {m: {"plain_s": r["plain"]["seconds"], "warm_s": r["warm"]["seconds"], "speedup": r["speedup"],
+                   "plain_steps": r["plain"]["steps"], "warm_plain_steps": r["warm"]["steps"],
+                   "jumps_accepted": r["warm"]["jumps_accepted"], "jumps_rejected": r["warm"]["jumps_rejected"],
+                   "rel_dF_final": r["rel_dF_final"], "defects_plain": r["plain"]["defects"],
+                   "defects_warm": r["warm"]["defects"], "same_defects": r["same_defects"]} for m, r in res.items()}
+    out = {
+        "metric": METRIC, "value": r1["speedup"], "unit": "x (time to |dF/F|<1e-6: plain / warm-started)",
+        "n_gpus": 1, "steps": r1["plain"]["steps"], "warmup": 0, "ms_per_step": 1e3 * r1["plain"]["seconds"] /
+        r1["plain"]["steps"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 step, bf16 network (fp32 accumulate)",
+        "data": "synthetic (seeded radial and random-director droplets; training pairs from the plain solver)",
+        "config": {"workload": "learned warm start: 192^3, R=88, T=300 K, W=1e-4 custom S=0.5, L1/L2/L3=6e-12/3e-12/"
+                               "1e-12; network trained on (Q_k, Q_k+500) pairs of R 40-120, " +
+                               ("T 300 K, W 1e-4 (matched to the application)" if args.surr_matched else
+                                "T 295-305 K, W 1e-5..1e-3"),
+                   "training": {k: v for k, v in tr.items() if k != "loss"},
+                   "train_loss_first_last": [float(tr["loss"][:100].mean()), float(tr["loss"][-100:].mean())],
+                   "radial_start": summary[1], "random_director_start": summary[0], "network_ms_192": net_ms},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / MFMA_BF16_PEAK_TFLOPS, "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -275,9 +320,14 @@ def main():
                          "instead of the explicit step")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
+    ap.add_argument("--surr-traj", type=int, default=256, help="surrogate_192: training trajectories")
+    ap.add_argument("--surr-matched", action="store_true",
+                    help="surrogate_192: train at the application's T and W instead of their ranges")
+    ap.add_argument("--surr-iters", type=int, default=1000,
+                    help="surrogate_192: Adam iterations per group of 32 trajectories")
     ap.add_argument("--workload", default=WORKLOADS[0], choices=WORKLOADS,
-                    help="droplet_l1l2l3_256 is the headline line; the others are the periodic noisy quench "
-                         "and the 64^3 L1 droplet")
+                    help="droplet_l1l2l3_256 is the headline line; the others are the periodic noisy quench, "
+                         "the 64^3 L1 droplet, the 64-droplet quench ladder and the learned warm start")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -302,6 +352,8 @@ def main():
         return 1
     if args.workload == "ladder_64":
         return run_ladder(q, args)
+    if args.workload == "surrogate_192":
+        return run_surrogate(q, args)
     p = workload_params(q, args.size, args.radius, args.workload)
     log(f"box {p.dims} R={p.radius_index} dt={p.dt:.5e}")
     if p.periodic:

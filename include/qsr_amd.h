@@ -239,9 +239,13 @@ int qsr_surr_dataset_clear(void* net);
 /* iters Adam steps on batches of `batch` random crop^3 windows (crop % 8 == 0)
  * of the pairs (input snapshot, target snapshot) pairs[2p], pairs[2p+1]; the
  * target is their difference.  A crop face inside its box leaves its 3-voxel
- * band unscored.  loss_per_iter (may be NULL) gets the masked mean square error. */
+ * band unscored.  Each pair's voxels are weighted by 1 / max(ms, weight_floor),
+ * ms the mean square of its target over the voxels a step can change, so the
+ * small changes late in a relaxation count more (weight_floor >= 1 weights
+ * every pair alike).  loss_per_iter (may be NULL) gets the mean over scored
+ * voxels of weight * |pred - target|^2 / 5. */
 int qsr_surr_train(void* net, const int* pairs, int npairs, int iters, int batch, int crop, double lr,
-                   uint64_t seed, double* loss_per_iter);
+                   uint64_t seed, double weight_floor, double* loss_per_iter);
 /* Propose Q + dQ for every voxel a step can change in a single-box context;
  * mode QSR_SURR_*.  The iteration counter is not advanced. */
 int qsr_surr_apply(void* net, void* ctx, int mode, qsr_surr_result* out);

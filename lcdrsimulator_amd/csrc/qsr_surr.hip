@@ -403,7 +403,7 @@ __global__ __launch_bounds__(256) void crop_kernel(const Snap* __restrict__ snap
     const int m3 = 3;
     const bool inner = (i >= m3 || J.ox == 0) && (i < C - m3 || J.ox + C == si.X) && (j >= m3 || J.oy == 0) &&
                        (j < C - m3 || J.oy + C == si.Y) && (k >= m3 || J.oz == 0) && (k < C - m3 || J.oz + C == si.Z);
-    w.mask[v] = (inner && si.mask[src]) ? 1.0f : 0.0f;
+    w.mask[v] = (inner && si.mask[src]) ? J.weight : 0.0f;
 }
 
 __global__ __launch_bounds__(256) void pack_batch_kernel(const float* __restrict__ x5, const float* __restrict__ t5,
@@ -478,7 +478,7 @@ __global__ __launch_bounds__(256) void loss_kernel(Work w, float scale) {
             acc[0] += double(mk) * double(d) * double(d);
             h[c] = bf16_bits(scale * mk * d);
         }
-        acc[1] += mk;
+        acc[1] += mk != 0.0f ? 1.0 : 0.0;  // scored voxels (mask values may be weights)
         u32x4 pk;
 #pragma unroll
         for (int c = 0; c < 4; ++c) pk[c] = unsigned(h[2 * c]) | (unsigned(h[2 * c + 1]) << 16);
@@ -504,6 +504,23 @@ __global__ __launch_bounds__(256) void sum_slots_kernel(const double* __restrict
         if (threadIdx.x == 0) out[s] = red[0];
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(256) void pair_power_kernel(const float* __restrict__ qa, const float* __restrict__ qb,
+                                                         const uint8_t* __restrict__ mask, long long nvox,
+                                                         double* __restrict__ partials) {
+    double acc[2] = {0.0, 0.0};
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
+         v += (long long)gridDim.x * blockDim.x) {
+        if (!mask[v]) continue;
+#pragma unroll
+        for (int c = 0; c < CH_Q; ++c) {
+            const double d = double(qb[c * nvox + v]) - double(qa[c * nvox + v]);
+            acc[0] += d * d;
+        }
+        acc[1] += 1.0;
+    }
+    block_sum_store<2>(acc, partials + 2 * blockIdx.x);
 }
 
 __global__ __launch_bounds__(256) void mask_bytes_kernel(long long nvox, BoxDev B, const RowMask* __restrict__ rows,
@@ -638,6 +655,14 @@ hipError_t launch_loss(hipStream_t st, const Work& w, float scale, long long* nb
 
 hipError_t launch_sum_slots(hipStream_t st, const double* partials, long long nblk, int nslot, double* out) {
     hipLaunchKernelGGL(sum_slots_kernel, dim3(1), dim3(256), 0, st, partials, nblk, nslot, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_power(hipStream_t st, const float* qa, const float* qb, const uint8_t* mask, long long nvox,
+                             double* partials, long long* nblk) {
+    const unsigned g = reduce_blocks(nvox);
+    *nblk = g;
+    hipLaunchKernelGGL(pair_power_kernel, dim3(g), dim3(256), 0, st, qa, qb, mask, nvox, partials);
     return hipGetLastError();
 }
 

@@ -95,9 +95,11 @@ struct Snap {
     int X, Y, Z, pad;
     long long nvox;
 };
-// One training crop: pair (input snapshot, target snapshot) and its origin.
+// One training crop: pair (input snapshot, target snapshot), its origin and
+// the weight of its scored voxels (the inverse of the pair's target mean square).
 struct CropJob {
-    int s_in, s_out, ox, oy, oz, pad;
+    int s_in, s_out, ox, oy, oz;
+    float weight;
 };
 
 // Training workspace views (batch of n crops of edge C, V = n*C^3 voxels).
@@ -133,6 +135,10 @@ hipError_t launch_pack_field(hipStream_t st, const float* q, long long cs, long 
 // partials of sum m (y - t)^2 and sum m per block; returns the block count.
 hipError_t launch_loss(hipStream_t st, const Work& w, float scale, long long* nblk);
 hipError_t launch_sum_slots(hipStream_t st, const double* partials, long long nblk, int nslot, double* out);
+// Partials (per block: sum over the voxels a step can change of |b - a|^2, their
+// count) of two snapshots of one box; returns the block count.
+hipError_t launch_pair_power(hipStream_t st, const float* qa, const float* qb, const uint8_t* mask, long long nvox,
+                             double* partials, long long* nblk);
 hipError_t launch_mask_bytes(hipStream_t st, long long nvox, const qsrk::BoxDev& box, const qsrk::RowMask* rows,
                              uint8_t* out);
 // Partials of sum over the voxels a step can change (box faces excluded) of |rhs - q|^2.

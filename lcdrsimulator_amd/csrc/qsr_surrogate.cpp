@@ -488,10 +488,11 @@ int qsr_surr_dataset_clear(void* net) {
 }
 
 int qsr_surr_train(void* net, const int* pairs, int npairs, int iters, int batch, int crop, double lr, uint64_t seed,
-                   double* loss_per_iter) {
+                   double weight_floor, double* loss_per_iter) {
     Net* n = as_net(net);
     if (int rc = check_net(n)) return rc;
-    if (!pairs || npairs < 1 || iters < 1 || batch < 1 || !(lr > 0.0) || !std::isfinite(lr))
+    if (!pairs || npairs < 1 || iters < 1 || batch < 1 || !(lr > 0.0) || !std::isfinite(lr) ||
+        !(weight_floor > 0.0) || !std::isfinite(weight_floor))
         return fail(QSR_E_ARG, "bad argument");
     if (crop < 8 || crop % 8 != 0) return fail(QSR_E_ARG, "crop must be a positive multiple of 8");
     const int ns = int(n->snaps.size());
@@ -507,7 +508,7 @@ int qsr_surr_train(void* net, const int* pairs, int npairs, int iters, int batch
     HIP_TRY(hipSetDevice(n->device));
     const long long V = (long long)batch * crop * crop * crop;
     if (int rc = ensure_work(*n, V)) return rc;
-    if (int rc = ensure_out(*n, 2LL * iters)) return rc;
+    if (int rc = ensure_out(*n, 2LL * std::max(iters, npairs))) return rc;
     if (n->d_snaps_stale) {
         discard(hipFree(n->d_snaps));
         n->d_snaps = nullptr;
@@ -528,13 +529,30 @@ int qsr_surr_train(void* net, const int* pairs, int npairs, int iters, int batch
         HIP_TRY(hipMalloc(&n->d_jobs, sizeof(CropJob) * njobs));
         n->jobs_cap = njobs;
     }
+    // Each pair's scored voxels are weighted by 1 / max(ms, weight_floor), ms
+    // the mean square of its target: pairs with small changes (late in a
+    // relaxation) count more, up to the floor.  A floor >= 1 weights all alike.
+    for (int p = 0; p < npairs; ++p) {
+        const SnapH &sa = n->snaps[pairs[2 * p]], &sb = n->snaps[pairs[2 * p + 1]];
+        long long nblk = 0;
+        HIP_TRY(launch_pair_power(n->st, sa.q, sb.q, sa.mask, sa.nvox, n->w.red, &nblk));
+        HIP_TRY(launch_sum_slots(n->st, n->w.red, nblk, 2, n->d_out + 2LL * p));
+    }
+    std::vector<double> pw(2 * size_t(npairs));
+    HIP_TRY(hipMemcpyAsync(pw.data(), n->d_out, sizeof(double) * pw.size(), hipMemcpyDeviceToHost, n->st));
+    HIP_TRY(hipStreamSynchronize(n->st));
+    std::vector<float> weight(npairs, 1.0f);
+    for (int p = 0; p < npairs; ++p) {
+        const double ms = pw[2 * p + 1] > 0 ? pw[2 * p] / (5.0 * pw[2 * p + 1]) : 0.0;
+        weight[p] = float(1.0 / std::max(ms, weight_floor));
+    }
     std::vector<CropJob> jobs(njobs);
     SplitMix r{seed};
     for (long long j = 0; j < njobs; ++j) {
         const int p = r.below(npairs);
         const SnapH& s = n->snaps[pairs[2 * p]];
         jobs[j] = CropJob{pairs[2 * p], pairs[2 * p + 1], r.below(s.X - crop + 1), r.below(s.Y - crop + 1),
-                          r.below(s.Z - crop + 1), 0};
+                          r.below(s.Z - crop + 1), weight[p]};
     }
     HIP_TRY(hipStreamSynchronize(n->st));
     HIP_TRY(hipMemcpy(n->d_jobs, jobs.data(), sizeof(CropJob) * njobs, hipMemcpyHostToDevice));

@@ -138,7 +138,7 @@ def load():
     L.qsr_surr_dataset_size.argtypes = [vp, C.POINTER(C.c_int)]
     L.qsr_surr_dataset_clear.argtypes = [vp]
     L.qsr_surr_train.argtypes = [vp, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
-                                 C.c_uint64, dp]
+                                 C.c_uint64, C.c_double, dp]
     L.qsr_surr_apply.argtypes = [vp, vp, C.c_int, C.POINTER(_SurrResult)]
     _lib = L
     return L
@@ -457,12 +457,14 @@ class Surrogate:
     def dataset_clear(self):
         _check(load().qsr_surr_dataset_clear(self._h))
 
-    def train(self, pairs, iters: int, batch: int = 4, crop: int = 32, lr: float = 1e-3, seed: int = 7) -> np.ndarray:
-        """Adam on random crops of (input, target) snapshot pairs; returns the per-iteration loss."""
+    def train(self, pairs, iters: int, batch: int = 4, crop: int = 32, lr: float = 1e-3, seed: int = 7,
+              weight_floor: float = 1.0) -> np.ndarray:
+        """Adam on random crops of (input, target) snapshot pairs, each pair weighted by
+        1 / max(mean square of its target, weight_floor); returns the per-iteration loss."""
         pr = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
         hist = np.zeros(int(iters))
         _check(load().qsr_surr_train(self._h, pr.ctypes.data_as(C.POINTER(C.c_int)), pr.shape[0], int(iters),
-                                     int(batch), int(crop), float(lr), int(seed),
+                                     int(batch), int(crop), float(lr), int(seed), float(weight_floor),
                                      hist.ctypes.data_as(C.POINTER(C.c_double))))
         return hist
 

@@ -1,0 +1,172 @@
+"""Learned warm start for the LdG droplet relaxation.
+
+No counterpart in the reference: QSR.cu relaxes with the plain loop only
+(sim_mode 1, QSR.cu:3126-3476; early stop on |dF/F| between log points,
+QSR.cu:3322-3335).  Here that loop stays both the fallback and the judge, and
+a learned step (:class:`lcdrsimulator_amd.Surrogate`) may be interleaved:
+
+* training data: trajectories of the plain solver itself, droplets of radius
+  R in [40, 120] voxels in (2R+8)^3 boxes, T in [295, 305] K, Rapini-Papoular
+  W log-uniform in [1e-5, 1e-3] J/m^2, half random-director and half radial
+  starts (S0 = 0.5), snapshots every ``gap`` steps; pairs (Q_k, Q_{k+gap});
+* training: the network's own forward / backward / Adam kernels on random
+  crops of the pairs;
+* application (:func:`relax_warm`): a checked jump (kept only if neither the
+  free energy nor the explicit-Euler residual rises) followed by ``polish``
+  plain steps, repeated until ``max_rejections`` jumps in a row are refused;
+  then plain steps with the reference's energy test until |dF/F| < tol
+  between two log points ``every`` steps apart.
+
+Both relaxations report wall time on the same context, the final energy and
+the mid-plane defect count, so the bench can state what the network gains.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from . import qsr as q
+
+ELASTIC = (6e-12, 3e-12, 1e-12)  # L1, L2, L3 (N)
+
+
+def droplet_params(R: float, T: float, W: float, N: int | None = None, elastic=ELASTIC) -> q.Params:
+    """A droplet of radius R voxels (box N^3, default 2R+8) at temperature T, weak anchoring W with
+    shell order 0.5, dx = 10 nm, gamma = 0.1, dt = the stability window's dt_max."""
+    N = N or int(2 * R + 8)
+    p = q.Params(Nx=N, Ny=N, Nz=N, dx=1e-8, dy=1e-8, dz=1e-8, T_star=307.0, T_high=T, kappa=0.0,
+                 L1=elastic[0], L2=elastic[1], L3=elastic[2], W=W, radius_index=float(R), boundary_custom=1,
+                 boundary_S=0.5, gamma=0.1)
+    w = q.dt_estimate(p.dx, p.dy, p.dz, p.gamma, p.a, p.b, p.c, p.T_star, 1, 0.0, *elastic, math.nan, 0.5, T, T)
+    p.dt = w["dt_max"]
+    return p
+
+
+def training_set(n_traj: int = 256, seed: int = 2025, T: float | None = None, W: float | None = None):
+    """(params, init mode, init seed) of the training trajectories; T and W are drawn from their
+    ranges unless fixed (the network sees only Q, not the temperature or the anchoring)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for t in range(n_traj):
+        R = float(rng.integers(40, 121))
+        Tt = float(rng.uniform(295.0, 305.0))
+        Wt = float(10.0 ** rng.uniform(-5.0, -3.0))
+        out.append((droplet_params(R, Tt if T is None else T, Wt if W is None else W), t % 2, 1000 + t))
+    return out
+
+
+def train(net: q.Surrogate, trajectories, gap: int = 500, n_snap: int = 7, group: int = 32,
+          iters_per_group: int = 1000, lr: float = 3e-4, batch: int = 8, crop: int = 32, weight_floor: float = 1.0,
+          log=None) -> dict:
+    """Generate (Q_k, Q_{k+gap}) pairs with the plain solver, ``group`` trajectories per batched
+    context, and train on each group's pairs (the dataset is cleared between groups, so device
+    memory holds one group's snapshots).  Returns timings and the loss history."""
+    t_gen = t_train = 0.0
+    hist = []
+    n_pairs = 0
+    for g0 in range(0, len(trajectories), group):
+        chunk = trajectories[g0:g0 + group]
+        ta = time.perf_counter()
+        with q.Solver([p for p, _, _ in chunk]) as s:
+            for b, (p, mode, seed) in enumerate(chunk):
+                s.upload(q.init_field(p, mode, 0.5, 0.0, seed), box=b)
+            ids = []
+            for k in range(n_snap):
+                if k:
+                    s.step(gap)
+                ids.append([net.dataset_add(s, box=b) for b in range(len(chunk))])
+        pairs = [(ids[k][b], ids[k + 1][b]) for b in range(len(chunk)) for k in range(n_snap - 1)]
+        tb = time.perf_counter()
+        h = net.train(pairs, iters=iters_per_group, batch=batch, crop=crop, lr=lr, seed=g0,
+                      weight_floor=weight_floor)
+        net.dataset_clear()
+        tc = time.perf_counter()
+        t_gen += tb - ta
+        t_train += tc - tb
+        n_pairs += len(pairs)
+        hist.append(h)
+        if log:
+            log(f"warm start training: group {g0 // group}: {len(chunk)} trajectories, {len(pairs)} pairs, "
+                f"generate {tb - ta:.1f} s, train {tc - tb:.1f} s, loss {h[:50].mean():.3e} -> {h[-50:].mean():.3e}")
+    return {"trajectories": len(trajectories), "pairs": n_pairs, "generate_s": t_gen, "train_s": t_train,
+            "loss": np.concatenate(hist) if hist else np.zeros(0)}
+
+
+def _energy(s: q.Solver) -> float:
+    return s.observables(want_defects=False, want_xi=False)["total"]
+
+
+def relax_plain(s: q.Solver, tol: float = 1e-6, every: int = 100, max_steps: int = 200000) -> dict:
+    """Plain explicit steps with the free energy checked every ``every`` steps, until |dF/F| < tol
+    between two checks (the energy part of QSR.cu:3322-3335) or max_steps."""
+    n, prev, rel, conv = 0, None, math.nan, False
+    t0 = time.perf_counter()
+    while n < max_steps:
+        s.step(every)
+        n += every
+        F = _energy(s)
+        if prev is not None:
+            rel = abs((F - prev) / prev)
+            if rel < tol:
+                conv = True
+                break
+        prev = F
+    secs = time.perf_counter() - t0
+    return {"steps": n, "seconds": secs, "converged": conv, "last_rel_dF": rel}
+
+
+def relax_warm(s: q.Solver, net: q.Surrogate, tol: float = 1e-6, every: int = 100, max_steps: int = 200000,
+               polish: int = 100, max_rejections: int = 2) -> dict:
+    """Checked learned jumps, each followed by ``polish`` plain steps, until ``max_rejections``
+    consecutive jumps are refused; then the plain loop of :func:`relax_plain` decides convergence
+    (no convergence test spans a jump).  ``steps`` counts plain steps only."""
+    n, acc, rej_total, rej, net_ms = 0, 0, 0, 0, 0.0
+    t0 = time.perf_counter()
+    while n < max_steps:
+        r = net.apply(s, q.SURR_CHECKED)
+        net_ms += r["ms_network"]
+        if r["accepted"]:
+            acc += 1
+            rej = 0
+        else:
+            rej += 1
+            rej_total += 1
+        s.step(polish)
+        n += polish
+        if rej >= max_rejections:
+            break
+    t_jumps = time.perf_counter() - t0
+    rest = relax_plain(s, tol, every, max_steps - n)
+    return {"steps": n + rest["steps"], "seconds": t_jumps + rest["seconds"], "converged": rest["converged"],
+            "last_rel_dF": rest["last_rel_dF"], "jumps_accepted": acc, "jumps_rejected": rej_total,
+            "network_ms": net_ms, "jump_phase_s": t_jumps}
+
+
+def compare(p: q.Params, net: q.Surrogate, init_mode: int, seed: int = 12345, tol: float = 1e-6, every: int = 100,
+            max_steps: int = 200000, polish: int = 100, max_rejections: int = 2, log=None) -> dict:
+    """Plain relaxation and warm-started relaxation of the same start on one context."""
+    Q0 = q.init_field(p, init_mode, 0.5, 0.0, seed)
+    out = {}
+    with q.Solver(p) as s:
+        s.upload(Q0)
+        s.autotune(5)
+        for name in ("plain", "warm"):
+            s.upload(Q0)
+            s.set_iteration(0)
+            if name == "plain":
+                r = relax_plain(s, tol, every, max_steps)
+            else:
+                r = relax_warm(s, net, tol, every, max_steps, polish, max_rejections)
+            o = s.observables()
+            r.update({"final_F": o["total"], "defects": int(o["defects"]), "plaquettes": int(o["plaquettes"])})
+            out[name] = r
+            if log:
+                log(f"warm start comparison, init mode {init_mode}, {name}: {r['steps']} steps, {r['seconds']:.3f} s, "
+                    f"F {o['total']:.9e}, defects {o['defects']}")
+    pl, wa = out["plain"], out["warm"]
+    out["speedup"] = pl["seconds"] / wa["seconds"] if wa["seconds"] > 0 else math.nan
+    out["rel_dF_final"] = abs(wa["final_F"] - pl["final_F"]) / abs(pl["final_F"])
+    out["same_defects"] = wa["defects"] == pl["defects"]
+    return out
