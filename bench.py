@@ -259,6 +259,8 @@ def run_surrogate(q, args):
     """Learned warm start (192^3, R = 88, T = 300 K, W = 1e-4, L1/L2/L3): train the network on plain-solver
     trajectories, then relax the same start plainly and warm-started; value = plain time / warm time to
     |dF/F| < 1e-6 between checks 100 steps apart (radial start; the random-director start is reported too)."""
+    import numpy as np
+
     from lcdrsimulator_amd import warmstart as ws
     trajs = ws.training_set(args.surr_traj, T=300.0 if args.surr_matched else None,
                             W=1e-4 if args.surr_matched else None)
