@@ -53,120 +53,173 @@ __device__ __forceinline__ bool changeable(const BoxDev& B, const RowMask* rows,
 
 // ---------------------------------------------------------------------------
 // Convolution (forward, and the input gradient as a forward convolution with
-// the taps flipped and the channel roles swapped).  Each wave computes S
-// segments of 16 voxels along k for all 16*NB output channels:
-//   C[co][voxel] += Wfrag[co][K] * act[K][voxel], K = (tap, ci) in steps of 32.
-// The weight fragments of the layer are staged in LDS once per block; the
-// activation operand of lane l is the 16-byte run of 8 channels of one tap at
-// voxel l%16 of the segment (zero outside the box).
+// the taps flipped and the channel roles swapped) as an implicit GEMM on the
+// matrix cores:  C[co][voxel] += Wfrag[co][K] * act[K][voxel], K = (tap, ci)
+// in steps of 32.
+//
+// A block owns 4 rows (j) x 16*S voxels (k) and streams along i over `ch`
+// planes of one batch element.  Its LDS holds the layer's weight fragments
+// and a ring of three activation planes (i-1, i, i+1) with a one-voxel halo in
+// j and k (zero outside the box), so all 27 taps of a plane are LDS reads; the
+// global loads of plane i+2 are issued before the matrix work of plane i and
+// land in the ring after it.  Wave w computes row j0+w, S segments of 16
+// voxels, all 16*NB output channels.  LDS plane layout: [row][channel group of
+// 8][k] in 16-byte units, so the 16 lanes of a group read consecutive units.
 // ---------------------------------------------------------------------------
-template <int CP, int NB, int EPI>
-__global__ __launch_bounds__(256) void conv_kernel(ConvArgs a) {
+template <int CP, int S>
+struct ConvTile {
+    static constexpr int G = CP / 8;          // 16-byte channel groups per voxel
+    static constexpr int TK = 16 * S, TJ = 4;
+    static constexpr int RW = TK + 2, RR = TJ + 2;
+    static constexpr int SLOT = RR * G * RW;  // 16-byte units per plane
+    static constexpr int UPT = (SLOT + 255) / 256;
+};
+
+template <int CP, int NB, int EPI, int S>
+__global__ __launch_bounds__(256) void conv_kernel(ConvArgs a, int ch, int ntj, int ntk, int nti) {
+    using T = ConvTile<CP, S>;
     constexpr int KS = ksteps(CP);
-    constexpr int S = 4;
-    extern __shared__ u32x4 wl[];
+    constexpr int G = T::G, RW = T::RW, SLOT = T::SLOT, UPT = T::UPT;
+    extern __shared__ u32x4 smem[];
+    u32x4* wl = smem;                    // NB * KS * 64 weight fragments
+    u32x4* ring = smem + NB * KS * 64;   // 3 plane slots
     const u32x4* wf = reinterpret_cast<const u32x4*>(a.frag);
     for (int f = threadIdx.x; f < NB * KS * 64; f += 256) wl[f] = wf[f];
+
+    int bid = blockIdx.x;
+    const int kt = bid % ntk;
+    bid /= ntk;
+    const int jt = bid % ntj;
+    bid /= ntj;
+    const int it = bid % nti;
+    const int b = bid / nti;
+    const int k0 = kt * T::TK, j0 = jt * T::TJ, i0 = it * ch, i1 = min(i0 + ch, a.X);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, col = lane & 15;
+    const int j = j0 + w;
+    const long long YZ = (long long)a.Y * a.Z;
+    const long long bbase = (long long)b * a.X * YZ;  // first voxel of batch element b
+
+    // staging: unit u = (r * RW + kk) * G + gg, consecutive threads read consecutive 16 B
+    auto load_plane = [&](int ip, u32x4 (&st)[UPT]) {
+#pragma unroll
+        for (int s = 0; s < UPT; ++s) {
+            const int u = threadIdx.x + s * 256;
+            st[s] = u32x4{0u, 0u, 0u, 0u};
+            if (u < SLOT) {
+                const int gg = u % G, kk = (u / G) % RW, r = u / (G * RW);
+                const int jj = j0 - 1 + r, kq = k0 - 1 + kk;
+                if (unsigned(ip) < unsigned(a.X) && unsigned(jj) < unsigned(a.Y) && unsigned(kq) < unsigned(a.Z))
+                    st[s] = *reinterpret_cast<const u32x4*>(
+                        a.in + (bbase + (long long)ip * YZ + (long long)jj * a.Z + kq) * CP + gg * 8);
+            }
+        }
+    };
+    auto store_plane = [&](int slot, const u32x4 (&st)[UPT]) {
+#pragma unroll
+        for (int s = 0; s < UPT; ++s) {
+            const int u = threadIdx.x + s * 256;
+            if (u < SLOT) {
+                const int gg = u % G, kk = (u / G) % RW, r = u / (G * RW);
+                ring[slot * SLOT + (r * G + gg) * RW + kk] = st[s];
+            }
+        }
+    };
+    u32x4 st[UPT];
+    load_plane(i0 - 1, st);
+    store_plane(0, st);
+    load_plane(i0, st);
+    store_plane(1, st);
+    load_plane(i0 + 1, st);
+    store_plane(2, st);
     __syncthreads();
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int g = lane >> 4, col = lane & 15;
-    const long long YZ = (long long)a.Y * a.Z;
-    const long long stride = (long long)gridDim.x * 4 * S;
-    for (long long base = ((long long)blockIdx.x * 4 + wave) * S; base < a.nseg; base += stride) {
-        long long vox[S];
-        int vi[S], vj[S], vk[S];
-        bool ok[S];
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            const long long seg = base + s;
-            const long long row = seg < a.nseg ? seg / a.segs_row : 0;
-            const int k = int(seg - row * a.segs_row) * 16 + col;
-            ok[s] = seg < a.nseg && k < a.Z;
-            vj[s] = int(row % a.Y);
-            vi[s] = int((row / a.Y) % a.X);
-            vk[s] = k;
-            vox[s] = row * a.Z + k;
-        }
+    for (int i = i0; i < i1; ++i) {
+        const bool more = i + 1 < i1;
+        if (more) load_plane(i + 2, st);  // in flight during the matrix work of plane i
+        const int sl0 = (i - i0 + 1) % 3;  // slot of plane i (plane p sits in slot (p - i0 + 1) % 3)
         f32x4 acc[S][NB];
 #pragma unroll
         for (int s = 0; s < S; ++s)
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) acc[s][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
-            // tap and first channel of this lane's 8 K values
+            // tap and channel group of this lane's 8 K values
             const int t = CP == 32 ? ks : ks * 4 + g;
-            const int c0 = CP == 32 ? g * 8 : 0;
+            const int gsel = CP == 32 ? g : 0;
             const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
-            const long long shift = di * YZ + dj * (long long)a.Z + dk;
-            u32x4 w[NB];
+            const int slot = (sl0 + 3 + di) % 3;
+            const int base = slot * SLOT + ((w + 1 + dj) * G + gsel) * RW + col + 1 + dk;
+            u32x4 wv[NB];
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) w[nb] = wl[(nb * KS + ks) * 64 + lane];
+            for (int nb = 0; nb < NB; ++nb) wv[nb] = wl[(nb * KS + ks) * 64 + lane];
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                const bool inb = ok[s] && t < TAPS && unsigned(vi[s] + di) < unsigned(a.X) &&
-                                 unsigned(vj[s] + dj) < unsigned(a.Y) && unsigned(vk[s] + dk) < unsigned(a.Z);
-                u32x4 b = {0u, 0u, 0u, 0u};
-                if (inb) b = *reinterpret_cast<const u32x4*>(a.in + (vox[s] + shift) * CP + c0);
+                u32x4 bv = {0u, 0u, 0u, 0u};
+                if (t < TAPS) bv = ring[base + s * 16];
 #pragma unroll
-                for (int nb = 0; nb < NB; ++nb) acc[s][nb] = mfma(w[nb], b, acc[s][nb]);
+                for (int nb = 0; nb < NB; ++nb) acc[s][nb] = mfma(wv[nb], bv, acc[s][nb]);
             }
         }
 
-        // lane holds output channels nb*16 + g*4 + r (r = 0..3) of voxel vox[s]
+        // lane holds output channels nb*16 + g*4 + r (r = 0..3) of voxel (i, j, k0 + s*16 + col)
+        if (j < a.Y) {
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-            if (!ok[s]) continue;
-            const long long v = vox[s];
+            for (int s = 0; s < S; ++s) {
+                const int k = k0 + s * 16 + col;
+                if (k >= a.Z) continue;
+                const long long v = bbase + (long long)i * YZ + (long long)j * a.Z + k;
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                const int co = nb * 16 + g * 4;
-                const f32x4 c = acc[s][nb];
-                if constexpr (EPI == EPI_HIDDEN) {
-                    const f32x4 bz = *reinterpret_cast<const f32x4*>(a.bias + co);
-                    const f32x4 z = c + bz;
-                    if (a.z_out) *reinterpret_cast<f32x4*>(a.z_out + v * 32 + co) = z;
-                    uint16_t h[4];
+                for (int nb = 0; nb < NB; ++nb) {
+                    const int co = nb * 16 + g * 4;
+                    const f32x4 c = acc[s][nb];
+                    if constexpr (EPI == EPI_HIDDEN) {
+                        const f32x4 bz = *reinterpret_cast<const f32x4*>(a.bias + co);
+                        const f32x4 z = c + bz;
+                        if (a.z_out) *reinterpret_cast<f32x4*>(a.z_out + v * 32 + co) = z;
+                        uint16_t h[4];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) h[r] = bf16_bits(gelu_f(z[r]));
-                    const uint2 pk = make_uint2(unsigned(h[0]) | (unsigned(h[1]) << 16),
-                                                unsigned(h[2]) | (unsigned(h[3]) << 16));
-                    *reinterpret_cast<uint2*>(a.a_out + v * 32 + co) = pk;
-                    if (a.a_cm) {
+                        for (int r = 0; r < 4; ++r) h[r] = bf16_bits(gelu_f(z[r]));
+                        const uint2 pk = make_uint2(unsigned(h[0]) | (unsigned(h[1]) << 16),
+                                                    unsigned(h[2]) | (unsigned(h[3]) << 16));
+                        *reinterpret_cast<uint2*>(a.a_out + v * 32 + co) = pk;
+                        if (a.a_cm) {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) a.a_cm[(long long)(co + r) * a.V + v] = h[r];
+                        }
+                    } else if constexpr (EPI == EPI_PRED) {
+                        if (co < 8) {
+                            const f32x4 bz = *reinterpret_cast<const f32x4*>(a.bias + co);
+                            *reinterpret_cast<f32x4*>(a.pred + v * 8 + co) = c + bz;
+                        }
+                    } else if constexpr (EPI == EPI_APPLY) {
+                        if (co < 8 && changeable(a.box, a.rows, i, j, k)) {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const int chn = co + r;
+                                if (chn < CH_Q) {
+                                    const long long idx = (long long)chn * a.cs + a.box.off + v;
+                                    a.q_new[idx] = a.q_old[idx] + (c[r] + a.bias[chn]);
+                                }
+                            }
+                        }
+                    } else {  // EPI_BWD
+                        const f32x4 z = *reinterpret_cast<const f32x4*>(a.z_in + v * 32 + co);
+                        uint16_t h[4];
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) h[r] = bf16_bits(c[r] * gelu_d(z[r]));
+                        const uint2 pk = make_uint2(unsigned(h[0]) | (unsigned(h[1]) << 16),
+                                                    unsigned(h[2]) | (unsigned(h[3]) << 16));
+                        *reinterpret_cast<uint2*>(a.a_out + v * 32 + co) = pk;
 #pragma unroll
                         for (int r = 0; r < 4; ++r) a.a_cm[(long long)(co + r) * a.V + v] = h[r];
                     }
-                } else if constexpr (EPI == EPI_PRED) {
-                    if (co < 8) {
-                        const f32x4 bz = *reinterpret_cast<const f32x4*>(a.bias + co);
-                        *reinterpret_cast<f32x4*>(a.pred + v * 8 + co) = c + bz;
-                    }
-                } else if constexpr (EPI == EPI_APPLY) {
-                    if (co < 8 && changeable(a.box, a.rows, vi[s], vj[s], vk[s])) {
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int ch = co + r;
-                            if (ch < CH_Q) {
-                                const long long idx = (long long)ch * a.cs + a.box.off + v;
-                                a.q_new[idx] = a.q_old[idx] + (c[r] + a.bias[ch]);
-                            }
-                        }
-                    }
-                } else {  // EPI_BWD
-                    const f32x4 z = *reinterpret_cast<const f32x4*>(a.z_in + v * 32 + co);
-                    uint16_t h[4];
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) h[r] = bf16_bits(c[r] * gelu_d(z[r]));
-                    const uint2 pk = make_uint2(unsigned(h[0]) | (unsigned(h[1]) << 16),
-                                                unsigned(h[2]) | (unsigned(h[3]) << 16));
-                    *reinterpret_cast<uint2*>(a.a_out + v * 32 + co) = pk;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) a.a_cm[(long long)(co + r) * a.V + v] = h[r];
                 }
             }
         }
+        __syncthreads();  // every wave is done with the slot of plane i-1
+        if (more) store_plane((i + 3 - i0) % 3, st);  // plane i+2 -> slot (i+2 - i0 + 1) % 3
+        __syncthreads();
     }
 }
 
@@ -562,14 +615,30 @@ __global__ __launch_bounds__(256) void copy_box_kernel(const float* __restrict__
 unsigned blocks_for(long long n) { return unsigned((n + 255) / 256); }
 unsigned reduce_blocks(long long n) { return std::max(1u, std::min(blocks_for(n), 2048u)); }
 
+template <int CP, int NB, int EPI, int S>
+hipError_t go_conv_s(hipStream_t st, const ConvArgs& a) {
+    using T = ConvTile<CP, S>;
+    const size_t lds = (size_t(NB) * ksteps(CP) * 64 + 3 * size_t(T::SLOT)) * 16;
+    const int ntj = (a.Y + T::TJ - 1) / T::TJ, ntk = (a.Z + T::TK - 1) / T::TK;
+    // planes per block: aim at about four blocks per CU over the whole launch
+    const long long tiles = (long long)a.n * ntj * ntk;
+    const long long want_nti = std::max(1LL, (1024 + tiles - 1) / tiles);
+    const int ch = int(std::max(2LL, (a.X + want_nti - 1) / want_nti));
+    const int nti = (a.X + ch - 1) / ch;
+    const long long grid = tiles * nti;
+    if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
+    // allow the kernel more than the default dynamic LDS; a device that cannot
+    // give it makes the launch below fail with an error
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&conv_kernel<CP, NB, EPI, S>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    hipLaunchKernelGGL((conv_kernel<CP, NB, EPI, S>), dim3(unsigned(grid)), dim3(256), lds, st, a, ch, ntj, ntk, nti);
+    return hipGetLastError();
+}
+
 template <int CP, int NB, int EPI>
 hipError_t go_conv(hipStream_t st, const ConvArgs& a) {
-    constexpr int S = 4;
-    const size_t lds = size_t(NB) * ksteps(CP) * 64 * 16;
-    const long long groups = (a.nseg + 4 * S - 1) / (4 * S);
-    const unsigned grid = unsigned(std::max(1LL, std::min(groups, 2048LL)));
-    hipLaunchKernelGGL((conv_kernel<CP, NB, EPI>), dim3(grid), dim3(256), lds, st, a);
-    return hipGetLastError();
+    // 64 voxels of k per tile, or 32 for short rows (training crops)
+    return a.Z <= 32 ? go_conv_s<CP, NB, EPI, 2>(st, a) : go_conv_s<CP, NB, EPI, 4>(st, a);
 }
 
 }  // namespace

@@ -32,7 +32,8 @@ def _batch(seed, shape, scale=0.3):
     return x, t, m
 
 
-@pytest.mark.parametrize("shape", [(1, 8, 8, 16), (2, 6, 7, 24), (1, 5, 9, 13), (1, 3, 4, 40), (3, 4, 4, 8)])
+@pytest.mark.parametrize("shape", [(1, 8, 8, 16), (2, 6, 7, 24), (1, 5, 9, 13), (1, 3, 4, 40), (3, 4, 4, 8),
+                                   (1, 5, 6, 80), (2, 3, 9, 67)])
 def test_forward_matches_oracle(qsr, shape):
     print(f"[surrogate forward] shape={shape}", flush=True)
     p = S.random_params(11, scale=2.0).astype(np.float32)
@@ -46,7 +47,7 @@ def test_forward_matches_oracle(qsr, shape):
     assert err <= FWD_TOL * scale
 
 
-@pytest.mark.parametrize("shape", [(1, 8, 8, 16), (2, 5, 6, 24), (1, 4, 4, 8), (1, 7, 3, 32)])
+@pytest.mark.parametrize("shape", [(1, 8, 8, 16), (2, 5, 6, 24), (1, 4, 4, 8), (1, 7, 3, 32), (1, 4, 5, 72)])
 def test_gradient_matches_oracle(qsr, shape):
     print(f"[surrogate gradient] shape={shape}", flush=True)
     p = S.random_params(12, scale=2.0).astype(np.float32)
