@@ -23,14 +23,20 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2/pi)
 constexpr float kGeluA = 0.044715f;
 
-__device__ __forceinline__ float gelu_f(float z) {
-    return 0.5f * z * (1.0f + tanhf(kGeluC * (z + kGeluA * z * z * z)));
+// GELU, tanh form: gelu(z) = z (1 + tanh u) / 2 with u = c (z + a z^3).  Since
+// (1 + tanh u) / 2 = 1 / (1 + e^(-2u)), it is evaluated as z / (1 + 2^(-2u log2 e))
+// with the hardware exp2 and reciprocal (e^(-2u) -> inf gives 0, -> 0 gives z).
+constexpr float kGeluK = -2.0f * kGeluC * 1.4426950408889634f;  // -2 c log2(e)
+__device__ __forceinline__ float gelu_sig(float z, float z2) {
+    const float e = __builtin_amdgcn_exp2f(kGeluK * (z * __builtin_fmaf(kGeluA, z2, 1.0f)));
+    return __builtin_amdgcn_rcpf(1.0f + e);
 }
-// d/dz of gelu_f
+__device__ __forceinline__ float gelu_f(float z) { return z * gelu_sig(z, z * z); }
+// d/dz of gelu_f: s + 2 z s (1 - s) u'(z), s = 1 / (1 + e^(-2u)), u' = c (1 + 3 a z^2)
 __device__ __forceinline__ float gelu_d(float z) {
     const float z2 = z * z;
-    const float t = tanhf(kGeluC * (z + kGeluA * z2 * z));
-    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * kGeluA * z2);
+    const float sg = gelu_sig(z, z2);
+    return __builtin_fmaf(2.0f * kGeluC * z * sg * (1.0f - sg), __builtin_fmaf(3.0f * kGeluA, z2, 1.0f), sg);
 }
 
 // Round to nearest even (the hardware conversion), as raw bits.
