@@ -49,6 +49,14 @@ __device__ __forceinline__ f32x4 mfma(u32x4 a, u32x4 b, f32x4 c) {
                                                    0, 0);
 }
 
+// Workgroup barrier ordering LDS only: a full __syncthreads() also waits for
+// every outstanding global access, including the output stores in flight.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ bool changeable(const BoxDev& B, const RowMask* rows, int i, int j, int k) {
     if (B.flags & qsrk::F_PERIODIC) return true;
     const RowMask m = rows[B.rows_off + (long long)i * B.Ny + j];
@@ -168,6 +176,8 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs a, int ch, int ntj, 
             }
         }
 
+        lds_barrier();  // every wave is done with the slot of plane i-1
+        if (more) store_plane((i + 3 - i0) % 3, st);  // plane i+2 -> slot (i+2 - i0 + 1) % 3
         // lane holds output channels nb*16 + g*4 + r (r = 0..3) of voxel (i, j, k0 + s*16 + col)
         if (j < a.Y) {
 #pragma unroll
@@ -223,9 +233,7 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs a, int ch, int ntj, 
                 }
             }
         }
-        __syncthreads();  // every wave is done with the slot of plane i-1
-        if (more) store_plane((i + 3 - i0) % 3, st);  // plane i+2 -> slot (i+2 - i0 + 1) % 3
-        __syncthreads();
+        lds_barrier();  // plane i+2 is in place before the next plane's reads
     }
 }
 
