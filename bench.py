@@ -255,6 +255,36 @@ def run_ladder(q, args):
     return 0
 
 
+def run_network_pass(q, args):
+    """The learned step's network pass alone (pack + three convolutions + apply) on the 192^3, R = 88
+    droplet, untrained weights: device time per pass from hipEvents on the context stream."""
+    import numpy as np
+
+    from lcdrsimulator_amd import warmstart as ws
+    p = ws.droplet_params(88.0, 300.0, 1e-4, N=192)
+    with q.Solver(p) as s, q.Surrogate(seed=3) as net:
+        s.upload(q.init_field(p, 1, 0.5, 0.0, 12345))
+        for _ in range(max(1, args.warmup)):
+            net.apply(s, q.SURR_PROPOSE)
+        ms = [net.apply(s, q.SURR_PROPOSE)["ms_network"] for _ in range(args.steps)]
+    net_ms = float(np.median(ms))
+    achieved = NET_FLOP_PER_VOXEL * p.n / (net_ms * 1e-3) / 1e12
+    log(f"network pass at 192^3: median {net_ms:.4f} ms over {args.steps} passes")
+    out = {
+        "metric": METRIC, "value": p.n / (net_ms * 1e-3), "unit": "voxels/s through the learned step's network",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": net_ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)",
+        "data": "synthetic (seeded radial droplet, untrained weights)",
+        "config": {"workload": "network pass of the learned step: 5-32-32-5 channels, 3x3x3, 192^3 box",
+                   "flop_per_voxel": NET_FLOP_PER_VOXEL, "voxels": p.n},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / MFMA_BF16_PEAK_TFLOPS, "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def run_surrogate(q, args):
     """Learned warm start (192^3, R = 88, T = 300 K, W = 1e-4, L1/L2/L3): train the network on plain-solver
     trajectories, then relax the same start plainly and warm-started; value = plain time / warm time to
@@ -262,6 +292,8 @@ def run_surrogate(q, args):
     import numpy as np
 
     from lcdrsimulator_amd import warmstart as ws
+    if args.surr_net_only:
+        return run_network_pass(q, args)
     trajs = ws.training_set(args.surr_traj, T=300.0 if args.surr_matched else None,
                             W=1e-4 if args.surr_matched else None)
     with q.Surrogate(seed=3) as net:
@@ -323,6 +355,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     ap.add_argument("--surr-traj", type=int, default=256, help="surrogate_192: training trajectories")
+    ap.add_argument("--surr-net-only", action="store_true",
+                    help="surrogate_192: time only the network pass (--steps passes after --warmup)")
     ap.add_argument("--surr-matched", action="store_true",
                     help="surrogate_192: train at the application's T and W instead of their ranges")
     ap.add_argument("--surr-iters", type=int, default=1000,
