@@ -36,7 +36,8 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-WORKLOADS = ("droplet_l1l2l3_256", "periodic_noise_256", "droplet_l1_64", "ladder_64", "surrogate_192")
+WORKLOADS = ("droplet_l1l2l3_256", "periodic_noise_256", "droplet_l1_64", "droplet_l1_256", "ladder_64",
+             "surrogate_192")
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 NET_FLOP_PER_VOXEL = 72576.0  # 2 * 27 * (5*32 + 32*32 + 32*5) multiply-adds of the 5-32-32-5 network
 
@@ -72,9 +73,12 @@ def workload_params(q, n=256, R=120.0, workload="droplet_l1l2l3_256"):
                           310.0, 295.0)
         p.dt = w["dt_max"]
         return p
-    if workload == "droplet_l1_64":
-        p = q.Params(Nx=64, Ny=64, Nz=64, dx=1e-8, dy=1e-8, dz=1e-8, T_star=307.0, T_high=300.0, kappa=0.0,
-                     L1=6e-12, gamma=0.1, radius_index=28.0)
+    if workload in ("droplet_l1_64", "droplet_l1_256"):
+        # the config-0 physics (L1 = 6e-12, strong equilibrium anchoring) at 64^3, R = 28 or 256^3, R = 120
+        big = workload == "droplet_l1_256"
+        m = 256 if big else 64
+        p = q.Params(Nx=m, Ny=m, Nz=m, dx=1e-8, dy=1e-8, dz=1e-8, T_star=307.0, T_high=300.0, kappa=0.0,
+                     L1=6e-12, gamma=0.1, radius_index=120.0 if big else 28.0)
         w = q.dt_estimate(p.dx, p.dy, p.dz, p.gamma, p.a, p.b, p.c, p.T_star, 1, 0.0, p.L1, 0, 0, math.nan, 0.5,
                           300.0, 300.0)
         p.dt = w["dt_max"]
@@ -153,8 +157,9 @@ def describe(p, workload):
     if workload == "periodic_noise_256":
         return (f"periodic bulk LdG quench {p.Nx}^3, kappa=6e-12, T ramp 310->295 K, Philox noise sigma="
                 f"{p.noise_sigma:.4e}, dt={p.dt:.5e}")
-    if workload == "droplet_l1_64":
-        return f"LdG droplet relaxation {p.Nx}^3, R=28, L1=6e-12, strong anchoring, dt={p.dt:.5e}"
+    if workload in ("droplet_l1_64", "droplet_l1_256"):
+        return (f"LdG droplet relaxation {p.Nx}^3, R={p.radius_index:g}, L1=6e-12, strong anchoring, "
+                f"dt={p.dt:.5e}")
     return (f"LdG droplet relaxation {p.Nx}^3, R={p.radius_index:g}, L1/L2/L3=6e-12/3e-12/1e-12, "
             f"weak anchoring W=1e-4 custom S=0.5, explicit Euler dt={p.dt:.5e}")
 
