@@ -71,7 +71,7 @@ __device__ __forceinline__ bool changeable(const BoxDev& B, const RowMask* rows,
 // matrix cores:  C[co][voxel] += Wfrag[co][K] * act[K][voxel], K = (tap, ci)
 // in steps of 32.
 //
-// A block owns 4 rows (j) x 16*S voxels (k) and streams along i over `ch`
+// A block owns TJ rows (j, one wave each) x 16*S voxels (k) and streams along i over `ch`
 // planes of one batch element.  Its LDS holds the layer's weight fragments
 // and a ring of three activation planes (i-1, i, i+1) with a one-voxel halo in
 // j and k (zero outside the box), so all 27 taps of a plane are LDS reads; the
@@ -80,25 +80,25 @@ __device__ __forceinline__ bool changeable(const BoxDev& B, const RowMask* rows,
 // voxels, all 16*NB output channels.  LDS plane layout: [row][channel group of
 // 8][k] in 16-byte units, so the 16 lanes of a group read consecutive units.
 // ---------------------------------------------------------------------------
-template <int CP, int S>
+template <int CP, int S, int TJ_>
 struct ConvTile {
     static constexpr int G = CP / 8;          // 16-byte channel groups per voxel
-    static constexpr int TK = 16 * S, TJ = 4;
+    static constexpr int TK = 16 * S, TJ = TJ_, NT = 64 * TJ_;
     static constexpr int RW = TK + 2, RR = TJ + 2;
     static constexpr int SLOT = RR * G * RW;  // 16-byte units per plane
-    static constexpr int UPT = (SLOT + 255) / 256;
+    static constexpr int UPT = (SLOT + NT - 1) / NT;
 };
 
-template <int CP, int NB, int EPI, int S>
-__global__ __launch_bounds__(256) void conv_kernel(ConvArgs a, int ch, int ntj, int ntk, int nti) {
-    using T = ConvTile<CP, S>;
+template <int CP, int NB, int EPI, int S, int TJ>
+__global__ __launch_bounds__(64 * TJ) void conv_kernel(ConvArgs a, int ch, int ntj, int ntk, int nti) {
+    using T = ConvTile<CP, S, TJ>;
     constexpr int KS = ksteps(CP);
-    constexpr int G = T::G, RW = T::RW, SLOT = T::SLOT, UPT = T::UPT;
+    constexpr int G = T::G, RW = T::RW, SLOT = T::SLOT, UPT = T::UPT, NT = T::NT;
     extern __shared__ u32x4 smem[];
     u32x4* wl = smem;                    // NB * KS * 64 weight fragments
     u32x4* ring = smem + NB * KS * 64;   // 3 plane slots
     const u32x4* wf = reinterpret_cast<const u32x4*>(a.frag);
-    for (int f = threadIdx.x; f < NB * KS * 64; f += 256) wl[f] = wf[f];
+    for (int f = threadIdx.x; f < NB * KS * 64; f += NT) wl[f] = wf[f];
 
     int bid = blockIdx.x;
     const int kt = bid % ntk;
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs a, int ch, int ntj, 
     auto load_plane = [&](int ip, u32x4 (&st)[UPT]) {
 #pragma unroll
         for (int s = 0; s < UPT; ++s) {
-            const int u = threadIdx.x + s * 256;
+            const int u = threadIdx.x + s * NT;
             st[s] = u32x4{0u, 0u, 0u, 0u};
             if (u < SLOT) {
                 const int gg = u % G, kk = (u / G) % RW, r = u / (G * RW);
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs a, int ch, int ntj, 
     auto store_plane = [&](int slot, const u32x4 (&st)[UPT]) {
 #pragma unroll
         for (int s = 0; s < UPT; ++s) {
-            const int u = threadIdx.x + s * 256;
+            const int u = threadIdx.x + s * NT;
             if (u < SLOT) {
                 const int gg = u % G, kk = (u / G) % RW, r = u / (G * RW);
                 ring[slot * SLOT + (r * G + gg) * RW + kk] = st[s];
@@ -629,9 +629,9 @@ __global__ __launch_bounds__(256) void copy_box_kernel(const float* __restrict__
 unsigned blocks_for(long long n) { return unsigned((n + 255) / 256); }
 unsigned reduce_blocks(long long n) { return std::max(1u, std::min(blocks_for(n), 2048u)); }
 
-template <int CP, int NB, int EPI, int S>
+template <int CP, int NB, int EPI, int S, int TJ>
 hipError_t go_conv_s(hipStream_t st, const ConvArgs& a) {
-    using T = ConvTile<CP, S>;
+    using T = ConvTile<CP, S, TJ>;
     const size_t lds = (size_t(NB) * ksteps(CP) * 64 + 3 * size_t(T::SLOT)) * 16;
     const int ntj = (a.Y + T::TJ - 1) / T::TJ, ntk = (a.Z + T::TK - 1) / T::TK;
     // planes per block: aim at about four blocks per CU over the whole launch
@@ -643,16 +643,18 @@ hipError_t go_conv_s(hipStream_t st, const ConvArgs& a) {
     if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
     // allow the kernel more than the default dynamic LDS; a device that cannot
     // give it makes the launch below fail with an error
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&conv_kernel<CP, NB, EPI, S>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&conv_kernel<CP, NB, EPI, S, TJ>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-    hipLaunchKernelGGL((conv_kernel<CP, NB, EPI, S>), dim3(unsigned(grid)), dim3(256), lds, st, a, ch, ntj, ntk, nti);
+    hipLaunchKernelGGL((conv_kernel<CP, NB, EPI, S, TJ>), dim3(unsigned(grid)), dim3(T::NT), lds, st, a, ch, ntj,
+                       ntk, nti);
     return hipGetLastError();
 }
 
 template <int CP, int NB, int EPI>
 hipError_t go_conv(hipStream_t st, const ConvArgs& a) {
-    // 64 voxels of k per tile, or 32 for short rows (training crops)
-    return a.Z <= 32 ? go_conv_s<CP, NB, EPI, 2>(st, a) : go_conv_s<CP, NB, EPI, 4>(st, a);
+    // 32 voxels of k per block and one wave per row: 12 rows (768 threads, three waves per SIMD,
+    // 146 KB of LDS for 32 -> 32) on large boxes, 8 rows on short ones (training crops)
+    return a.Y > 32 ? go_conv_s<CP, NB, EPI, 2, 12>(st, a) : go_conv_s<CP, NB, EPI, 2, 8>(st, a);
 }
 
 }  // namespace

@@ -33,7 +33,7 @@ def _batch(seed, shape, scale=0.3):
 
 
 @pytest.mark.parametrize("shape", [(1, 8, 8, 16), (2, 6, 7, 24), (1, 5, 9, 13), (1, 3, 4, 40), (3, 4, 4, 8),
-                                   (1, 5, 6, 80), (2, 3, 9, 67)])
+                                   (1, 5, 6, 80), (2, 3, 9, 67), (1, 4, 40, 24), (1, 3, 37, 48)])
 def test_forward_matches_oracle(qsr, shape):
     print(f"[surrogate forward] shape={shape}", flush=True)
     p = S.random_params(11, scale=2.0).astype(np.float32)
