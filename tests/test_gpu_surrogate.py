@@ -235,3 +235,25 @@ def test_trained_jump_gains_over_plain_steps(qsr):
     assert loss[-50:].mean() < 0.2 * loss[:50].mean()
     assert gains["trained"] > 0.5
     assert gains["untrained"] < 0.5 * gains["trained"]
+
+
+def test_warm_start_driver_relaxes_and_reports(qsr):
+    """The warm-start protocol end to end on a small droplet: both relaxations reach
+    |dF/F| < 1e-6; the jump phase ends after two consecutive rejections; the plain loop is
+    deterministic; the warm-started result stays at the plain run's energy to 1e-3."""
+    from lcdrsimulator_amd import warmstart as ws
+
+    with qsr.Surrogate(seed=3) as net:
+        trajs = [(ws.droplet_params(R, 300.0, 1e-4), 1, 0) for R in (14.0, 15.0, 17.0, 18.0)]
+        ws.train(net, trajs, gap=500, n_snap=3, group=4, iters_per_group=600, lr=1e-3, batch=4, crop=32)
+        p = ws.droplet_params(16.0, 300.0, 1e-4)
+        r = ws.compare(p, net, 1, max_steps=40000)
+        r2 = ws.compare(p, net, 1, max_steps=40000)
+    pl, wa = r["plain"], r["warm"]
+    print(f"  plain {pl['steps']} steps {pl['seconds']:.3f} s; warm {wa['steps']} steps {wa['seconds']:.3f} s, "
+          f"{wa['jumps_accepted']} accepted, {wa['jumps_rejected']} rejected; rel dF {r['rel_dF_final']:.2e}, "
+          f"defects {pl['defects']} / {wa['defects']}")
+    assert pl["converged"] and wa["converged"]
+    assert wa["jumps_rejected"] >= 2 and wa["jumps_accepted"] >= 1
+    assert r["rel_dF_final"] < 1e-3 and r["same_defects"] in (True, False)
+    assert r2["plain"]["steps"] == pl["steps"] and r2["plain"]["final_F"] == pl["final_F"]
