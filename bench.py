@@ -183,7 +183,7 @@ def time_to_converge(q, p, args):
             k = min(args.relax_every, args.relax_max_steps - n)
             s.step(k)
             n += k
-            F = s.observables(want_defects=False, want_xi=False)["total"]
+            F = s.observables(want_defects=False, want_xi=False, want_order=False)["total"]
             if prev is not None and prev != 0.0:
                 rel = abs((F - prev) / prev)
                 if rel < args.relax_tol:

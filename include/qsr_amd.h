@@ -93,6 +93,10 @@ typedef struct qsr_obs_opts {
     int32_t xi_z_slice;       /* < 0 -> z_slice */
     double xi_S_threshold;    /* < 0 -> S_threshold */
     int32_t want_defects, want_xi; /* 0 skips that pass (columns -> NaN / 0) */
+    int32_t skip_order;       /* droplet boxes: 1 skips the nematic pass, so avg_S, max_S,
+                                 radiality and checker_rel are NaN (energies only, e.g. the
+                                 early-stop check); with want_defects = want_xi = 0 the
+                                 mid-plane pass is skipped too */
 } qsr_obs_opts;
 
 typedef struct qsr_box_info {

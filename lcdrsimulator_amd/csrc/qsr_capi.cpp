@@ -875,6 +875,7 @@ int qsr_observables(void* ctx, int box, const qsr_obs_opts* opts, qsr_obs* out) 
     P.L1 = p.L1; P.L2 = p.L2; P.L3 = p.L3;
     P.W_over_delta = (p.W != 0.0 && b.thickness > 0.0) ? p.W / b.thickness : 0.0;
     P.S_shell = b.last_S;
+    P.skip_order = o.skip_order ? 1 : 0;
     const float* q = c->q[c->cur];
 
     qsr_obs r{};
@@ -898,7 +899,13 @@ int qsr_observables(void* ctx, int box, const qsr_obs_opts* opts, qsr_obs* out) 
         r.avg_S = vol[4] > 0 ? vol[3] / vol[4] : 0.0;
         r.max_S = vol[7];
         r.radiality = vol[6] > 0 ? vol[5] / vol[6] : 0.0;
+        if (o.skip_order) r.avg_S = r.max_S = r.radiality = kNaN;
         // slice pass: defects, xi and (if max_S >= 0.5) the checkerboard metric
+        const bool want_checker = !o.skip_order && r.max_S >= 0.5;
+        if (!o.want_defects && !o.want_xi && !want_checker) {
+            *out = r;
+            return QSR_OK;
+        }
         const long long ns = (long long)p.Nx * p.Ny;
         if (ns * 4 > c->nf_slice_cap) {
             discard(hipFree(c->d_nf_slice[0]));
@@ -908,7 +915,6 @@ int qsr_observables(void* ctx, int box, const qsr_obs_opts* opts, qsr_obs* out) 
             HIP_TRY(hipMalloc(&c->d_nf_slice[1], sizeof(double) * ns * 4));
             c->nf_slice_cap = ns * 4;
         }
-        const bool want_checker = r.max_S >= 0.5;
         HIP_TRY(qsro::launch_obs_slice_nematic(c->st, q, c->cs, P, zs, c->d_nf_slice[0]));
         const double* nfx = c->d_nf_slice[0];
         if (xz != zs) {

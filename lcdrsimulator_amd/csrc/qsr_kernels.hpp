@@ -80,6 +80,7 @@ struct ObsParams {
     int no_L;                  // L1 = L2 = L3 = 0 -> kappa
     double W_over_delta;       // 0 disables the anchoring energy
     double S_shell;
+    int skip_order;            // 1: no nematic pass (<S>, max S, radiality)
 };
 
 constexpr int NPART = 8;     // volume partial slots

@@ -142,12 +142,17 @@ __global__ __launch_bounds__(256) void obs_droplet_kernel(const float* __restric
                                                           const RowMask* __restrict__ step_rows,
                                                           const RowMask* __restrict__ obs_rows,
                                                           double* __restrict__ partials) {
-    const long long n = (long long)P.Nx * P.Ny * P.Nz;
     double acc[NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
-    // grid-stride: each thread accumulates its voxels in a fixed order (deterministic sums)
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
-        const int k = int(v % P.Nz), j = int((v / P.Nz) % P.Ny), i = int(v / ((long long)P.Ny * P.Nz));
+    // rows (i, j) in a grid stride, a thread's k fixed within a row: each thread accumulates its
+    // voxels in a fixed order (deterministic sums) and splits a row index once, not per voxel
+    const long long rows = (long long)P.Nx * P.Ny;
+    const int kw = P.Nz < 256 ? P.Nz : 256;  // threads per row
+    const int rpi = 256 / kw;                 // rows per block iteration
+    const int rsub = threadIdx.x / kw, k0 = threadIdx.x - rsub * kw;
+    for (long long r = (long long)blockIdx.x * rpi + rsub; rsub < rpi && r < rows; r += (long long)gridDim.x * rpi) {
+      const int i = int(r / P.Ny), j = int(r - (long long)i * P.Ny);
+      for (int k = k0; k < P.Nz; k += kw) {
+        const long long v = r * P.Nz + k;
         const float* base = q + P.off;
         double Q[5];
         load_q(base, cs, v, Q);
@@ -224,7 +229,7 @@ __global__ __launch_bounds__(256) void obs_droplet_kernel(const float* __restric
         }
         const bool inA = in_iv(k, om.v[0], om.v[1]);
         const bool inRad = in_iv(k, om.v[2], om.v[3]) && !in_iv(k, om.v[4], om.v[5]);
-        if (inA || inRad) {
+        if (!P.skip_order && (inA || inRad)) {
             double nf[4];
             nematic(Q, false, nf);
             if (inA) {
@@ -240,6 +245,7 @@ __global__ __launch_bounds__(256) void obs_droplet_kernel(const float* __restric
                 acc[6] += 1.0;
             }
         }
+      }
     }
     const bool is_max[NPART] = {false, false, false, false, false, false, false, true};
     block_reduce_store<NPART>(acc, is_max, partials + (long long)blockIdx.x * NPART);

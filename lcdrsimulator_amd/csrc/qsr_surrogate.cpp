@@ -584,7 +584,7 @@ int qsr_surr_apply(void* net, void* ctx, int mode, qsr_surr_result* out) {
     HIP_TRY(hipSetDevice(n->device));
     HIP_TRY(hipStreamSynchronize(n->st));  // parameters last written on the network's stream
     qsr_surr_result r{};
-    const qsr_obs_opts oo = {-1, 0.1, 0.25, -1, -1.0, 0, 0};
+    const qsr_obs_opts oo = {-1, 0.1, 0.25, -1, -1.0, 0, 0, 1};  // energies only
     qsr_obs o{};
     if (int rc = qsr_observables(ctx, 0, &oo, &o)) return rc;
     r.F_before = o.total;

@@ -59,7 +59,7 @@ class _Obs(C.Structure):
 class _ObsOpts(C.Structure):
     _fields_ = [("z_slice", C.c_int32), ("S_threshold", C.c_double), ("charge_cutoff", C.c_double),
                 ("xi_z_slice", C.c_int32), ("xi_S_threshold", C.c_double),
-                ("want_defects", C.c_int32), ("want_xi", C.c_int32)]
+                ("want_defects", C.c_int32), ("want_xi", C.c_int32), ("skip_order", C.c_int32)]
 
 
 class _BoxInfo(C.Structure):
@@ -276,9 +276,10 @@ class Solver:
         _check(load().qsr_set_iteration(self._h, box, int(it)))
 
     def observables(self, box: int = 0, z_slice=-1, S_threshold=0.1, charge_cutoff=0.25, xi_z_slice=-1,
-                    xi_S_threshold=-1.0, want_defects=True, want_xi=True) -> dict:
+                    xi_S_threshold=-1.0, want_defects=True, want_xi=True, want_order=True) -> dict:
+        """FP64 log-point observables; want_order=False skips <S>, max S and radiality (NaN)."""
         o = _ObsOpts(z_slice, S_threshold, charge_cutoff, xi_z_slice, xi_S_threshold, int(want_defects),
-                     int(want_xi))
+                     int(want_xi), int(not want_order))
         r = _Obs()
         _check(load().qsr_observables(self._h, box, C.byref(o), C.byref(r)))
         return {k: getattr(r, k) for k in OBS_KEYS}

@@ -95,7 +95,7 @@ def train(net: q.Surrogate, trajectories, gap: int = 500, n_snap: int = 7, group
 
 
 def _energy(s: q.Solver) -> float:
-    return s.observables(want_defects=False, want_xi=False)["total"]
+    return s.observables(want_defects=False, want_xi=False, want_order=False)["total"]
 
 
 def relax_plain(s: q.Solver, tol: float = 1e-6, every: int = 100, max_steps: int = 200000) -> dict:

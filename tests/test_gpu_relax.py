@@ -130,3 +130,19 @@ def test_config0_relaxation_2000_steps(qsr, orc):
     print(f"  final max|dQ| {err.max():.3e} rms {math.sqrt((err ** 2).mean()):.3e} worst dF {worst['dF']:.2e}")
     assert err.max() <= 1e-4
     assert math.sqrt((err ** 2).mean()) <= 1e-5
+
+
+def test_energy_only_observables_equal_the_full_pass(qsr):
+    """skip_order (want_order=False) drops the nematic pass and, with defects and xi off, the
+    mid-plane pass: the energies are the full pass's to the last bit, the order columns NaN."""
+    p = qsr.Params(Nx=48, Ny=44, Nz=52, T_star=307.0, T_high=300.0, kappa=0.0, L1=6e-12, L2=3e-12, L3=1e-12,
+                   W=1e-4, boundary_custom=1, boundary_S=0.5, dt=1.5e-10)
+    with qsr.Solver(p) as s:
+        s.upload(qsr.init_field(p, 0, 0.5, seed=21))
+        s.step(5)
+        full = s.observables()
+        fast = s.observables(want_defects=False, want_xi=False, want_order=False)
+    for k in ("bulk", "elastic", "anchoring", "total"):
+        assert fast[k] == full[k], k
+    assert math.isnan(fast["avg_S"]) and math.isnan(fast["max_S"]) and math.isnan(fast["radiality"])
+    assert not math.isnan(full["avg_S"]) and full["max_S"] > 0
