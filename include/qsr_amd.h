@@ -198,7 +198,12 @@ int qsr_init_field(const qsr_params* p, int mode, double S0, double noise_amplit
  * the plain relaxation loop (QSR.cu:3126-3476), which stays the fallback and
  * the judge of every state.  A three-layer 3x3x3 convolution network (5 -> 32
  * -> 32 -> 5 channels, GELU, 36,357 parameters) predicts dQ = Q(k+N) - Q(k)
- * from Q(k); it runs on BF16 matrix cores with FP32 accumulation, and is
+ * from Q(k) (field input), or modulates the increment u(k) = E(Q(k)) - Q(k)
+ * of one explicit Euler step E: dQ = N u (1 + y) per voxel and component, the
+ * network reading x = u / s (increment input; s the RMS of u per component over
+ * the voxels a step can change, box faces excluded) or x = Q(k) (gain input).
+ * A zero network then is the plain step extrapolated N times.  It runs on BF16
+ * matrix cores with FP32 accumulation, and is
  * trained with its own backward and Adam kernels on pairs of fields the plain
  * solver produced.  Parameters are one flat FP32 array:
  * W1[27][5][32] b1[32] W2[27][32][32] b2[32] W3[27][32][5] b3[5], tap
@@ -210,11 +215,18 @@ enum {
     QSR_SURR_FORCE = 1,   /* keep the step */
     QSR_SURR_PROPOSE = 2  /* measure, then restore the previous field */
 };
+enum {
+    QSR_SURR_INPUT_FIELD = 0,     /* x = Q(k), dQ = y (the default) */
+    QSR_SURR_INPUT_INCREMENT = 1, /* x = u(k) / s, dQ = N u (1 + y) */
+    QSR_SURR_INPUT_GAIN = 2       /* x = Q(k), dQ = N u (1 + y) */
+};
 typedef struct qsr_surr_result {
     double F_before, F_after;     /* total free energy (J) before / with the step */
     double res_before, res_after; /* |Q' - Q|_2 of one explicit Euler step over the voxels a step can change */
     int32_t accepted;             /* 1 if the context now holds the stepped field */
     double ms_network;            /* device time of the network pass */
+    double scale;                 /* the scale of the step kept (or of the last one tried) */
+    int32_t tried;                /* proposals evaluated (0: nothing to propose, zero increment) */
 } qsr_surr_result;
 
 int qsr_surr_num_params(size_t* n);
@@ -230,19 +242,33 @@ int qsr_surr_load(void* net, const char* path);
 /* One pass over a host batch: x5, target5 (may be NULL = 0) and pred5 are
  * (nbatch, X, Y, Z, 5) FP32, mask (may be NULL = 1) is (nbatch, X, Y, Z).
  * loss = sum mask * |pred - target|^2 / (5 nbatch X Y Z); grad receives its
- * gradient (flat layout).  loss / grad need Z % 8 == 0.  Outputs may be NULL. */
+ * gradient (flat layout), pred = y.  loss / grad need Z % 8 == 0.  Outputs
+ * may be NULL. */
 int qsr_surr_eval(void* net, int nbatch, int X, int Y, int Z, const float* x5, const float* target5,
                   const float* mask, double* loss, float* pred5, float* grad);
+/* The same with the modulated prediction of increment and gain input:
+ * pred = m5 (1 + y), m5 (nbatch, X, Y, Z, 5) FP32 (the normalised increment u / s). */
+int qsr_surr_eval_gain(void* net, int nbatch, int X, int Y, int Z, const float* x5, const float* m5,
+                       const float* target5, const float* mask, double* loss, float* pred5, float* grad);
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected) with a given gradient. */
 int qsr_surr_adam_step(void* net, const float* grad, double lr);
 /* Training data: snapshot the current field of a context's box (and which of
  * its voxels a step can change) into device memory; returns its index. */
 int qsr_surr_dataset_add(void* net, void* ctx, int box, int* snapshot);
+/* The same, also recording the increment of one explicit Euler step from the
+ * field (needed by the input snapshot of a pair under increment input). */
+int qsr_surr_dataset_add_increment(void* net, void* ctx, int box, int* snapshot);
+/* What the network reads and, for increment input, the horizon N its output is
+ * scaled by (N > 0).  Training and application use the setting of the call. */
+int qsr_surr_set_input(void* net, int kind, double horizon);
 int qsr_surr_dataset_size(void* net, int* n);
 int qsr_surr_dataset_clear(void* net);
 /* iters Adam steps on batches of `batch` random crop^3 windows (crop % 8 == 0)
  * of the pairs (input snapshot, target snapshot) pairs[2p], pairs[2p+1]; the
- * target is their difference.  A crop face inside its box leaves its 3-voxel
+ * target is their difference (field input), or the target is the difference
+ * / (N s) and the prediction (u / s)(1 + y), u the input snapshot's recorded
+ * increment, with the network reading u / s (increment input) or the input
+ * snapshot's field (gain input).  A crop face inside its box leaves its 3-voxel
  * band unscored.  Each pair's voxels are weighted by 1 / max(ms, weight_floor),
  * ms the mean square of its target over the voxels a step can change, so the
  * small changes late in a relaxation count more (weight_floor >= 1 weights
@@ -253,6 +279,10 @@ int qsr_surr_train(void* net, const int* pairs, int npairs, int iters, int batch
 /* Propose Q + dQ for every voxel a step can change in a single-box context;
  * mode QSR_SURR_*.  The iteration counter is not advanced. */
 int qsr_surr_apply(void* net, void* ctx, int mode, qsr_surr_result* out);
+/* The same with the proposals Q + scales[i] dQ, i = 0 .. nscales-1 (scales > 0):
+ * QSR_SURR_CHECKED keeps the first that passes the check (one network pass;
+ * later proposals rescale the first), FORCE and PROPOSE use scales[0]. */
+int qsr_surr_apply_ladder(void* net, void* ctx, int mode, const double* scales, int nscales, qsr_surr_result* out);
 
 #ifdef __cplusplus
 }

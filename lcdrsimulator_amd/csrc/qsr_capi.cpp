@@ -1051,6 +1051,7 @@ int field_view(void* ctx, int box, FieldView* out) {
     v.st = c->st;
     v.cur = c->q[c->cur];
     v.other = c->q[1 - c->cur];
+    v.rhs = c->q_rhs;
     v.cs = c->cs;
     v.nvox = c->boxes[box].nvox;
     v.box = c->h_boxes[box];
@@ -1069,7 +1070,7 @@ int commit_other(void* ctx) {
     return QSR_OK;
 }
 
-int euler_residual(void* ctx, int box, double* res) {
+int euler_residual(void* ctx, int box, double* res, long long* count) {
     Ctx* c = as_ctx(ctx);
     if (!c || !res || box < 0 || box >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad context or box");
     if (int rc = check_uploaded(*c)) return rc;
@@ -1090,11 +1091,12 @@ int euler_residual(void* ctx, int box, double* res) {
     long long nblk = 0;
     HIP_TRY(qsrs::launch_residual(c->st, c->q[c->cur], c->q_rhs, c->cs, c->h_boxes[box], c->d_step_rows, c->d_part,
                                   &nblk));
-    HIP_TRY(qsrs::launch_sum_slots(c->st, c->d_part, nblk, 1, c->d_out));
-    double r = 0.0;
-    HIP_TRY(hipMemcpyAsync(&r, c->d_out, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(qsrs::launch_sum_slots(c->st, c->d_part, nblk, 2, c->d_out));
+    double r[2] = {0.0, 0.0};
+    HIP_TRY(hipMemcpyAsync(r, c->d_out, sizeof(r), hipMemcpyDeviceToHost, c->st));
     HIP_TRY(hipStreamSynchronize(c->st));
-    *res = std::sqrt(r);
+    *res = std::sqrt(r[0]);
+    if (count) *count = (long long)r[1];
     return QSR_OK;
 }
 

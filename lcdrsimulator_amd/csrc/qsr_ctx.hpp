@@ -16,6 +16,7 @@ struct FieldView {
     hipStream_t st = nullptr;
     const float* cur = nullptr;  // the current field (five SoA planes of cs floats)
     float* other = nullptr;      // the other ping-pong buffer
+    const float* rhs = nullptr;  // one explicit step of the current field, after euler_residual (else null)
     long long cs = 0, nvox = 0;
     qsrk::BoxDev box{};
     const qsrk::RowMask* rows = nullptr;  // step row masks (packed over all boxes)
@@ -32,7 +33,9 @@ int commit_other(void* ctx);
 // sqrt of the sum over the box's changeable interior voxels of |Q' - Q|^2,
 // Q' = one explicit Euler step without noise of the current field (the
 // semi-implicit right-hand side with no stabiliser); the field is unchanged.
-int euler_residual(void* ctx, int box, double* res);
+// Q' stays in FieldView::rhs until the next step; count (may be null)
+// receives the number of voxels summed over.
+int euler_residual(void* ctx, int box, double* res, long long* count = nullptr);
 // Sets the calling thread's qsr_last_error() message and returns code.
 int set_error(int code, const std::string& msg);
 

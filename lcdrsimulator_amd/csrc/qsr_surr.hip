@@ -65,6 +65,12 @@ __device__ __forceinline__ bool changeable(const BoxDev& B, const RowMask* rows,
     return evolve && !(shell && (B.flags & (qsrk::F_SKIPSHELL | qsrk::F_BC)));
 }
 
+// The voxels the explicit-Euler residual sums over: changeable, box faces excluded.
+__device__ __forceinline__ bool stepped_interior(const BoxDev& B, const RowMask* rows, int i, int j, int k) {
+    const bool face = i == 0 || j == 0 || k == 0 || i == B.Nx - 1 || j == B.Ny - 1 || k == B.Nz - 1;
+    return !face && changeable(B, rows, i, j, k);
+}
+
 // ---------------------------------------------------------------------------
 // Convolution (forward, and the input gradient as a forward convolution with
 // the taps flipped and the channel roles swapped) as an implicit GEMM on the
@@ -215,7 +221,14 @@ __global__ __launch_bounds__(64 * TJ) void conv_kernel(ConvArgs a, int ch, int n
                                 const int chn = co + r;
                                 if (chn < CH_Q) {
                                     const long long idx = (long long)chn * a.cs + a.box.off + v;
-                                    a.q_new[idx] = a.q_old[idx] + (c[r] + a.bias[chn]);
+                                    const float y = c[r] + a.bias[chn];
+                                    float d = y;
+                                    if (a.rhs) {  // increment input: the step's own increment, modulated
+                                        const bool face = i == 0 || j == 0 || k == 0 || i == a.box.Nx - 1 ||
+                                                          j == a.box.Ny - 1 || k == a.box.Nz - 1;
+                                        d = face ? 0.0f : (a.rhs[idx] - a.q_old[idx]) * (1.0f + y);
+                                    }
+                                    a.q_new[idx] = __builtin_fmaf(a.out_scale, d, a.q_old[idx]);
                                 }
                             }
                         }
@@ -455,14 +468,18 @@ __global__ __launch_bounds__(256) void crop_kernel(const Snap* __restrict__ snap
     const CropJob J = jobs[b];
     const Snap si = snaps[J.s_in], so = snaps[J.s_out];
     const long long src = ((long long)(J.ox + i) * si.Y + (J.oy + j)) * si.Z + (J.oz + k);
-    float qi[5], d[8];
+    float xi[5], mi[5], d[8];
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
-        qi[c] = si.q[c * si.nvox + src];
-        d[c] = so.q[c * so.nvox + src] - qi[c];
+        const float qi = si.q[c * si.nvox + src];
+        d[c] = (so.q[c * so.nvox + src] - qi) * J.out_scale;
+        mi[c] = J.kind ? si.inc[c * si.nvox + src] * J.in_scale : 0.0f;
+        xi[c] = J.kind == 1 ? mi[c] : qi;
     }
     d[5] = d[6] = d[7] = 0.0f;
-    store_x(w, v, qi);
+    store_x(w, v, xi);
+    *reinterpret_cast<f32x4*>(w.mod + v * 8) = f32x4{mi[0], mi[1], mi[2], mi[3]};
+    *reinterpret_cast<f32x4*>(w.mod + v * 8 + 4) = f32x4{mi[4], 0.0f, 0.0f, 0.0f};
     *reinterpret_cast<f32x4*>(w.tgt + v * 8) = f32x4{d[0], d[1], d[2], d[3]};
     *reinterpret_cast<f32x4*>(w.tgt + v * 8 + 4) = f32x4{d[4], d[5], d[6], d[7]};
     // a crop face that is not a box face sees zero padding instead of the true
@@ -474,13 +491,21 @@ __global__ __launch_bounds__(256) void crop_kernel(const Snap* __restrict__ snap
 }
 
 __global__ __launch_bounds__(256) void pack_batch_kernel(const float* __restrict__ x5, const float* __restrict__ t5,
-                                                         const float* __restrict__ mk, Work w) {
+                                                         const float* __restrict__ mk, const float* __restrict__ m5,
+                                                         Work w) {
     const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= w.V) return;
     float q[5], d[8] = {};
 #pragma unroll
     for (int c = 0; c < 5; ++c) q[c] = x5[v * 5 + c];
     store_x(w, v, q);
+    float mo[5] = {};
+    if (m5) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) mo[c] = m5[v * 5 + c];
+    }
+    *reinterpret_cast<f32x4*>(w.mod + v * 8) = f32x4{mo[0], mo[1], mo[2], mo[3]};
+    *reinterpret_cast<f32x4*>(w.mod + v * 8 + 4) = f32x4{mo[4], 0.0f, 0.0f, 0.0f};
     if (t5) {
 #pragma unroll
         for (int c = 0; c < 5; ++c) d[c] = t5[v * 5 + c];
@@ -503,12 +528,14 @@ __global__ __launch_bounds__(256) void pack_field_kernel(const float* __restrict
     *reinterpret_cast<u32x4*>(x + v * 8) = pk;
 }
 
-__global__ __launch_bounds__(256) void unpack_pred_kernel(const float* __restrict__ pred, long long V,
-                                                          float* __restrict__ out5) {
+__global__ __launch_bounds__(256) void unpack_pred_kernel(Work w, float* __restrict__ out5) {
     const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
+    if (v >= w.V) return;
 #pragma unroll
-    for (int c = 0; c < 5; ++c) out5[v * 5 + c] = pred[v * 8 + c];
+    for (int c = 0; c < 5; ++c) {
+        const float y = w.pred[v * 8 + c];
+        out5[v * 5 + c] = w.gain ? w.mod[v * 8 + c] * (1.0f + y) : y;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -541,9 +568,18 @@ __global__ __launch_bounds__(256) void loss_kernel(Work w, float scale) {
         uint16_t h[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            const float d = c < CH_Q ? w.pred[v * 8 + c] - w.tgt[v * 8 + c] : 0.0f;
+            float d = 0.0f, g = 1.0f;
+            if (c < CH_Q) {
+                const float y = w.pred[v * 8 + c];
+                if (w.gain) {  // prediction m (1 + y): dL/dy = dL/dp * m
+                    g = w.mod[v * 8 + c];
+                    d = g * (1.0f + y) - w.tgt[v * 8 + c];
+                } else {
+                    d = y - w.tgt[v * 8 + c];
+                }
+            }
             acc[0] += double(mk) * double(d) * double(d);
-            h[c] = bf16_bits(scale * mk * d);
+            h[c] = bf16_bits(scale * mk * d * g);
         }
         acc[1] += mk != 0.0f ? 1.0 : 0.0;  // scored voxels (mask values may be weights)
         u32x4 pk;
@@ -602,20 +638,77 @@ __global__ __launch_bounds__(256) void residual_kernel(const float* __restrict__
                                                        long long cs, BoxDev B, const RowMask* __restrict__ rows,
                                                        double* __restrict__ partials) {
     const long long nvox = (long long)B.Nx * B.Ny * B.Nz;
-    double acc[1] = {0.0};
+    double acc[2] = {0.0, 0.0};
     for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
          v += (long long)gridDim.x * blockDim.x) {
         const int k = int(v % B.Nz), j = int((v / B.Nz) % B.Ny), i = int(v / ((long long)B.Nz * B.Ny));
-        const bool face = i == 0 || j == 0 || k == 0 || i == B.Nx - 1 || j == B.Ny - 1 || k == B.Nz - 1;
-        if (face || !changeable(B, rows, i, j, k)) continue;
+        if (!stepped_interior(B, rows, i, j, k)) continue;
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
             const long long idx = (long long)c * cs + B.off + v;
             const double d = double(rhs[idx]) - double(q[idx]);
             acc[0] += d * d;
         }
+        acc[1] += 1.0;
     }
-    block_sum_store<1>(acc, partials + blockIdx.x);
+    block_sum_store<2>(acc, partials + 2 * blockIdx.x);
+}
+
+// The increment of one explicit step, rhs - q, on the voxels the residual sums
+// over (zero elsewhere: rhs is not written on static voxels), per box voxel:
+// five planes of nvox floats (a dataset snapshot's layout).
+__global__ __launch_bounds__(256) void increment_box_kernel(const float* __restrict__ q, const float* __restrict__ rhs,
+                                                            long long cs, BoxDev B, const RowMask* __restrict__ rows,
+                                                            float* __restrict__ dst) {
+    const long long nvox = (long long)B.Nx * B.Ny * B.Nz;
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    const int k = int(v % B.Nz), j = int((v / B.Nz) % B.Ny), i = int(v / ((long long)B.Nz * B.Ny));
+    const bool on = stepped_interior(B, rows, i, j, k);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        const long long idx = (long long)c * cs + B.off + v;
+        dst[(long long)c * nvox + v] = on ? rhs[idx] - q[idx] : 0.0f;
+    }
+}
+
+// Network input from the increment: x = bf16((rhs - q) * inv_s) (5 of 8
+// channels) on the voxels the residual sums over, zero elsewhere.
+__global__ __launch_bounds__(256) void pack_increment_kernel(const float* __restrict__ q, const float* __restrict__ rhs,
+                                                             long long cs, BoxDev B, const RowMask* __restrict__ rows,
+                                                             float inv_s, uint16_t* __restrict__ x) {
+    const long long nvox = (long long)B.Nx * B.Ny * B.Nz;
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    const int k = int(v % B.Nz), j = int((v / B.Nz) % B.Ny), i = int(v / ((long long)B.Nz * B.Ny));
+    const bool on = stepped_interior(B, rows, i, j, k);
+    uint16_t h[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const long long idx = (long long)c * cs + B.off + v;
+        h[c] = (c < CH_Q && on) ? bf16_bits((rhs[idx] - q[idx]) * inv_s) : uint16_t(0);
+    }
+    u32x4 pk;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) pk[c] = unsigned(h[2 * c]) | (unsigned(h[2 * c + 1]) << 16);
+    *reinterpret_cast<u32x4*>(x + v * 8) = pk;
+}
+
+// A proposed step taken back to a fraction of itself: other = cur + ratio (other - cur)
+// on the voxels a step can change (where the proposal wrote).
+__global__ __launch_bounds__(256) void rescale_step_kernel(const float* __restrict__ cur, float* __restrict__ other,
+                                                           long long cs, BoxDev B, const RowMask* __restrict__ rows,
+                                                           float ratio) {
+    const long long nvox = (long long)B.Nx * B.Ny * B.Nz;
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    const int k = int(v % B.Nz), j = int((v / B.Nz) % B.Ny), i = int(v / ((long long)B.Nz * B.Ny));
+    if (!changeable(B, rows, i, j, k)) return;
+#pragma unroll
+    for (int c = 0; c < CH_Q; ++c) {
+        const long long idx = (long long)c * cs + B.off + v;
+        other[idx] = __builtin_fmaf(ratio, other[idx] - cur[idx], cur[idx]);
+    }
 }
 
 __global__ __launch_bounds__(256) void copy_box_kernel(const float* __restrict__ q, long long cs, long long off,
@@ -714,10 +807,10 @@ hipError_t launch_crop(hipStream_t st, const Snap* snaps, const CropJob* jobs, i
     return hipGetLastError();
 }
 
-hipError_t launch_pack_batch(hipStream_t st, const float* x5, const float* t5, const float* m, long long V,
-                             const Work& w) {
+hipError_t launch_pack_batch(hipStream_t st, const float* x5, const float* t5, const float* m, const float* m5,
+                             long long V, const Work& w) {
     (void)V;
-    hipLaunchKernelGGL(pack_batch_kernel, dim3(blocks_for(w.V)), dim3(256), 0, st, x5, t5, m, w);
+    hipLaunchKernelGGL(pack_batch_kernel, dim3(blocks_for(w.V)), dim3(256), 0, st, x5, t5, m, m5, w);
     return hipGetLastError();
 }
 
@@ -726,8 +819,8 @@ hipError_t launch_pack_field(hipStream_t st, const float* q, long long cs, long 
     return hipGetLastError();
 }
 
-hipError_t launch_unpack_pred(hipStream_t st, const float* pred, long long V, float* out5) {
-    hipLaunchKernelGGL(unpack_pred_kernel, dim3(blocks_for(V)), dim3(256), 0, st, pred, V, out5);
+hipError_t launch_unpack_pred(hipStream_t st, const Work& w, float* out5) {
+    hipLaunchKernelGGL(unpack_pred_kernel, dim3(blocks_for(w.V)), dim3(256), 0, st, w, out5);
     return hipGetLastError();
 }
 
@@ -763,6 +856,29 @@ hipError_t launch_residual(hipStream_t st, const float* q, const float* rhs, lon
     const unsigned g = reduce_blocks(nvox);
     *nblk = g;
     hipLaunchKernelGGL(residual_kernel, dim3(g), dim3(256), 0, st, q, rhs, cs, box, rows, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_increment_box(hipStream_t st, const float* q, const float* rhs, long long cs,
+                                const qsrk::BoxDev& box, const qsrk::RowMask* rows, float* dst) {
+    const long long nvox = (long long)box.Nx * box.Ny * box.Nz;
+    hipLaunchKernelGGL(increment_box_kernel, dim3(blocks_for(nvox)), dim3(256), 0, st, q, rhs, cs, box, rows, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_increment(hipStream_t st, const float* q, const float* rhs, long long cs,
+                                 const qsrk::BoxDev& box, const qsrk::RowMask* rows, float inv_s, uint16_t* x) {
+    const long long nvox = (long long)box.Nx * box.Ny * box.Nz;
+    hipLaunchKernelGGL(pack_increment_kernel, dim3(blocks_for(nvox)), dim3(256), 0, st, q, rhs, cs, box, rows, inv_s,
+                       x);
+    return hipGetLastError();
+}
+
+hipError_t launch_rescale_step(hipStream_t st, const float* cur, float* other, long long cs, const qsrk::BoxDev& box,
+                               const qsrk::RowMask* rows, float ratio) {
+    const long long nvox = (long long)box.Nx * box.Ny * box.Nz;
+    hipLaunchKernelGGL(rescale_step_kernel, dim3(blocks_for(nvox)), dim3(256), 0, st, cur, other, cs, box, rows,
+                       ratio);
     return hipGetLastError();
 }
 

@@ -2,8 +2,9 @@
 // host launchers of its kernels (qsr_surr.hip).
 //
 // Network: three 3x3x3 "same" convolutions (zero padding), 5 -> 32 -> 32 -> 5
-// channels, GELU (tanh form) after the first two.  It maps a Q field (the five
-// independent components Qxx, Qxy, Qxz, Qyy, Qyz) to a predicted change dQ.
+// channels, GELU (tanh form) after the first two.  It maps a five-component
+// field (the independent components Qxx, Qxy, Qxz, Qyy, Qyz of Q itself, or of
+// one explicit step's increment of Q, normalised) to a predicted change dQ.
 //
 //   z1 = conv(W1, x) + b1     a1 = bf16(gelu(z1))
 //   z2 = conv(W2, a1) + b2    a2 = bf16(gelu(z2))
@@ -64,7 +65,9 @@ struct Frags {
 enum : int {
     EPI_HIDDEN = 0,  // z = acc + b -> optional FP32 z, a = bf16(gelu(z)) (NDHWC and optional cm)
     EPI_PRED = 1,    // y = acc + b -> FP32 NDHWC, 8 channels (5 used)
-    EPI_APPLY = 2,   // q_new = q_old + y on the voxels a step can change
+    EPI_APPLY = 2,   // q_new = q_old + out_scale * y (field input) or
+                     // q_old + out_scale * u (1 + y), u = rhs - q_old (increment input),
+                     // on the voxels a step can change
     EPI_BWD = 3,     // dz = bf16(acc * gelu'(z_in)) (NDHWC and cm)
 };
 
@@ -82,24 +85,34 @@ struct ConvArgs {
     float* pred;            // EPI_PRED
     const float* q_old;     // EPI_APPLY: SoA planes of the context
     float* q_new;
+    float out_scale;
+    const float* rhs;       // EPI_APPLY under increment input: one explicit step of q_old (else null)
     long long cs;
     qsrk::BoxDev box;
     const qsrk::RowMask* rows;
 };
 
 // One dataset snapshot: five FP32 planes of nvox voxels plus a byte per voxel
-// (1 = a plain step can change it).
+// (1 = a plain step can change it), and optionally the increment of one
+// explicit step from it in the same layout (null if not recorded).
 struct Snap {
     const float* q;
     const uint8_t* mask;
     int X, Y, Z, pad;
     long long nvox;
+    const float* inc;
 };
-// One training crop: pair (input snapshot, target snapshot), its origin and
-// the weight of its scored voxels (the inverse of the pair's target mean square).
+// One training crop: pair (input snapshot, target snapshot), its origin, the
+// weight of its scored voxels (the inverse of the pair's target mean square)
+// and the input kind.  Field input: x = q_in, target = q_out - q_in.
+// Increment input: x = inc_in * in_scale, target = (q_out - q_in) * out_scale,
+// modulator inc_in * in_scale.  Gain input: x = q_in, the rest as increment input.
 struct CropJob {
     int s_in, s_out, ox, oy, oz;
     float weight;
+    int kind;
+    float in_scale, out_scale;
+    int pad;
 };
 
 // Training workspace views (batch of n crops of edge C, V = n*C^3 voxels).
@@ -109,6 +122,8 @@ struct Work {
     float *z1, *z2;              // 32 ch
     uint16_t *a1, *a1_cm, *a2, *a2_cm;
     float *pred, *tgt, *mask;    // 8 ch, 8 ch, 1
+    float* mod;                  // 8 ch FP32: the modulator m of a modulated prediction
+    int gain;                    // 1: the prediction is m (1 + y), else y
     uint16_t *dy, *dy_cm;        // 8 ch
     uint16_t *dz2, *dz2_cm, *dz1, *dz1_cm;
     float* part;                 // weight-gradient partials
@@ -128,8 +143,10 @@ hipError_t launch_pack_frags(hipStream_t st, const float* params, Frags f);
 hipError_t launch_adam(hipStream_t st, float* params, const float* grad, float* m, float* v, long long n, float lr_t,
                        float b1, float b2, float eps, float inv_sqrt_bc2);
 hipError_t launch_crop(hipStream_t st, const Snap* snaps, const CropJob* jobs, int nbatch, int C, const Work& w);
-hipError_t launch_pack_batch(hipStream_t st, const float* x5, const float* t5, const float* m, long long V,
-                             const Work& w);
+// Host batch (staged on the device): x5, t5 (may be null), mask (may be null), and the
+// modulator m5 of a modulated prediction (may be null).
+hipError_t launch_pack_batch(hipStream_t st, const float* x5, const float* t5, const float* m, const float* m5,
+                             long long V, const Work& w);
 hipError_t launch_pack_field(hipStream_t st, const float* q, long long cs, long long off, long long V, uint16_t* x);
 // Output gradient of the masked mean-square loss: dy = 2 m (y - t) / (5 V) in BF16;
 // partials of sum m (y - t)^2 and sum m per block; returns the block count.
@@ -141,10 +158,22 @@ hipError_t launch_pair_power(hipStream_t st, const float* qa, const float* qb, c
                              double* partials, long long* nblk);
 hipError_t launch_mask_bytes(hipStream_t st, long long nvox, const qsrk::BoxDev& box, const qsrk::RowMask* rows,
                              uint8_t* out);
-// Partials of sum over the voxels a step can change (box faces excluded) of |rhs - q|^2.
+// Partials (per block: sum of |rhs - q|^2, voxel count) over the voxels a step can
+// change, box faces excluded.
 hipError_t launch_residual(hipStream_t st, const float* q, const float* rhs, long long cs, const qsrk::BoxDev& box,
                            const qsrk::RowMask* rows, double* partials, long long* nblk);
 hipError_t launch_copy_box(hipStream_t st, const float* q, long long cs, long long off, long long nvox, float* dst);
-hipError_t launch_unpack_pred(hipStream_t st, const float* pred, long long V, float* out5);
+// One explicit step's increment rhs - q of a box on the voxels the residual sums
+// over (zero elsewhere), into five planes of the box's voxel count.
+hipError_t launch_increment_box(hipStream_t st, const float* q, const float* rhs, long long cs,
+                                const qsrk::BoxDev& box, const qsrk::RowMask* rows, float* dst);
+// Network input (NDHWC, 8 channels, BF16) from the increment scaled by inv_s.
+hipError_t launch_pack_increment(hipStream_t st, const float* q, const float* rhs, long long cs,
+                                 const qsrk::BoxDev& box, const qsrk::RowMask* rows, float inv_s, uint16_t* x);
+// other = cur + ratio (other - cur) on the voxels a step can change.
+hipError_t launch_rescale_step(hipStream_t st, const float* cur, float* other, long long cs, const qsrk::BoxDev& box,
+                               const qsrk::RowMask* rows, float ratio);
+// (nbatch, X, Y, Z, 5) output: y, or m (1 + y) when w.gain is set.
+hipError_t launch_unpack_pred(hipStream_t st, const Work& w, float* out5);
 
 }  // namespace qsrs

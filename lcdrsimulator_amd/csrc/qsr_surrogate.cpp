@@ -46,9 +46,19 @@ struct SplitMix {
 struct SnapH {
     float* q = nullptr;
     uint8_t* mask = nullptr;
+    float* inc = nullptr;  // one explicit step's increment (dataset_add_increment)
+    double inc_rms = 0.0;  // its RMS per component over the voxels the residual sums over
     int X = 0, Y = 0, Z = 0;
     long long nvox = 0;
 };
+
+void free_snap(SnapH& s) {
+    discard(hipFree(s.q));
+    discard(hipFree(s.mask));
+    discard(hipFree(s.inc));
+    s.q = s.inc = nullptr;
+    s.mask = nullptr;
+}
 
 struct Net {
     int device = 0;
@@ -58,6 +68,8 @@ struct Net {
     void* frag_mem = nullptr;
     long long t = 0;  // Adam steps taken
     bool has_params = false;
+    int input_kind = QSR_SURR_INPUT_FIELD;
+    double horizon = 1.0;  // N: increment input predicts dQ = N s y
     // training / evaluation workspace
     Work w{};
     long long w_cap = 0;
@@ -78,10 +90,7 @@ struct Net {
     hipEvent_t ev[2] = {nullptr, nullptr};
 
     ~Net() {
-        for (SnapH& s : snaps) {
-            discard(hipFree(s.q));
-            discard(hipFree(s.mask));
-        }
+        for (SnapH& s : snaps) free_snap(s);
         for (void* p : {(void*)P, (void*)G, (void*)M, (void*)Vm, frag_mem, w_mem, (void*)stage, (void*)d_out,
                         (void*)ax, (void*)aa1, (void*)aa2, (void*)d_snaps, (void*)d_jobs})
             discard(hipFree(p));
@@ -106,13 +115,14 @@ int ensure_work(Net& n, long long V) {
     auto al = [](long long b) { return (b + 255) / 256 * 256; };
     const long long sizes[] = {b16_8,  b16_8,  f32_32, f32_32, b16_32, b16_32, b16_32, b16_32, f32_8, f32_8,
                                f32_1,  b16_8,  b16_8,  b16_32, b16_32, b16_32, b16_32, part_floats() * 4,
-                               2048 * 2 * 8};
+                               2048 * 2 * 8, f32_8};
+    constexpr int NBUF = int(sizeof(sizes) / sizeof(sizes[0]));
     long long total = 0;
     for (long long s : sizes) total += al(s);
     HIP_TRY(hipMalloc(&n.w_mem, total));
     char* p = static_cast<char*>(n.w_mem);
-    void* ptrs[19];
-    for (int i = 0; i < 19; ++i) {
+    void* ptrs[NBUF];
+    for (int i = 0; i < NBUF; ++i) {
         ptrs[i] = p;
         p += al(sizes[i]);
     }
@@ -136,6 +146,7 @@ int ensure_work(Net& n, long long V) {
     w.dz1_cm = (uint16_t*)ptrs[16];
     w.part = (float*)ptrs[17];
     w.red = (double*)ptrs[18];
+    w.mod = (float*)ptrs[19];
     w.V = V;
     n.w_cap = V;
     return QSR_OK;
@@ -378,9 +389,13 @@ int qsr_surr_load(void* net, const char* path) {
     return qsr_surr_set_params(net, P.data(), P.size());
 }
 
-int qsr_surr_eval(void* net, int nbatch, int X, int Y, int Z, const float* x5, const float* target5,
-                  const float* mask, double* loss_out, float* pred5, float* grad) {
-    Net* n = as_net(net);
+}  // extern "C"
+
+namespace {
+
+// One pass over a host batch; with m5 the prediction is m5 (1 + y), else y.
+int eval_batch(Net* n, int nbatch, int X, int Y, int Z, const float* x5, const float* m5, const float* target5,
+               const float* mask, double* loss_out, float* pred5, float* grad) {
     if (int rc = check_net(n)) return rc;
     if (nbatch < 1 || X < 1 || Y < 1 || Z < 1 || !x5) return fail(QSR_E_ARG, "bad batch");
     if ((loss_out || grad) && Z % 8 != 0) return fail(QSR_E_ARG, "Z must be a multiple of 8 for the loss");
@@ -388,7 +403,7 @@ int qsr_surr_eval(void* net, int nbatch, int X, int Y, int Z, const float* x5, c
     const long long V = (long long)nbatch * X * Y * Z;
     if (int rc = ensure_work(*n, V)) return rc;
     if (int rc = ensure_out(*n, 2)) return rc;
-    const long long need = V * 11;
+    const long long need = V * 16;
     if (need > n->stage_cap) {
         discard(hipFree(n->stage));
         n->stage = nullptr;
@@ -399,10 +414,13 @@ int qsr_surr_eval(void* net, int nbatch, int X, int Y, int Z, const float* x5, c
     float* dx = n->stage;
     float* dt = n->stage + V * 5;
     float* dm = n->stage + V * 10;
+    float* dg = n->stage + V * 11;
     HIP_TRY(hipMemcpyAsync(dx, x5, sizeof(float) * V * 5, hipMemcpyHostToDevice, n->st));
     if (target5) HIP_TRY(hipMemcpyAsync(dt, target5, sizeof(float) * V * 5, hipMemcpyHostToDevice, n->st));
     if (mask) HIP_TRY(hipMemcpyAsync(dm, mask, sizeof(float) * V, hipMemcpyHostToDevice, n->st));
-    HIP_TRY(launch_pack_batch(n->st, dx, target5 ? dt : nullptr, mask ? dm : nullptr, V, n->w));
+    if (m5) HIP_TRY(hipMemcpyAsync(dg, m5, sizeof(float) * V * 5, hipMemcpyHostToDevice, n->st));
+    n->w.gain = m5 ? 1 : 0;
+    HIP_TRY(launch_pack_batch(n->st, dx, target5 ? dt : nullptr, mask ? dm : nullptr, m5 ? dg : nullptr, V, n->w));
     if (int rc = forward(*n, nbatch, X, Y, Z, grad != nullptr)) return rc;
     if (loss_out || grad) {
         if (int rc = loss(*n, n->d_out)) return rc;
@@ -416,11 +434,26 @@ int qsr_surr_eval(void* net, int nbatch, int X, int Y, int Z, const float* x5, c
         if (loss_out) *loss_out = s[0] / (5.0 * double(V));
     }
     if (pred5) {
-        HIP_TRY(launch_unpack_pred(n->st, n->w.pred, V, dx));
+        HIP_TRY(launch_unpack_pred(n->st, n->w, dx));
         HIP_TRY(hipMemcpyAsync(pred5, dx, sizeof(float) * V * 5, hipMemcpyDeviceToHost, n->st));
     }
     HIP_TRY(hipStreamSynchronize(n->st));
     return QSR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsr_surr_eval(void* net, int nbatch, int X, int Y, int Z, const float* x5, const float* target5,
+                  const float* mask, double* loss_out, float* pred5, float* grad) {
+    return eval_batch(as_net(net), nbatch, X, Y, Z, x5, nullptr, target5, mask, loss_out, pred5, grad);
+}
+
+int qsr_surr_eval_gain(void* net, int nbatch, int X, int Y, int Z, const float* x5, const float* m5,
+                       const float* target5, const float* mask, double* loss_out, float* pred5, float* grad) {
+    if (!m5) return fail(QSR_E_ARG, "null modulator");
+    return eval_batch(as_net(net), nbatch, X, Y, Z, x5, m5, target5, mask, loss_out, pred5, grad);
 }
 
 int qsr_surr_adam_step(void* net, const float* grad, double lr) {
@@ -434,31 +467,68 @@ int qsr_surr_adam_step(void* net, const float* grad, double lr) {
     return QSR_OK;
 }
 
-int qsr_surr_dataset_add(void* net, void* ctx, int box, int* snapshot) {
-    Net* n = as_net(net);
+}  // extern "C"
+
+namespace {
+
+int dataset_add(Net* n, void* ctx, int box, bool with_increment, int* snapshot) {
     if (!n) return fail(QSR_E_ARG, "null network");
     qsrc::FieldView v;
     if (int rc = qsrc::field_view(ctx, box, &v)) return rc;
     if (!v.uploaded) return fail(QSR_E_STATE, "box has no field");
     if (v.device != n->device) return fail(QSR_E_ARG, "network and context are on different devices");
     HIP_TRY(hipSetDevice(n->device));
+    double res = 0.0;
+    long long cnt = 0;
+    if (with_increment) {
+        // one explicit step of the whole context into its step buffer; the field is unchanged
+        if (int rc = qsrc::euler_residual(ctx, box, &res, &cnt)) return rc;
+        if (int rc = qsrc::field_view(ctx, box, &v)) return rc;
+    }
     SnapH s;
     s.X = v.box.Nx;
     s.Y = v.box.Ny;
     s.Z = v.box.Nz;
     s.nvox = v.nvox;
-    HIP_TRY(hipMalloc(&s.q, sizeof(float) * 5 * s.nvox));
-    if (hipMalloc(&s.mask, s.nvox) != hipSuccess) {
-        discard(hipFree(s.q));
+    const bool ok = hipMalloc(&s.q, sizeof(float) * 5 * s.nvox) == hipSuccess &&
+                    hipMalloc(&s.mask, s.nvox) == hipSuccess &&
+                    (!with_increment || hipMalloc(&s.inc, sizeof(float) * 5 * s.nvox) == hipSuccess);
+    if (!ok) {
+        free_snap(s);
         return fail(QSR_E_HIP, "out of device memory for the snapshot");
     }
+    s.inc_rms = cnt > 0 ? res / std::sqrt(5.0 * double(cnt)) : 0.0;
     n->snaps.push_back(s);
     n->d_snaps_stale = true;
     // on the context's stream, ordered after its steps
     HIP_TRY(launch_copy_box(v.st, v.cur, v.cs, v.box.off, s.nvox, s.q));
     HIP_TRY(launch_mask_bytes(v.st, s.nvox, v.box, v.rows, s.mask));
+    if (with_increment) HIP_TRY(launch_increment_box(v.st, v.cur, v.rhs, v.cs, v.box, v.rows, s.inc));
     HIP_TRY(hipStreamSynchronize(v.st));
     if (snapshot) *snapshot = int(n->snaps.size()) - 1;
+    return QSR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsr_surr_dataset_add(void* net, void* ctx, int box, int* snapshot) {
+    return dataset_add(as_net(net), ctx, box, false, snapshot);
+}
+
+int qsr_surr_dataset_add_increment(void* net, void* ctx, int box, int* snapshot) {
+    return dataset_add(as_net(net), ctx, box, true, snapshot);
+}
+
+int qsr_surr_set_input(void* net, int kind, double horizon) {
+    Net* n = as_net(net);
+    if (!n) return fail(QSR_E_ARG, "null network");
+    if (kind != QSR_SURR_INPUT_FIELD && kind != QSR_SURR_INPUT_INCREMENT && kind != QSR_SURR_INPUT_GAIN)
+        return fail(QSR_E_ARG, "unknown input kind");
+    if (!(horizon > 0.0) || !std::isfinite(horizon)) return fail(QSR_E_ARG, "horizon must be positive");
+    n->input_kind = kind;
+    n->horizon = horizon;
     return QSR_OK;
 }
 
@@ -474,10 +544,7 @@ int qsr_surr_dataset_clear(void* net) {
     if (!n) return fail(QSR_E_ARG, "null network");
     HIP_TRY(hipSetDevice(n->device));
     HIP_TRY(hipStreamSynchronize(n->st));
-    for (SnapH& s : n->snaps) {
-        discard(hipFree(s.q));
-        discard(hipFree(s.mask));
-    }
+    for (SnapH& s : n->snaps) free_snap(s);
     n->snaps.clear();
     // the device table points at the buffers just freed: drop it, so training
     // rebuilds it even when the next dataset has the same number of snapshots
@@ -504,6 +571,8 @@ int qsr_surr_train(void* net, const int* pairs, int npairs, int iters, int batch
             return fail(QSR_E_ARG, "pair " + std::to_string(p) + ": snapshots differ in size");
         if (sa.X < crop || sa.Y < crop || sa.Z < crop)
             return fail(QSR_E_ARG, "pair " + std::to_string(p) + ": box smaller than the crop");
+        if (n->input_kind != QSR_SURR_INPUT_FIELD && !(sa.inc && sa.inc_rms > 0.0 && std::isfinite(sa.inc_rms)))
+            return fail(QSR_E_ARG, "pair " + std::to_string(p) + ": input snapshot has no (nonzero) increment");
     }
     HIP_TRY(hipSetDevice(n->device));
     const long long V = (long long)batch * crop * crop * crop;
@@ -515,7 +584,7 @@ int qsr_surr_train(void* net, const int* pairs, int npairs, int iters, int batch
         std::vector<Snap> hs(n->snaps.size());
         for (size_t i = 0; i < hs.size(); ++i) {
             const SnapH& s = n->snaps[i];
-            hs[i] = Snap{s.q, s.mask, s.X, s.Y, s.Z, 0, s.nvox};
+            hs[i] = Snap{s.q, s.mask, s.X, s.Y, s.Z, 0, s.nvox, s.inc};
         }
         HIP_TRY(hipMalloc(&n->d_snaps, sizeof(Snap) * hs.size()));
         HIP_TRY(hipMemcpy(n->d_snaps, hs.data(), sizeof(Snap) * hs.size(), hipMemcpyHostToDevice));
@@ -541,21 +610,31 @@ int qsr_surr_train(void* net, const int* pairs, int npairs, int iters, int batch
     std::vector<double> pw(2 * size_t(npairs));
     HIP_TRY(hipMemcpyAsync(pw.data(), n->d_out, sizeof(double) * pw.size(), hipMemcpyDeviceToHost, n->st));
     HIP_TRY(hipStreamSynchronize(n->st));
-    std::vector<float> weight(npairs, 1.0f);
+    // increment and gain input: target = (q_out - q_in) / (N s), modulator inc / s, s of the
+    // input snapshot; the network reads inc / s (increment) or q_in (gain)
+    const bool incr = n->input_kind != QSR_SURR_INPUT_FIELD;
+    std::vector<float> weight(npairs, 1.0f), in_scale(npairs, 1.0f), out_scale(npairs, 1.0f);
     for (int p = 0; p < npairs; ++p) {
+        if (incr) {
+            const double s_in = n->snaps[pairs[2 * p]].inc_rms;
+            in_scale[p] = float(1.0 / s_in);
+            out_scale[p] = float(1.0 / (n->horizon * s_in));
+        }
         const double ms = pw[2 * p + 1] > 0 ? pw[2 * p] / (5.0 * pw[2 * p + 1]) : 0.0;
-        weight[p] = float(1.0 / std::max(ms, weight_floor));
+        weight[p] = float(1.0 / std::max(ms * double(out_scale[p]) * double(out_scale[p]), weight_floor));
     }
     std::vector<CropJob> jobs(njobs);
     SplitMix r{seed};
     for (long long j = 0; j < njobs; ++j) {
         const int p = r.below(npairs);
         const SnapH& s = n->snaps[pairs[2 * p]];
-        jobs[j] = CropJob{pairs[2 * p], pairs[2 * p + 1], r.below(s.X - crop + 1), r.below(s.Y - crop + 1),
-                          r.below(s.Z - crop + 1), weight[p]};
+        const int ox = r.below(s.X - crop + 1), oy = r.below(s.Y - crop + 1), oz = r.below(s.Z - crop + 1);
+        jobs[j] = CropJob{pairs[2 * p], pairs[2 * p + 1], ox, oy, oz, weight[p], n->input_kind, in_scale[p],
+                          out_scale[p], 0};
     }
     HIP_TRY(hipStreamSynchronize(n->st));
     HIP_TRY(hipMemcpy(n->d_jobs, jobs.data(), sizeof(CropJob) * njobs, hipMemcpyHostToDevice));
+    n->w.gain = incr ? 1 : 0;
     for (int it = 0; it < iters; ++it) {
         HIP_TRY(launch_crop(n->st, n->d_snaps, n->d_jobs + (long long)it * batch, batch, crop, n->w));
         if (int rc = forward(*n, batch, crop, crop, crop, true)) return rc;
@@ -572,10 +651,15 @@ int qsr_surr_train(void* net, const int* pairs, int npairs, int iters, int batch
     return QSR_OK;
 }
 
-int qsr_surr_apply(void* net, void* ctx, int mode, qsr_surr_result* out) {
+int qsr_surr_apply_ladder(void* net, void* ctx, int mode, const double* scales, int nscales,
+                          qsr_surr_result* out) {
     Net* n = as_net(net);
     if (int rc = check_net(n)) return rc;
-    if (!out || mode < QSR_SURR_CHECKED || mode > QSR_SURR_PROPOSE) return fail(QSR_E_ARG, "bad argument");
+    if (!out || mode < QSR_SURR_CHECKED || mode > QSR_SURR_PROPOSE || !scales || nscales < 1)
+        return fail(QSR_E_ARG, "bad argument");
+    for (int i = 0; i < nscales; ++i)
+        if (!(scales[i] > 0.0) || !std::isfinite(scales[i])) return fail(QSR_E_ARG, "scales must be positive");
+    if (mode != QSR_SURR_CHECKED) nscales = 1;
     qsrc::FieldView v;
     if (int rc = qsrc::field_view(ctx, 0, &v)) return rc;
     if (v.nboxes != 1) return fail(QSR_E_ARG, "the learned step applies to single-box contexts");
@@ -588,7 +672,20 @@ int qsr_surr_apply(void* net, void* ctx, int mode, qsr_surr_result* out) {
     qsr_obs o{};
     if (int rc = qsr_observables(ctx, 0, &oo, &o)) return rc;
     r.F_before = o.total;
-    if (int rc = qsrc::euler_residual(ctx, 0, &r.res_before)) return rc;
+    long long cnt = 0;
+    if (int rc = qsrc::euler_residual(ctx, 0, &r.res_before, &cnt)) return rc;
+    if (int rc = qsrc::field_view(ctx, 0, &v)) return rc;  // the step buffer now holds E(Q)
+
+    // increment and gain input modulate u = E(Q) - Q (s its RMS per component): nothing to
+    // propose if it is zero
+    const bool incr = n->input_kind != QSR_SURR_INPUT_FIELD;
+    const double s_inc = cnt > 0 ? r.res_before / std::sqrt(5.0 * double(cnt)) : 0.0;
+    if (incr && !(s_inc > 0.0 && std::isfinite(s_inc))) {
+        r.F_after = r.F_before;
+        r.res_after = r.res_before;
+        *out = r;
+        return QSR_OK;
+    }
 
     const long long V = v.nvox;
     if (V > n->a_cap) {
@@ -606,7 +703,10 @@ int qsr_surr_apply(void* net, void* ctx, int mode, qsr_surr_result* out) {
     const hipStream_t st = v.st;
     ConvArgs a = conv_dims(1, v.box.Nx, v.box.Ny, v.box.Nz);
     HIP_TRY(hipEventRecord(n->ev[0], st));
-    HIP_TRY(launch_pack_field(st, v.cur, v.cs, v.box.off, V, n->ax));
+    if (n->input_kind == QSR_SURR_INPUT_INCREMENT)
+        HIP_TRY(launch_pack_increment(st, v.cur, v.rhs, v.cs, v.box, v.rows, float(1.0 / s_inc), n->ax));
+    else
+        HIP_TRY(launch_pack_field(st, v.cur, v.cs, v.box.off, V, n->ax));
     a.in = n->ax;
     a.frag = n->fr.f1;
     a.bias = n->fr.b1;
@@ -623,6 +723,8 @@ int qsr_surr_apply(void* net, void* ctx, int mode, qsr_surr_result* out) {
     a.a_out = nullptr;
     a.q_old = v.cur;
     a.q_new = v.other;
+    a.out_scale = float(scales[0] * (incr ? n->horizon : 1.0));
+    a.rhs = incr ? v.rhs : nullptr;
     a.cs = v.cs;
     a.box = v.box;
     a.rows = v.rows;
@@ -633,22 +735,36 @@ int qsr_surr_apply(void* net, void* ctx, int mode, qsr_surr_result* out) {
     HIP_TRY(hipEventElapsedTime(&ms, n->ev[0], n->ev[1]));
     r.ms_network = ms;
 
-    if (int rc = qsrc::commit_other(ctx)) return rc;
-    if (int rc = qsr_observables(ctx, 0, &oo, &o)) return rc;
-    r.F_after = o.total;
-    if (int rc = qsrc::euler_residual(ctx, 0, &r.res_after)) return rc;
+    // proposal i lies in the other buffer; the previous field stays in the current one
     bool keep = false;
-    if (mode == QSR_SURR_FORCE) keep = true;
-    else if (mode == QSR_SURR_CHECKED)
-        keep = std::isfinite(r.F_after) && std::isfinite(r.res_after) && r.F_after <= r.F_before &&
-               r.res_after <= r.res_before;
-    if (!keep) {
-        // the previous field is intact in the other buffer
+    for (int i = 0; i < nscales && !keep; ++i) {
+        if (i > 0) {
+            if (int rc = qsrc::field_view(ctx, 0, &v)) return rc;
+            HIP_TRY(launch_rescale_step(st, v.cur, v.other, v.cs, v.box, v.rows, float(scales[i] / scales[i - 1])));
+        }
+        r.scale = scales[i];
+        r.tried = i + 1;
         if (int rc = qsrc::commit_other(ctx)) return rc;
+        if (int rc = qsr_observables(ctx, 0, &oo, &o)) return rc;
+        r.F_after = o.total;
+        if (int rc = qsrc::euler_residual(ctx, 0, &r.res_after)) return rc;
+        if (mode == QSR_SURR_FORCE) keep = true;
+        else if (mode == QSR_SURR_CHECKED)
+            keep = std::isfinite(r.F_after) && std::isfinite(r.res_after) && r.F_after <= r.F_before &&
+                   r.res_after <= r.res_before;
+        if (!keep) {
+            // back to the previous field; the proposal stays in the other buffer
+            if (int rc = qsrc::commit_other(ctx)) return rc;
+        }
     }
     r.accepted = keep ? 1 : 0;
     *out = r;
     return QSR_OK;
+}
+
+int qsr_surr_apply(void* net, void* ctx, int mode, qsr_surr_result* out) {
+    const double one = 1.0;
+    return qsr_surr_apply_ladder(net, ctx, mode, &one, 1, out);
 }
 
 }  // extern "C"

@@ -77,14 +77,17 @@ EXPORTS = (
     "qsr_surr_num_params", "qsr_surr_create", "qsr_surr_destroy", "qsr_surr_init_params", "qsr_surr_set_params",
     "qsr_surr_get_params", "qsr_surr_save", "qsr_surr_load", "qsr_surr_eval", "qsr_surr_adam_step",
     "qsr_surr_dataset_add", "qsr_surr_dataset_size", "qsr_surr_dataset_clear", "qsr_surr_train", "qsr_surr_apply",
+    "qsr_surr_dataset_add_increment", "qsr_surr_set_input", "qsr_surr_apply_ladder", "qsr_surr_eval_gain",
 )
 
 SURR_CHECKED, SURR_FORCE, SURR_PROPOSE = 0, 1, 2
+SURR_INPUT_FIELD, SURR_INPUT_INCREMENT, SURR_INPUT_GAIN = 0, 1, 2
 
 
 class _SurrResult(C.Structure):
     _fields_ = [("F_before", C.c_double), ("F_after", C.c_double), ("res_before", C.c_double),
-                ("res_after", C.c_double), ("accepted", C.c_int32), ("ms_network", C.c_double)]
+                ("res_after", C.c_double), ("accepted", C.c_int32), ("ms_network", C.c_double),
+                ("scale", C.c_double), ("tried", C.c_int32)]
 
 
 def load():
@@ -140,6 +143,10 @@ def load():
     L.qsr_surr_train.argtypes = [vp, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                  C.c_uint64, C.c_double, dp]
     L.qsr_surr_apply.argtypes = [vp, vp, C.c_int, C.POINTER(_SurrResult)]
+    L.qsr_surr_dataset_add_increment.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
+    L.qsr_surr_set_input.argtypes = [vp, C.c_int, C.c_double]
+    L.qsr_surr_apply_ladder.argtypes = [vp, vp, C.c_int, dp, C.c_int, C.POINTER(_SurrResult)]
+    L.qsr_surr_eval_gain.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, fp, fp, fp, fp, dp, fp, fp]
     _lib = L
     return L
 
@@ -419,8 +426,9 @@ class Surrogate:
     def load(self, path):
         _check(load().qsr_surr_load(self._h, os.fsencode(str(path))))
 
-    def eval(self, x5, target5=None, mask=None, want_pred=True, want_grad=False):
-        """x5, target5: (n, X, Y, Z, 5); mask: (n, X, Y, Z).  Returns (loss, pred5, grad)."""
+    def eval(self, x5, target5=None, mask=None, want_pred=True, want_grad=False, modulator=None):
+        """x5, target5: (n, X, Y, Z, 5); mask: (n, X, Y, Z).  Returns (loss, pred5, grad).
+        With ``modulator`` g (same shape as x5) the prediction is g (1 + y), else y."""
         x5 = np.ascontiguousarray(x5, dtype=np.float32)
         nb, X, Y, Z, c5 = x5.shape
         if c5 != 5:
@@ -435,8 +443,15 @@ class Surrogate:
         grad = np.empty(self.nparams, dtype=np.float32) if want_grad else None
         want_loss = t5 is not None or want_grad
         loss = C.c_double(0.0)
-        _check(load().qsr_surr_eval(self._h, nb, X, Y, Z, _fptr(x5), _fptr(t5), _fptr(m),
-                                    C.byref(loss) if want_loss else None, _fptr(pred), _fptr(grad)))
+        if modulator is None:
+            _check(load().qsr_surr_eval(self._h, nb, X, Y, Z, _fptr(x5), _fptr(t5), _fptr(m),
+                                        C.byref(loss) if want_loss else None, _fptr(pred), _fptr(grad)))
+        else:
+            g = np.ascontiguousarray(modulator, dtype=np.float32)
+            if g.shape != x5.shape:
+                raise ValueError("modulator shape mismatch")
+            _check(load().qsr_surr_eval_gain(self._h, nb, X, Y, Z, _fptr(x5), _fptr(g), _fptr(t5), _fptr(m),
+                                             C.byref(loss) if want_loss else None, _fptr(pred), _fptr(grad)))
         return (loss.value if want_loss else None), pred, grad
 
     def adam_step(self, grad, lr: float):
@@ -445,9 +460,18 @@ class Surrogate:
             raise ValueError("gradient size mismatch")
         _check(load().qsr_surr_adam_step(self._h, _fptr(g), float(lr)))
 
-    def dataset_add(self, solver: "Solver", box: int = 0) -> int:
+    def set_input(self, kind: int, horizon: float = 1.0):
+        """SURR_INPUT_FIELD: the network reads Q and predicts dQ = y.  SURR_INPUT_INCREMENT
+        and SURR_INPUT_GAIN: it modulates u, one explicit step's increment of Q:
+        dQ = horizon * u * (1 + y), reading u / s (s the RMS of u per component) or Q."""
+        _check(load().qsr_surr_set_input(self._h, int(kind), float(horizon)))
+
+    def dataset_add(self, solver: "Solver", box: int = 0, increment: bool = False) -> int:
+        """Snapshot a box's field (and, with ``increment``, one explicit step's increment
+        from it, which the input snapshot of a pair needs under increment input)."""
         i = C.c_int()
-        _check(load().qsr_surr_dataset_add(self._h, solver._h, box, C.byref(i)))
+        fn = load().qsr_surr_dataset_add_increment if increment else load().qsr_surr_dataset_add
+        _check(fn(self._h, solver._h, box, C.byref(i)))
         return i.value
 
     def dataset_size(self) -> int:
@@ -469,7 +493,14 @@ class Surrogate:
                                      hist.ctypes.data_as(C.POINTER(C.c_double))))
         return hist
 
-    def apply(self, solver: "Solver", mode: int = SURR_CHECKED) -> dict:
+    def apply(self, solver: "Solver", mode: int = SURR_CHECKED, scales=None) -> dict:
+        """Propose Q + dQ (or, with ``scales``, Q + scales[i] dQ in turn, keeping the first
+        that passes the check) on a single-box solver."""
         r = _SurrResult()
-        _check(load().qsr_surr_apply(self._h, solver._h, int(mode), C.byref(r)))
+        if scales is None:
+            _check(load().qsr_surr_apply(self._h, solver._h, int(mode), C.byref(r)))
+        else:
+            sc = np.ascontiguousarray(scales, dtype=np.float64).ravel()
+            _check(load().qsr_surr_apply_ladder(self._h, solver._h, int(mode),
+                                                sc.ctypes.data_as(C.POINTER(C.c_double)), sc.size, C.byref(r)))
         return {f: getattr(r, f) for f, _ in _SurrResult._fields_}

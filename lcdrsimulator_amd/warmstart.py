@@ -11,11 +11,11 @@ a learned step (:class:`lcdrsimulator_amd.Surrogate`) may be interleaved:
   starts (S0 = 0.5), snapshots every ``gap`` steps; pairs (Q_k, Q_{k+gap});
 * training: the network's own forward / backward / Adam kernels on random
   crops of the pairs;
-* application (:func:`relax_warm`): a checked jump (kept only if neither the
-  free energy nor the explicit-Euler residual rises) followed by ``polish``
-  plain steps, repeated until ``max_rejections`` jumps in a row are refused;
-  then plain steps with the reference's energy test until |dF/F| < tol
-  between two log points ``every`` steps apart.
+* application (:func:`relax_warm`): cycles of a checked learned step (kept
+  only if neither the free energy nor the explicit-Euler residual rises; a
+  ladder of step fractions is tried in turn) and plain steps, with the
+  reference's energy test |dF/F| < tol between two log points ``every`` steps
+  apart that no jump separates.
 
 Both relaxations report wall time on the same context, the final energy and
 the mid-plane defect count, so the bench can state what the network gains.
@@ -57,12 +57,36 @@ def training_set(n_traj: int = 256, seed: int = 2025, T: float | None = None, W:
     return out
 
 
+def snapshot_pairs(n_snap: int, stride: int, gap: int):
+    """Snapshot steps and the index pairs (m, m') of the pairs (Q_k, Q_{k+gap}), k = 0, stride,
+    ... (n_snap input times): returns (steps, input flags, pairs).  Where gap is a multiple of
+    stride, the inputs are their own targets' inputs (n_snap steps in all)."""
+    ks = [m * stride for m in range(n_snap)]
+    if gap % stride == 0:
+        d = gap // stride
+        ks = ks[:max(n_snap, d + 1)]
+        steps = ks
+        pairs = [(m, m + d) for m in range(n_snap - d)]
+        inputs = [m < n_snap - d for m in range(len(steps))]
+        return steps, inputs, pairs
+    if gap > stride:
+        raise ValueError("gap must be a multiple of the stride or shorter than it")
+    steps = sorted(set(ks) | {k + gap for k in ks})
+    idx = {t: i for i, t in enumerate(steps)}
+    pairs = [(idx[k], idx[k + gap]) for k in ks]
+    inputs = [t in set(ks) for t in steps]
+    return steps, inputs, pairs
+
+
 def train(net: q.Surrogate, trajectories, gap: int = 500, n_snap: int = 7, group: int = 32,
           iters_per_group: int = 1000, lr: float = 3e-4, batch: int = 8, crop: int = 32, weight_floor: float = 1.0,
-          log=None) -> dict:
+          stride: int | None = None, increment: bool = False, log=None) -> dict:
     """Generate (Q_k, Q_{k+gap}) pairs with the plain solver, ``group`` trajectories per batched
-    context, and train on each group's pairs (the dataset is cleared between groups, so device
-    memory holds one group's snapshots).  Returns timings and the loss history."""
+    context (snapshots every ``stride`` steps, default ``gap``; with ``increment`` each also records
+    one explicit step's increment, the network's input under increment input), and train on each
+    group's pairs (the dataset is cleared between groups, so device memory holds one group's
+    snapshots).  Returns timings and the loss history."""
+    stride = stride or gap
     t_gen = t_train = 0.0
     hist = []
     n_pairs = 0
@@ -72,12 +96,14 @@ def train(net: q.Surrogate, trajectories, gap: int = 500, n_snap: int = 7, group
         with q.Solver([p for p, _, _ in chunk]) as s:
             for b, (p, mode, seed) in enumerate(chunk):
                 s.upload(q.init_field(p, mode, 0.5, 0.0, seed), box=b)
-            ids = []
-            for k in range(n_snap):
-                if k:
-                    s.step(gap)
-                ids.append([net.dataset_add(s, box=b) for b in range(len(chunk))])
-        pairs = [(ids[k][b], ids[k + 1][b]) for b in range(len(chunk)) for k in range(n_snap - 1)]
+            steps, inputs, idx_pairs = snapshot_pairs(n_snap, stride, gap)
+            ids, t = [], 0
+            for t_next, is_input in zip(steps, inputs):
+                if t_next > t:
+                    s.step(t_next - t)
+                    t = t_next
+                ids.append([net.dataset_add(s, box=b, increment=increment and is_input) for b in range(len(chunk))])
+        pairs = [(ids[m][b], ids[m2][b]) for b in range(len(chunk)) for m, m2 in idx_pairs]
         tb = time.perf_counter()
         h = net.train(pairs, iters=iters_per_group, batch=batch, crop=crop, lr=lr, seed=g0,
                       weight_floor=weight_floor)
@@ -118,34 +144,47 @@ def relax_plain(s: q.Solver, tol: float = 1e-6, every: int = 100, max_steps: int
 
 
 def relax_warm(s: q.Solver, net: q.Surrogate, tol: float = 1e-6, every: int = 100, max_steps: int = 200000,
-               polish: int = 100, max_rejections: int = 2) -> dict:
-    """Checked learned jumps, each followed by ``polish`` plain steps, until ``max_rejections``
-    consecutive jumps are refused; then the plain loop of :func:`relax_plain` decides convergence
-    (no convergence test spans a jump).  ``steps`` counts plain steps only."""
-    n, acc, rej_total, rej, net_ms = 0, 0, 0, 0, 0.0
+               scales=(1.0, 0.5, 0.25)) -> dict:
+    """Cycles of one checked learned step (the ladder ``scales``: the first proposal that
+    raises neither F nor the residual is kept, else the field is left as it was) and 2 x ``every``
+    plain steps with the energy checked after each half; converged when |dF/F| < tol between
+    the two checks of a cycle, which no jump separates (the energy part of QSR.cu:3322-3335).
+    ``steps`` counts plain steps only."""
+    n, att, acc, net_ms = 0, 0, 0, 0.0
+    kept = []
+    conv, rel = False, math.nan
     t0 = time.perf_counter()
     while n < max_steps:
-        r = net.apply(s, q.SURR_CHECKED)
+        r = net.apply(s, q.SURR_CHECKED, scales=list(scales))
+        att += 1
         net_ms += r["ms_network"]
         if r["accepted"]:
             acc += 1
-            rej = 0
-        else:
-            rej += 1
-            rej_total += 1
-        s.step(polish)
-        n += polish
-        if rej >= max_rejections:
+            kept.append(r["scale"])
+        s.step(every)
+        Fa = _energy(s)
+        s.step(every)
+        Fb = _energy(s)
+        n += 2 * every
+        rel = abs((Fb - Fa) / Fa)
+        if rel < tol:
+            conv = True
             break
-    t_jumps = time.perf_counter() - t0
-    rest = relax_plain(s, tol, every, max_steps - n)
-    return {"steps": n + rest["steps"], "seconds": t_jumps + rest["seconds"], "converged": rest["converged"],
-            "last_rel_dF": rest["last_rel_dF"], "jumps_accepted": acc, "jumps_rejected": rej_total,
-            "network_ms": net_ms, "jump_phase_s": t_jumps}
+    return {"steps": n, "seconds": time.perf_counter() - t0, "converged": conv, "last_rel_dF": rel,
+            "jumps_accepted": acc, "jumps_rejected": att - acc, "scales_kept": kept, "network_ms": net_ms}
+
+
+def zero_output_layer(net: q.Surrogate):
+    """Zero the last layer's weights and bias (W3, b3: the tail of the flat parameter array), so
+    that a modulating network starts as the plain step extrapolated and training moves it from
+    there."""
+    p = net.params
+    p[-(27 * 32 * 5 + 5):] = 0.0
+    net.params = p
 
 
 def compare(p: q.Params, net: q.Surrogate, init_mode: int, seed: int = 12345, tol: float = 1e-6, every: int = 100,
-            max_steps: int = 200000, polish: int = 100, max_rejections: int = 2, log=None) -> dict:
+            max_steps: int = 200000, scales=(1.0, 0.5, 0.25), log=None) -> dict:
     """Plain relaxation and warm-started relaxation of the same start on one context."""
     Q0 = q.init_field(p, init_mode, 0.5, 0.0, seed)
     out = {}
@@ -158,7 +197,7 @@ def compare(p: q.Params, net: q.Surrogate, init_mode: int, seed: int = 12345, to
             if name == "plain":
                 r = relax_plain(s, tol, every, max_steps)
             else:
-                r = relax_warm(s, net, tol, every, max_steps, polish, max_rejections)
+                r = relax_warm(s, net, tol, every, max_steps, scales)
             o = s.observables()
             r.update({"final_F": o["total"], "defects": int(o["defects"]), "plaquettes": int(o["plaquettes"])})
             out[name] = r

@@ -16,6 +16,12 @@ include/qsr_amd.h and lcdrsimulator_amd/csrc/qsr_surr.hpp:
     gelu(z) = z/2 (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3)))
     loss = sum_v m_v sum_c (y - target)^2 / (5 V)
 
+Under increment and gain input the network's output modulates a given field g
+(the normalised increment of one explicit step): the predicted change is
+p = g (1 + y), and
+
+    loss = sum_v m_v sum_c (g (1 + y) - target)^2 / (5 V),  dL/dy = 2 m (p - target) g / (5 V)
+
 With ``emulate=True`` the values the GPU keeps in BF16 are rounded the same way
 (round to nearest even): x, the weights, a1, a2 and the three output gradients
 dy, dz2, dz1.  Everything else is FP64.
@@ -123,16 +129,23 @@ def forward(params, x, emulate=True):
     return dict(x=xb, W1=W1, W2=W2, W3=W3, z1=z1, a1=a1, z2=z2, a2=a2, y=y)
 
 
-def loss_and_grad(params, x, target, mask=None, emulate=True):
-    """loss = sum m |y - target|^2 / (5 V) and its gradient in the flat layout."""
+def gain_prediction(g, y):
+    """Modulated prediction g (1 + y)."""
+    return np.asarray(g, dtype=np.float64) * (1.0 + y)
+
+
+def loss_and_grad(params, x, target, mask=None, emulate=True, mod=None):
+    """loss = sum m |y - target|^2 / (5 V) (or, with a modulator ``mod`` = g, of g (1 + y) - target)
+    and its gradient in the flat layout."""
     f = forward(params, x, emulate)
     r = bf16 if emulate else (lambda v: v)
     V = np.prod(x.shape[:4])
     m = np.ones(x.shape[:4]) if mask is None else np.asarray(mask, dtype=np.float64)
-    d = f["y"] - target
+    g = None if mod is None else np.asarray(mod, dtype=np.float64)
+    d = (f["y"] if g is None else gain_prediction(g, f["y"])) - target
     loss = float(np.sum(m[..., None] * d * d) / (5.0 * V))
     scale = np.float32(2.0 / (5.0 * V)) if emulate else 2.0 / (5.0 * V)
-    dy = r(np.float64(scale) * m[..., None] * d)
+    dy = r(np.float64(scale) * m[..., None] * d * (1.0 if g is None else g))
     dW3, db3 = conv_weight_grad(f["a2"], dy)
     dz2 = r(conv_input_grad(dy, f["W3"]) * gelu_grad(f["z2"]))
     dW2, db2 = conv_weight_grad(f["a1"], dz2)

@@ -238,22 +238,200 @@ def test_trained_jump_gains_over_plain_steps(qsr):
 
 
 def test_warm_start_driver_relaxes_and_reports(qsr):
-    """The warm-start protocol end to end on a small droplet: both relaxations reach
-    |dF/F| < 1e-6; the jump phase ends after two consecutive rejections; the plain loop is
-    deterministic; the warm-started result stays at the plain run's energy to 1e-3."""
+    """The warm-start protocol end to end on a small droplet with the zero modulating network
+    (the plain step extrapolated): both relaxations reach |dF/F| < 1e-6, the warm-started one
+    ends at the plain run's energy to 1e-5 with the same defect count, and both loops are
+    deterministic."""
     from lcdrsimulator_amd import warmstart as ws
 
-    with qsr.Surrogate(seed=3) as net:
-        trajs = [(ws.droplet_params(R, 300.0, 1e-4), 1, 0) for R in (14.0, 15.0, 17.0, 18.0)]
-        ws.train(net, trajs, gap=500, n_snap=3, group=4, iters_per_group=600, lr=1e-3, batch=4, crop=32)
+    with qsr.Surrogate(seed=None) as net:
+        net.params = np.zeros(S.NPARAM, dtype=np.float32)
+        net.set_input(qsr.SURR_INPUT_INCREMENT, 500.0)
         p = ws.droplet_params(16.0, 300.0, 1e-4)
         r = ws.compare(p, net, 1, max_steps=40000)
         r2 = ws.compare(p, net, 1, max_steps=40000)
     pl, wa = r["plain"], r["warm"]
     print(f"  plain {pl['steps']} steps {pl['seconds']:.3f} s; warm {wa['steps']} steps {wa['seconds']:.3f} s, "
-          f"{wa['jumps_accepted']} accepted, {wa['jumps_rejected']} rejected; rel dF {r['rel_dF_final']:.2e}, "
-          f"defects {pl['defects']} / {wa['defects']}")
+          f"{wa['jumps_accepted']} accepted, {wa['jumps_rejected']} rejected; F {pl['final_F']:.9e} / "
+          f"{wa['final_F']:.9e} (rel {r['rel_dF_final']:.2e}), defects {pl['defects']} / {wa['defects']}")
     assert pl["converged"] and wa["converged"]
-    assert wa["jumps_rejected"] >= 2 and wa["jumps_accepted"] >= 1
-    assert r["rel_dF_final"] < 1e-3 and r["same_defects"] in (True, False)
-    assert r2["plain"]["steps"] == pl["steps"] and r2["plain"]["final_F"] == pl["final_F"]
+    assert wa["jumps_accepted"] >= 1
+    assert r["rel_dF_final"] <= 1e-4 and r["same_defects"]
+    for k in ("steps", "final_F"):
+        assert r2["plain"][k] == pl[k] and r2["warm"][k] == wa[k]
+
+
+# ---------------------------------------------------------------------------
+# Increment input: the network reads x = u / s (u = one explicit step's
+# increment, s its RMS per component) and proposes Q + N u (1 + y).
+# ---------------------------------------------------------------------------
+HORIZON = 500.0
+
+
+def _increment(Q, Q1):
+    """u = Q1 - Q in FP32 on the voxels a step changed (box faces excluded), its RMS s."""
+    step = np.any(Q1 != Q, axis=-1)
+    step[0, :, :] = step[-1, :, :] = step[:, 0, :] = step[:, -1, :] = step[:, :, 0] = step[:, :, -1] = False
+    u = np.where(step[..., None], Q1.astype(np.float32) - Q.astype(np.float32), np.float32(0))
+    s = math.sqrt(float((u.astype(np.float64) ** 2).sum()) / (5.0 * step.sum()))
+    return u, s, step
+
+
+@pytest.mark.parametrize("kind", ["increment", "gain"])
+def test_modulated_apply_force_writes_q_plus_modulated_increment(qsr, kind):
+    """dQ = N u (1 + y) on the voxels a step changes, y the network's output for u / s
+    (increment input) or for Q (gain input)."""
+    p = _droplet(qsr)
+    Q0 = qsr.init_field(p, 0, 0.5, seed=3).astype(np.float32).astype(np.float64)
+    with qsr.Solver(p) as s, qsr.Surrogate(seed=9) as net:
+        net.set_input(qsr.SURR_INPUT_INCREMENT if kind == "increment" else qsr.SURR_INPUT_GAIN, HORIZON)
+        s.upload(Q0)
+        s.step(10)
+        Q = s.download()
+        s.step(1)
+        Q1 = s.download()
+        s.upload(Q)
+        r = net.apply(s, qsr.SURR_FORCE)
+        Qn = s.download()
+        u, srms, step = _increment(Q, Q1)
+        x = (u * np.float32(1.0 / srms)) if kind == "increment" else Q.astype(np.float32)
+        _, y, _ = net.eval(x[None])
+    dq = np.float32(HORIZON) * u * (np.float32(1) + y[0])
+    expect = np.where(step[..., None], (Q.astype(np.float32) + dq).astype(np.float64), Q)
+    err = np.abs(Qn - expect).max()
+    print(f"  {kind}: s = {srms:.3e}, max |dQ| {np.abs(dq).max():.3e}, max err {err:.3e}")
+    assert r["accepted"] == 1 and r["tried"] == 1 and r["scale"] == 1.0
+    # the residual the library reports is |u|_2 over the same voxels
+    assert abs(r["res_before"] - srms * math.sqrt(5.0 * step.sum())) <= 1e-6 * r["res_before"]
+    assert err <= FWD_TOL * np.abs(dq).max() + 1e-6
+    np.testing.assert_array_equal(Qn[~step], Q[~step])
+
+
+@pytest.mark.parametrize("shape", [(1, 8, 8, 16), (2, 5, 6, 24)])
+def test_modulated_loss_and_gradient_match_oracle(qsr, shape):
+    """Increment and gain input: the prediction is g (1 + y) and the loss is taken on it."""
+    p = S.random_params(12, scale=2.0).astype(np.float32)
+    x, t, m = _batch(5, shape)
+    g5 = np.random.default_rng(6).normal(scale=0.5, size=x.shape).astype(np.float32)
+    with qsr.Surrogate(seed=None) as net:
+        net.params = p
+        loss, pred, g = net.eval(x, t, m, want_pred=True, want_grad=True, modulator=g5)
+    lo, go, f = S.loss_and_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64),
+                                m.astype(np.float64), emulate=True, mod=g5)
+    ref = S.gain_prediction(g5, f["y"])
+    print(f"  loss gpu {loss:.6e} oracle {lo:.6e}; pred max err {np.abs(pred - ref).max():.3e}")
+    assert np.abs(pred - ref).max() <= FWD_TOL * np.abs(ref).max()
+    assert abs(loss - lo) <= 1e-4 * lo
+    for name, a, b in BLOCKS:
+        rel = np.linalg.norm(g[a:b] - go[a:b]) / np.linalg.norm(go[a:b])
+        assert rel <= GRAD_TOL, (name, rel)
+
+
+def test_increment_input_is_zero_for_a_field_at_rest(qsr):
+    """An isotropic periodic box (Q = 0) does not move: nothing is proposed, the field stays."""
+    p = qsr.Params(Nx=16, Ny=16, Nz=16, periodic=1, T_star=307.0, T_high=310.0, kappa=6e-12, gamma=0.1, dt=1e-10)
+    with qsr.Solver(p) as s, qsr.Surrogate(seed=9) as net:
+        net.set_input(qsr.SURR_INPUT_INCREMENT, HORIZON)
+        s.upload(np.zeros((16, 16, 16, 5)))
+        r = net.apply(s, qsr.SURR_FORCE)
+        assert r["tried"] == 0 and r["accepted"] == 0 and r["res_before"] == 0.0
+        assert r["F_after"] == r["F_before"]
+        assert not np.any(s.download())
+
+
+def test_increment_training_needs_increment_snapshots(qsr):
+    p = _droplet(qsr, n=24, R=8.0)
+    with qsr.Solver(p) as s, qsr.Surrogate(seed=1) as net:
+        s.upload(qsr.init_field(p, 1, 0.5, seed=0))
+        a = net.dataset_add(s)
+        s.step(50)
+        b = net.dataset_add(s, increment=True)
+        net.set_input(qsr.SURR_INPUT_INCREMENT, HORIZON)
+        with pytest.raises(qsr.QSRError):
+            net.train([(a, b)], iters=1, batch=1, crop=16)
+        hist = net.train([(b, a)], iters=2, batch=1, crop=16, lr=1e-4)
+        assert np.all(np.isfinite(hist))
+        with pytest.raises(qsr.QSRError):
+            net.set_input(5, HORIZON)
+        with pytest.raises(qsr.QSRError):
+            net.set_input(qsr.SURR_INPUT_INCREMENT, 0.0)
+
+
+def _trained_increment_net(qsr, net, kind, iters=1200):
+    from lcdrsimulator_amd import warmstart as ws
+    net.set_input(kind, HORIZON)
+    trajs = [(ws.droplet_params(R, 300.0, 1e-4), m, 0) for R in (14.0, 15.0, 17.0, 18.0) for m in (0, 1)]
+    return ws.train(net, trajs, gap=500, n_snap=9, stride=250, group=8, iters_per_group=iters, lr=1e-3,
+                    batch=4, crop=32, increment=True)
+
+
+@pytest.mark.parametrize("kind", ["increment", "gain"])
+def test_modulated_jump_gains_over_plain_steps_and_ladder_rescales(qsr, kind):
+    """A network trained under increment (gain) input on eight small droplets (R = 14-18, radial and
+    random-director starts), applied to an unseen radial droplet (R = 16) 300 steps into its
+    relaxation: the checked ladder keeps a proposal that carries the free energy a large part of
+    the way 500 plain steps do (the zero network, i.e. the plain step extrapolated, is printed
+    beside it); the kept proposal equals Q + scale dQ of a forced proposal at scale 1 (the
+    ladder rescales the first)."""
+    from lcdrsimulator_amd import warmstart as ws
+
+    def F(s):
+        return s.observables(want_defects=False, want_xi=False, want_order=False)["total"]
+
+    ladder = [8.0, 4.0, 2.0, 1.0, 0.5]
+    with qsr.Surrogate(seed=3) as net, qsr.Surrogate(seed=None) as raw:
+        k = qsr.SURR_INPUT_INCREMENT if kind == "increment" else qsr.SURR_INPUT_GAIN
+        info = _trained_increment_net(qsr, net, k)
+        raw.params = np.zeros(S.NPARAM, dtype=np.float32)
+        raw.set_input(k, HORIZON)
+        loss = info["loss"]
+        print(f"  {info['pairs']} pairs, training loss {loss[:50].mean():.3e} -> {loss[-50:].mean():.3e}")
+        p = ws.droplet_params(16.0, 300.0, 1e-4)
+        with qsr.Solver(p) as s:
+            s.upload(qsr.init_field(p, 1, 0.5, 0.0, 0))
+            s.step(300)
+            Q = s.download()
+            Fa = F(s)
+            s.step(500)
+            Fb = F(s)
+            gains, kept = {}, {}
+            for name, n in (("trained", net), ("zero network", raw)):
+                s.upload(Q)
+                r = n.apply(s, qsr.SURR_CHECKED, scales=ladder)
+                gains[name] = (Fa - F(s)) / (Fa - Fb)
+                kept[name] = (r, s.download())
+                print(f"  {name}: accepted {r['accepted']} at scale {r['scale']} after {r['tried']} tries, "
+                      f"res {r['res_before']:.3e} -> {r['res_after']:.3e}, energy gain {gains[name]:.3f} of 500 steps")
+            r, Qk = kept["trained"]
+            s.upload(Q)
+            net.apply(s, qsr.SURR_FORCE)
+            dQ = s.download() - Q
+    assert loss[-50:].mean() < 0.8 * loss[:50].mean()
+    assert r["accepted"] == 1 and r["tried"] >= 2, "the ladder must start above the kept scale"
+    np.testing.assert_allclose(Qk, Q + r["scale"] * dQ, rtol=0, atol=2e-6)
+    assert gains["trained"] > 0.5
+
+
+def test_trained_warm_start_reaches_the_plain_energy_sooner(qsr):
+    """What the trained network gains over the method without it, end to end: a network trained
+    under increment input on eight small droplets relaxes an unseen radial droplet (R = 16) to
+    |dF/F| < 1e-6 in fewer plain steps than the plain solver, at the plain run's energy (1e-5)
+    and with the same defect count.  The zero network (the plain step extrapolated) is printed
+    beside it."""
+    from lcdrsimulator_amd import warmstart as ws
+
+    with qsr.Surrogate(seed=3) as net, qsr.Surrogate(seed=None) as zero:
+        ws.zero_output_layer(net)
+        _trained_increment_net(qsr, net, qsr.SURR_INPUT_INCREMENT)
+        zero.params = np.zeros(S.NPARAM, dtype=np.float32)
+        zero.set_input(qsr.SURR_INPUT_INCREMENT, HORIZON)
+        p = ws.droplet_params(16.0, 300.0, 1e-4)
+        r = ws.compare(p, net, 1, max_steps=40000)
+        rz = ws.compare(p, zero, 1, max_steps=40000)
+    for name, x in (("trained", r), ("zero network", rz)):
+        print(f"  {name}: plain {x['plain']['steps']} steps, warm {x['warm']['steps']} steps "
+              f"({x['warm']['jumps_accepted']} kept, {x['warm']['jumps_rejected']} refused), "
+              f"F {x['plain']['final_F']:.9e} / {x['warm']['final_F']:.9e} (rel {x['rel_dF_final']:.2e}), "
+              f"defects {x['plain']['defects']} / {x['warm']['defects']}")
+    assert r["warm"]["converged"] and r["warm"]["steps"] < r["plain"]["steps"]
+    assert r["rel_dF_final"] <= 1e-4 and r["same_defects"]

@@ -52,3 +52,17 @@ def test_training_set_matched_fixes_temperature_and_anchoring(ws):
     # same seed -> same draws as the unmatched set (radii agree)
     u = ws.training_set(16)
     assert [p.radius_index for p, _, _ in t] == [p.radius_index for p, _, _ in u]
+
+
+def test_snapshot_pairs_cover_every_gap_apart_pair():
+    from lcdrsimulator_amd import warmstart as ws
+    for n, stride, gap in ((7, 500, 500), (13, 250, 500), (2, 250, 500), (5, 250, 50), (4, 100, 100), (3, 7, 5)):
+        steps, inputs, pairs = ws.snapshot_pairs(n, stride, gap)
+        assert steps == sorted(set(steps)) and len(inputs) == len(steps)
+        want = [(k, k + gap) for k in range(0, n * stride, stride) if gap % stride or k + gap < n * stride]
+        assert [(steps[a], steps[b]) for a, b in pairs] == want
+        assert all(inputs[a] for a, _ in pairs)
+    assert ws.snapshot_pairs(7, 500, 500)[2] == [(m, m + 1) for m in range(6)]
+    assert ws.snapshot_pairs(2, 250, 500)[2] == []
+    with pytest.raises(ValueError):
+        ws.snapshot_pairs(7, 300, 500)
