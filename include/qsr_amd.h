@@ -283,6 +283,12 @@ int qsr_surr_apply(void* net, void* ctx, int mode, qsr_surr_result* out);
  * QSR_SURR_CHECKED keeps the first that passes the check (one network pass;
  * later proposals rescale the first), FORCE and PROPOSE use scales[0]. */
 int qsr_surr_apply_ladder(void* net, void* ctx, int mode, const double* scales, int nscales, qsr_surr_result* out);
+/* Batched contexts: the same for each listed box (uploaded, not held, listed
+ * once), one network pass per box; each box is kept or restored on its own by
+ * its own energy and residual, and unlisted boxes are untouched.  out[i] is
+ * the result of boxes[i]. */
+int qsr_surr_apply_boxes(void* net, void* ctx, int mode, const double* scales, int nscales, const int* boxes,
+                         int nboxes, qsr_surr_result* out);
 
 #ifdef __cplusplus
 }

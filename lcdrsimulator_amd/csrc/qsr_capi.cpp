@@ -1058,6 +1058,7 @@ int field_view(void* ctx, int box, FieldView* out) {
     v.rows = c->d_step_rows;
     v.nboxes = int(c->boxes.size());
     v.uploaded = c->boxes[box].uploaded;
+    v.active = c->boxes[box].active;
     *out = v;
     return QSR_OK;
 }
@@ -1070,9 +1071,39 @@ int commit_other(void* ctx) {
     return QSR_OK;
 }
 
-int euler_residual(void* ctx, int box, double* res, long long* count) {
+int swap_buffers(void* ctx) {
     Ctx* c = as_ctx(ctx);
-    if (!c || !res || box < 0 || box >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad context or box");
+    if (!c) return fail(QSR_E_ARG, "bad context");
+    c->cur = 1 - c->cur;
+    return QSR_OK;
+}
+
+int copy_current_to_other(void* ctx) {
+    Ctx* c = as_ctx(ctx);
+    if (!c) return fail(QSR_E_ARG, "bad context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMemcpyAsync(c->q[1 - c->cur], c->q[c->cur], sizeof(float) * 5 * c->cs, hipMemcpyDeviceToDevice,
+                           c->st));
+    return QSR_OK;
+}
+
+int copy_box_other_to_current(void* ctx, int box) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || box < 0 || box >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad context or box");
+    HIP_TRY(hipSetDevice(c->device));
+    const qsrk::BoxDev& B = c->h_boxes[box];
+    const long long n = c->boxes[box].nvox;
+    for (int comp = 0; comp < 5; ++comp) {
+        const long long o = (long long)comp * c->cs + B.off;
+        HIP_TRY(hipMemcpyAsync(c->q[c->cur] + o, c->q[1 - c->cur] + o, sizeof(float) * n, hipMemcpyDeviceToDevice,
+                               c->st));
+    }
+    return QSR_OK;
+}
+
+int euler_rhs(void* ctx) {
+    Ctx* c = as_ctx(ctx);
+    if (!c) return fail(QSR_E_ARG, "bad context");
     if (int rc = check_uploaded(*c)) return rc;
     HIP_TRY(hipSetDevice(c->device));
     const int nb = int(c->boxes.size());
@@ -1087,6 +1118,14 @@ int euler_residual(void* ctx, int box, double* res, long long* count) {
     if (c->ntiles > 0)
         HIP_TRY(qsrk::launch_step(c->elastic, c->variant, false, c->ntiles, c->st, c->q[c->cur], c->q_rhs, c->cs,
                                   c->d_boxes, c->d_tiles, c->d_dyn, nb, c->d_step_rows, c->ch, c->d_rhs0));
+    return QSR_OK;
+}
+
+int box_residual(void* ctx, int box, double* res, long long* count) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !res || box < 0 || box >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad context or box");
+    if (!c->q_rhs) return fail(QSR_E_STATE, "no explicit step computed");
+    HIP_TRY(hipSetDevice(c->device));
     if (int rc = ensure_partials(*c, 4096)) return rc;
     long long nblk = 0;
     HIP_TRY(qsrs::launch_residual(c->st, c->q[c->cur], c->q_rhs, c->cs, c->h_boxes[box], c->d_step_rows, c->d_part,
@@ -1098,6 +1137,13 @@ int euler_residual(void* ctx, int box, double* res, long long* count) {
     *res = std::sqrt(r[0]);
     if (count) *count = (long long)r[1];
     return QSR_OK;
+}
+
+int euler_residual(void* ctx, int box, double* res, long long* count) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !res || box < 0 || box >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad context or box");
+    if (int rc = euler_rhs(ctx)) return rc;
+    return box_residual(ctx, box, res, count);
 }
 
 }  // namespace qsrc

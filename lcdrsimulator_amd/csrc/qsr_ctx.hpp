@@ -22,6 +22,7 @@ struct FieldView {
     const qsrk::RowMask* rows = nullptr;  // step row masks (packed over all boxes)
     int nboxes = 0;
     bool uploaded = false;
+    bool active = true;  // held boxes are not stepped (their step buffer is not written)
 };
 
 // Fails with QSR_E_ARG for a bad context or box index.
@@ -36,6 +37,16 @@ int commit_other(void* ctx);
 // Q' stays in FieldView::rhs until the next step; count (may be null)
 // receives the number of voxels summed over.
 int euler_residual(void* ctx, int box, double* res, long long* count = nullptr);
+// The two halves of euler_residual: one explicit step of every active box into
+// FieldView::rhs, then the residual of one box from it.
+int euler_rhs(void* ctx);
+int box_residual(void* ctx, int box, double* res, long long* count = nullptr);
+// Batched contexts: swap the ping-pong buffers of every box (the caller makes
+// sure both hold what it wants for every box); copy the current field into
+// the other buffer; copy one box's region of the other buffer into the current.
+int swap_buffers(void* ctx);
+int copy_current_to_other(void* ctx);
+int copy_box_other_to_current(void* ctx, int box);
 // Sets the calling thread's qsr_last_error() message and returns code.
 int set_error(int code, const std::string& msg);
 

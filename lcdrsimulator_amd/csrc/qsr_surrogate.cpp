@@ -254,6 +254,60 @@ int check_net(Net* n) {
     return QSR_OK;
 }
 
+// Network pass over one box of a context, on the context's stream: the proposal
+// q_old + out_scale * (y, or u (1 + y) under increment and gain input) into
+// v.other on the voxels a step can change, from v.cur (and v.rhs = E(v.cur)).
+int propose_box(Net& n, const qsrc::FieldView& v, double s_inc, float out_scale, double* ms_out) {
+    const bool incr = n.input_kind != QSR_SURR_INPUT_FIELD;
+    const long long V = v.nvox;
+    if (V > n.a_cap) {
+        for (uint16_t** p : {&n.ax, &n.aa1, &n.aa2}) {
+            discard(hipFree(*p));
+            *p = nullptr;
+        }
+        n.a_cap = 0;
+        HIP_TRY(hipMalloc(&n.ax, sizeof(uint16_t) * 8 * V));
+        HIP_TRY(hipMalloc(&n.aa1, sizeof(uint16_t) * 32 * V));
+        HIP_TRY(hipMalloc(&n.aa2, sizeof(uint16_t) * 32 * V));
+        n.a_cap = V;
+    }
+    const hipStream_t st = v.st;
+    ConvArgs a = conv_dims(1, v.box.Nx, v.box.Ny, v.box.Nz);
+    HIP_TRY(hipEventRecord(n.ev[0], st));
+    if (n.input_kind == QSR_SURR_INPUT_INCREMENT)
+        HIP_TRY(launch_pack_increment(st, v.cur, v.rhs, v.cs, v.box, v.rows, float(1.0 / s_inc), n.ax));
+    else
+        HIP_TRY(launch_pack_field(st, v.cur, v.cs, v.box.off, V, n.ax));
+    a.in = n.ax;
+    a.frag = n.fr.f1;
+    a.bias = n.fr.b1;
+    a.a_out = n.aa1;
+    HIP_TRY(launch_conv(8, 2, EPI_HIDDEN, st, a));
+    a.in = n.aa1;
+    a.frag = n.fr.f2;
+    a.bias = n.fr.b2;
+    a.a_out = n.aa2;
+    HIP_TRY(launch_conv(32, 2, EPI_HIDDEN, st, a));
+    a.in = n.aa2;
+    a.frag = n.fr.f3;
+    a.bias = n.fr.b3;
+    a.a_out = nullptr;
+    a.q_old = v.cur;
+    a.q_new = v.other;
+    a.out_scale = out_scale;
+    a.rhs = incr ? v.rhs : nullptr;
+    a.cs = v.cs;
+    a.box = v.box;
+    a.rows = v.rows;
+    HIP_TRY(launch_conv(32, 1, EPI_APPLY, st, a));
+    HIP_TRY(hipEventRecord(n.ev[1], st));
+    HIP_TRY(hipEventSynchronize(n.ev[1]));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, n.ev[0], n.ev[1]));
+    *ms_out = ms;
+    return QSR_OK;
+}
+
 int upload_params(Net& n, const float* host) {
     HIP_TRY(hipSetDevice(n.device));
     HIP_TRY(hipMemcpyAsync(n.P, host, sizeof(float) * NPARAM, hipMemcpyHostToDevice, n.st));
@@ -662,8 +716,9 @@ int qsr_surr_apply_ladder(void* net, void* ctx, int mode, const double* scales, 
     if (mode != QSR_SURR_CHECKED) nscales = 1;
     qsrc::FieldView v;
     if (int rc = qsrc::field_view(ctx, 0, &v)) return rc;
-    if (v.nboxes != 1) return fail(QSR_E_ARG, "the learned step applies to single-box contexts");
+    if (v.nboxes != 1) return fail(QSR_E_ARG, "a batched context takes qsr_surr_apply_boxes");
     if (!v.uploaded) return fail(QSR_E_STATE, "box has no field");
+    if (!v.active) return fail(QSR_E_STATE, "box is held");
     if (v.device != n->device) return fail(QSR_E_ARG, "network and context are on different devices");
     HIP_TRY(hipSetDevice(n->device));
     HIP_TRY(hipStreamSynchronize(n->st));  // parameters last written on the network's stream
@@ -687,53 +742,8 @@ int qsr_surr_apply_ladder(void* net, void* ctx, int mode, const double* scales, 
         return QSR_OK;
     }
 
-    const long long V = v.nvox;
-    if (V > n->a_cap) {
-        for (uint16_t** p : {&n->ax, &n->aa1, &n->aa2}) {
-            discard(hipFree(*p));
-            *p = nullptr;
-        }
-        n->a_cap = 0;
-        HIP_TRY(hipMalloc(&n->ax, sizeof(uint16_t) * 8 * V));
-        HIP_TRY(hipMalloc(&n->aa1, sizeof(uint16_t) * 32 * V));
-        HIP_TRY(hipMalloc(&n->aa2, sizeof(uint16_t) * 32 * V));
-        n->a_cap = V;
-    }
-    // the network runs on the context's stream, after its steps
+    if (int rc = propose_box(*n, v, s_inc, float(scales[0] * (incr ? n->horizon : 1.0)), &r.ms_network)) return rc;
     const hipStream_t st = v.st;
-    ConvArgs a = conv_dims(1, v.box.Nx, v.box.Ny, v.box.Nz);
-    HIP_TRY(hipEventRecord(n->ev[0], st));
-    if (n->input_kind == QSR_SURR_INPUT_INCREMENT)
-        HIP_TRY(launch_pack_increment(st, v.cur, v.rhs, v.cs, v.box, v.rows, float(1.0 / s_inc), n->ax));
-    else
-        HIP_TRY(launch_pack_field(st, v.cur, v.cs, v.box.off, V, n->ax));
-    a.in = n->ax;
-    a.frag = n->fr.f1;
-    a.bias = n->fr.b1;
-    a.a_out = n->aa1;
-    HIP_TRY(launch_conv(8, 2, EPI_HIDDEN, st, a));
-    a.in = n->aa1;
-    a.frag = n->fr.f2;
-    a.bias = n->fr.b2;
-    a.a_out = n->aa2;
-    HIP_TRY(launch_conv(32, 2, EPI_HIDDEN, st, a));
-    a.in = n->aa2;
-    a.frag = n->fr.f3;
-    a.bias = n->fr.b3;
-    a.a_out = nullptr;
-    a.q_old = v.cur;
-    a.q_new = v.other;
-    a.out_scale = float(scales[0] * (incr ? n->horizon : 1.0));
-    a.rhs = incr ? v.rhs : nullptr;
-    a.cs = v.cs;
-    a.box = v.box;
-    a.rows = v.rows;
-    HIP_TRY(launch_conv(32, 1, EPI_APPLY, st, a));
-    HIP_TRY(hipEventRecord(n->ev[1], st));
-    HIP_TRY(hipEventSynchronize(n->ev[1]));
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, n->ev[0], n->ev[1]));
-    r.ms_network = ms;
 
     // proposal i lies in the other buffer; the previous field stays in the current one
     bool keep = false;
@@ -765,6 +775,93 @@ int qsr_surr_apply_ladder(void* net, void* ctx, int mode, const double* scales, 
 int qsr_surr_apply(void* net, void* ctx, int mode, qsr_surr_result* out) {
     const double one = 1.0;
     return qsr_surr_apply_ladder(net, ctx, mode, &one, 1, out);
+}
+
+int qsr_surr_apply_boxes(void* net, void* ctx, int mode, const double* scales, int nscales, const int* boxes,
+                         int nsel, qsr_surr_result* out) {
+    Net* n = as_net(net);
+    if (int rc = check_net(n)) return rc;
+    if (!out || !boxes || nsel < 1 || mode < QSR_SURR_CHECKED || mode > QSR_SURR_PROPOSE || !scales || nscales < 1)
+        return fail(QSR_E_ARG, "bad argument");
+    for (int i = 0; i < nscales; ++i)
+        if (!(scales[i] > 0.0) || !std::isfinite(scales[i])) return fail(QSR_E_ARG, "scales must be positive");
+    if (mode != QSR_SURR_CHECKED) nscales = 1;
+    qsrc::FieldView v;
+    if (int rc = qsrc::field_view(ctx, 0, &v)) return rc;
+    if (v.device != n->device) return fail(QSR_E_ARG, "network and context are on different devices");
+    std::vector<char> seen(static_cast<size_t>(v.nboxes), 0);
+    for (int i = 0; i < nsel; ++i) {
+        const int b = boxes[i];
+        if (b < 0 || b >= v.nboxes) return fail(QSR_E_ARG, "box " + std::to_string(b) + " out of range");
+        if (seen[b]) return fail(QSR_E_ARG, "box " + std::to_string(b) + " listed twice");
+        seen[b] = 1;
+        if (int rc = qsrc::field_view(ctx, b, &v)) return rc;
+        if (!v.uploaded) return fail(QSR_E_STATE, "box " + std::to_string(b) + " has no field");
+        if (!v.active) return fail(QSR_E_STATE, "box " + std::to_string(b) + " is held");
+    }
+    HIP_TRY(hipSetDevice(n->device));
+    HIP_TRY(hipStreamSynchronize(n->st));  // parameters last written on the network's stream
+    const bool incr = n->input_kind != QSR_SURR_INPUT_FIELD;
+    const qsr_obs_opts oo = {-1, 0.1, 0.25, -1, -1.0, 0, 0, 1};  // energies only
+    qsr_obs o{};
+    std::vector<qsr_surr_result> R(static_cast<size_t>(nsel));
+    std::vector<double> s_inc(static_cast<size_t>(nsel), 0.0);
+    // energies and residuals before, from one explicit step of every active box
+    if (int rc = qsrc::euler_rhs(ctx)) return rc;
+    for (int i = 0; i < nsel; ++i) {
+        if (int rc = qsr_observables(ctx, boxes[i], &oo, &o)) return rc;
+        R[i].F_before = o.total;
+        long long cnt = 0;
+        if (int rc = qsrc::box_residual(ctx, boxes[i], &R[i].res_before, &cnt)) return rc;
+        s_inc[i] = cnt > 0 ? R[i].res_before / std::sqrt(5.0 * double(cnt)) : 0.0;
+        R[i].F_after = R[i].F_before;
+        R[i].res_after = R[i].res_before;
+    }
+    // proposals into the other buffer, which holds a copy of the current field elsewhere
+    if (int rc = qsrc::copy_current_to_other(ctx)) return rc;
+    std::vector<int> pending;
+    for (int i = 0; i < nsel; ++i) {
+        if (incr && !(s_inc[i] > 0.0 && std::isfinite(s_inc[i]))) continue;  // nothing to propose
+        if (int rc = qsrc::field_view(ctx, boxes[i], &v)) return rc;
+        if (int rc = propose_box(*n, v, s_inc[i], float(scales[0] * (incr ? n->horizon : 1.0)), &R[i].ms_network))
+            return rc;
+        pending.push_back(i);
+    }
+    if (!pending.empty()) {
+        // current: the proposals (every other box unchanged); other: the previous field
+        if (int rc = qsrc::swap_buffers(ctx)) return rc;
+        for (int r = 0; r < nscales && !pending.empty(); ++r) {
+            if (r > 0)
+                for (int i : pending) {
+                    if (int rc = qsrc::field_view(ctx, boxes[i], &v)) return rc;
+                    // the context owns both buffers: the current one is rescaled in place
+                    HIP_TRY(launch_rescale_step(v.st, v.other, const_cast<float*>(v.cur), v.cs, v.box, v.rows,
+                                                float(scales[r] / scales[r - 1])));
+                }
+            if (int rc = qsrc::euler_rhs(ctx)) return rc;
+            std::vector<int> still;
+            for (int i : pending) {
+                R[i].scale = scales[r];
+                R[i].tried = r + 1;
+                if (int rc = qsr_observables(ctx, boxes[i], &oo, &o)) return rc;
+                R[i].F_after = o.total;
+                if (int rc = qsrc::box_residual(ctx, boxes[i], &R[i].res_after)) return rc;
+                bool keep = mode == QSR_SURR_FORCE;
+                if (mode == QSR_SURR_CHECKED)
+                    keep = std::isfinite(R[i].F_after) && std::isfinite(R[i].res_after) &&
+                           R[i].F_after <= R[i].F_before && R[i].res_after <= R[i].res_before;
+                if (keep) R[i].accepted = 1;
+                else still.push_back(i);
+            }
+            pending.swap(still);
+        }
+        for (int i : pending)  // refused at every scale: back to the previous field
+            if (int rc = qsrc::copy_box_other_to_current(ctx, boxes[i])) return rc;
+        if (int rc = qsrc::field_view(ctx, 0, &v)) return rc;
+        HIP_TRY(hipStreamSynchronize(v.st));
+    }
+    for (int i = 0; i < nsel; ++i) out[i] = R[i];
+    return QSR_OK;
 }
 
 }  // extern "C"

@@ -78,6 +78,7 @@ EXPORTS = (
     "qsr_surr_get_params", "qsr_surr_save", "qsr_surr_load", "qsr_surr_eval", "qsr_surr_adam_step",
     "qsr_surr_dataset_add", "qsr_surr_dataset_size", "qsr_surr_dataset_clear", "qsr_surr_train", "qsr_surr_apply",
     "qsr_surr_dataset_add_increment", "qsr_surr_set_input", "qsr_surr_apply_ladder", "qsr_surr_eval_gain",
+    "qsr_surr_apply_boxes",
 )
 
 SURR_CHECKED, SURR_FORCE, SURR_PROPOSE = 0, 1, 2
@@ -147,6 +148,8 @@ def load():
     L.qsr_surr_set_input.argtypes = [vp, C.c_int, C.c_double]
     L.qsr_surr_apply_ladder.argtypes = [vp, vp, C.c_int, dp, C.c_int, C.POINTER(_SurrResult)]
     L.qsr_surr_eval_gain.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, fp, fp, fp, fp, dp, fp, fp]
+    L.qsr_surr_apply_boxes.argtypes = [vp, vp, C.c_int, dp, C.c_int, C.POINTER(C.c_int), C.c_int,
+                                       C.POINTER(_SurrResult)]
     _lib = L
     return L
 
@@ -504,3 +507,13 @@ class Surrogate:
             _check(load().qsr_surr_apply_ladder(self._h, solver._h, int(mode),
                                                 sc.ctypes.data_as(C.POINTER(C.c_double)), sc.size, C.byref(r)))
         return {f: getattr(r, f) for f, _ in _SurrResult._fields_}
+
+    def apply_boxes(self, solver: "Solver", boxes, mode: int = SURR_CHECKED, scales=(1.0,)) -> list:
+        """The ladder of :meth:`apply` on each listed box of a batched solver, each kept or
+        restored on its own; one result dict per listed box."""
+        bx = np.ascontiguousarray(boxes, dtype=np.int32).ravel()
+        sc = np.ascontiguousarray(scales, dtype=np.float64).ravel()
+        res = (_SurrResult * max(bx.size, 1))()
+        _check(load().qsr_surr_apply_boxes(self._h, solver._h, int(mode), sc.ctypes.data_as(C.POINTER(C.c_double)),
+                                           sc.size, bx.ctypes.data_as(C.POINTER(C.c_int)), bx.size, res))
+        return [{f: getattr(res[i], f) for f, _ in _SurrResult._fields_} for i in range(bx.size)]

@@ -174,6 +174,63 @@ def relax_warm(s: q.Solver, net: q.Surrogate, tol: float = 1e-6, every: int = 10
             "jumps_accepted": acc, "jumps_rejected": att - acc, "scales_kept": kept, "network_ms": net_ms}
 
 
+def relax_batch(s: q.Solver, n_boxes: int, net: q.Surrogate | None = None, tol: float = 1e-6, every: int = 100,
+                max_steps: int = 200000, scales=(1.0, 0.5, 0.25), backoff: bool = True) -> dict:
+    """Relax every box of a batched solver, retiring each (qsr_set_box_active) once its energy
+    test passes.  Plain (no ``net``): the test of :func:`relax_plain` per box, checks every
+    ``every`` steps.  Warm-started: the cycles of :func:`relax_warm`, with one checked learned step
+    per active box per cycle (:meth:`Surrogate.apply_boxes`); with ``backoff`` a box whose step was
+    refused sits out the next 1, 2, 4, ... cycles (reset when one is kept).  ``steps`` per box
+    counts plain steps."""
+    active = list(range(n_boxes))
+    steps = [0] * n_boxes
+    prev = [None] * n_boxes
+    wait, gap = [0] * n_boxes, [0] * n_boxes
+    done, acc, att, net_ms = 0, 0, 0, 0.0
+    t0 = time.perf_counter()
+
+    def energy(b):
+        return s.observables(box=b, want_defects=False, want_xi=False, want_order=False)["total"]
+
+    while active and done < max_steps:
+        if net is None:
+            s.step(every)
+            done += every
+            F = {b: energy(b) for b in active}
+        else:
+            sel = [b for b in active if wait[b] == 0]
+            for b in active:
+                wait[b] = max(0, wait[b] - 1)
+            res = net.apply_boxes(s, sel, q.SURR_CHECKED, scales) if sel else []
+            for b, r in zip(sel, res):
+                if r["accepted"]:
+                    gap[b] = 0
+                elif backoff:
+                    gap[b] = max(1, 2 * gap[b])
+                    wait[b] = gap[b]
+            att += len(res)
+            acc += sum(r["accepted"] for r in res)
+            net_ms += sum(r["ms_network"] for r in res)
+            s.step(every)
+            Fa = {b: energy(b) for b in active}
+            s.step(every)
+            done += 2 * every
+            F = {b: energy(b) for b in active}
+        still = []
+        for b in active:
+            ref = prev[b] if net is None else Fa[b]
+            if ref is not None and abs((F[b] - ref) / ref) < tol:
+                steps[b] = done
+                s.set_active(b, False)
+            else:
+                still.append(b)
+            prev[b] = F[b]
+        active = still
+    return {"seconds": time.perf_counter() - t0, "steps": steps, "total_steps": done,
+            "converged": n_boxes - len(active), "jumps_accepted": acc, "jumps_rejected": att - acc,
+            "network_ms": net_ms}
+
+
 def zero_output_layer(net: q.Surrogate):
     """Zero the last layer's weights and bias (W3, b3: the tail of the flat parameter array), so
     that a modulating network starts as the plain step extrapolated and training moves it from
