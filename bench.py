@@ -290,21 +290,10 @@ def run_ladder_relax(q, args):
         for b, Q in enumerate(fields):
             s.upload(Q, box=b)
             s.set_iteration(0, box=b)
-        prev, steps_b, active, done = [None] * n, [0] * n, set(range(n)), 0
-        t0 = time.perf_counter()
-        while active and done < nmax:
-            s.step(every)
-            done += every
-            for b in sorted(active):
-                F = energy(s, b)
-                if prev[b] is not None and abs((F - prev[b]) / prev[b]) < tol:
-                    steps_b[b] = done
-                    s.set_active(b, False)
-                    active.discard(b)
-                prev[b] = F
-        t_batched = time.perf_counter() - t0
+        rb = ws.relax_batch(s, n, None, tol, every, nmax)
+        t_batched, steps_b, done, n_conv = rb["seconds"], rb["steps"], rb["total_steps"], rb["converged"]
         final_b = [s.observables(box=b, want_xi=False) for b in range(n)]
-    log(f"ladder relaxation batched: {t_batched:.3f} s, {done} steps, {n - len(active)} of {n} converged")
+    log(f"ladder relaxation batched: {t_batched:.3f} s, {done} steps, {n_conv} of {n} converged")
     # batched and warm-started: the trained network and the zero network
     warm = {}
     with q.Surrogate(seed=3) as net, q.Surrogate(seed=None) as zero:
@@ -366,7 +355,7 @@ def run_ladder_relax(q, args):
                                "anchoring, L1=6e-12) at T=300 K to |dF/F|<1e-6 between checks every "
                                f"{every} steps",
                    "batched_s": t_batched, "sequential_s": t_seq, "autotune": tune,
-                   "converged_batched": n - len(active), "steps_per_droplet_batched": steps_b,
+                   "converged_batched": n_conv, "steps_per_droplet_batched": steps_b,
                    "steps_per_droplet_sequential": steps_s,
                    "same_steps": int(sum(a == c for a, c in zip(steps_b, steps_s))),
                    "same_defects": same_defects, "max_rel_dF_final": max_rel_dF,

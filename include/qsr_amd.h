@@ -141,6 +141,10 @@ int qsr_step_imex(void* ctx, int nsteps, double L_stab, int jacobi_iters);
 /* FP64 log-point observables (replaces reduce_energy, radiality, defect and xi
  * passes with their full-array D2H copies, QSR.cu:2323-2410, 2787-2874). */
 int qsr_observables(void* ctx, int box, const qsr_obs_opts* opts, qsr_obs* out);
+/* The total free energy of each listed box (qsr_observables' `total` with
+ * energies only, bit for bit), all reductions enqueued before one host
+ * synchronisation: the early-stop check of a batched context. */
+int qsr_energies(void* ctx, const int* boxes, int n, double* totals);
 int qsr_box_info_get(void* ctx, int box, qsr_box_info* out);
 /* Iteration counter of a box (T(iter) follows the protocol). */
 int qsr_set_iteration(void* ctx, int box, int64_t iteration);
@@ -286,7 +290,8 @@ int qsr_surr_apply_ladder(void* net, void* ctx, int mode, const double* scales, 
 /* Batched contexts: the same for each listed box (uploaded, not held, listed
  * once), one network pass per box; each box is kept or restored on its own by
  * its own energy and residual, and unlisted boxes are untouched.  out[i] is
- * the result of boxes[i]. */
+ * the result of boxes[i]; its ms_network is the device time of all listed
+ * boxes' network passes. */
 int qsr_surr_apply_boxes(void* net, void* ctx, int mode, const double* scales, int nscales, const int* boxes,
                          int nboxes, qsr_surr_result* out);
 

@@ -120,6 +120,11 @@ struct Ctx {
     long long* d_vox_prefix = nullptr;
     long long nvox_total = 0;
     float4* d_rhs0 = nullptr;  // per box (0, 0, 0, 1): the RHS mode with no stabiliser (residuals)
+    // many-box reductions (energies, residuals): per-box partial regions and result slots
+    double* d_mpart = nullptr;
+    long long mpart_cap = 0;
+    double* d_mout = nullptr;
+    long long mout_cap = 0;
 
     ~Ctx() {
         if (gexec) discard(hipGraphExecDestroy(gexec));
@@ -128,7 +133,7 @@ struct Ctx {
         for (void* p : {(void*)d_boxes, (void*)d_tiles, (void*)d_step_rows, (void*)d_obs_rows, (void*)d_face_list,
                         (void*)d_dyn, (void*)d_part, (void*)d_out, (void*)d_nf_slice[0], (void*)d_nf_slice[1],
                         (void*)d_nf3, (void*)q_rhs, (void*)d_imex, (void*)d_jcoef, (void*)d_vox_prefix,
-                        (void*)d_jflags, (void*)d_rhs0})
+                        (void*)d_jflags, (void*)d_rhs0, (void*)d_mpart, (void*)d_mout})
             discard(hipFree(p));
         if (h_dyn) discard(hipHostFree(h_dyn));
         for (hipEvent_t e : ev)
@@ -846,6 +851,97 @@ int qsr_box_info_get(void* ctx, int box, qsr_box_info* out) {
     return QSR_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Volume-pass parameters of one box at its last stepped temperature.
+qsro::ObsParams obs_params(const Ctx& c, int box, bool skip_order) {
+    const Box& b = c.boxes[box];
+    const qsr_params& p = b.p;
+    qsro::ObsParams P{};
+    P.Nx = p.Nx; P.Ny = p.Ny; P.Nz = p.Nz; P.periodic = p.periodic;
+    P.off = b.off; P.rows_off = b.rows_off;
+    P.dx = p.dx; P.dy = p.dy; P.dz = p.dz;
+    P.cx = b.g.cx; P.cy = b.g.cy; P.cz = b.g.cz; P.rx = b.g.rx; P.ry = b.g.ry; P.rz = b.g.rz;
+    P.min_spacing = b.g.min_spacing;
+    const double dT = b.last_T - p.T_star;
+    if (p.modechoice == 1) { P.c2 = (3.0 * p.a * dT) / 2.0; P.c3 = (3.0 * p.b) / 2.0; P.c4 = (9.0 * p.c) / 8.0; }
+    else { P.c2 = (3.0 * p.a * dT) / 2.0; P.c3 = (9.0 * p.b) / 2.0; P.c4 = (9.0 * p.c) / 4.0; }
+    P.no_L = (p.L1 == 0.0 && p.L2 == 0.0 && p.L3 == 0.0);
+    P.kappa = p.periodic && !P.no_L ? p.L1 : p.kappa;
+    P.L1 = p.L1; P.L2 = p.L2; P.L3 = p.L3;
+    P.W_over_delta = (p.W != 0.0 && b.thickness > 0.0) ? p.W / b.thickness : 0.0;
+    P.S_shell = b.last_S;
+    P.skip_order = skip_order ? 1 : 0;
+    return P;
+}
+
+int ensure_many(Ctx& c, long long part_doubles, long long out_doubles) {
+    if (part_doubles > c.mpart_cap) {
+        discard(hipFree(c.d_mpart));
+        c.d_mpart = nullptr;
+        c.mpart_cap = 0;
+        HIP_TRY(hipMalloc(&c.d_mpart, sizeof(double) * part_doubles));
+        c.mpart_cap = part_doubles;
+    }
+    if (out_doubles > c.mout_cap) {
+        discard(hipFree(c.d_mout));
+        c.d_mout = nullptr;
+        c.mout_cap = 0;
+        HIP_TRY(hipMalloc(&c.d_mout, sizeof(double) * out_doubles));
+        c.mout_cap = out_doubles;
+    }
+    return QSR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsr_energies(void* ctx, const int* boxes, int n, double* totals) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !boxes || !totals || n < 0) return fail(QSR_E_ARG, "bad argument");
+    for (int i = 0; i < n; ++i)
+        if (boxes[i] < 0 || boxes[i] >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad box");
+    if (n == 0) return QSR_OK;
+    if (int rc = check_uploaded(*c)) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    long long part = 0;
+    for (int i = 0; i < n; ++i) part += (c->boxes[boxes[i]].nvox + 255) / 256 * qsro::NPART;
+    if (int rc = ensure_many(*c, part, (long long)n * qsro::NPART)) return rc;
+    const float* q = c->q[c->cur];
+    long long off = 0;
+    std::vector<int> periodic;
+    for (int i = 0; i < n; ++i) {
+        const Box& b = c->boxes[boxes[i]];
+        if (b.p.periodic) {  // the periodic pass also fills a nematic field: one box at a time
+            periodic.push_back(i);
+            continue;
+        }
+        const qsro::ObsParams P = obs_params(*c, boxes[i], true);
+        long long nblk = 0;
+        HIP_TRY(qsro::launch_obs_droplet(c->st, q, c->cs, P, c->d_step_rows, c->d_obs_rows, c->d_mpart + off, &nblk));
+        HIP_TRY(qsro::launch_obs_finalize(c->st, c->d_mpart + off, nblk, qsro::NPART, 1u << 7,
+                                          c->d_mout + (long long)i * qsro::NPART));
+        off += (b.nvox + 255) / 256 * qsro::NPART;
+    }
+    std::vector<double> v(size_t(n) * qsro::NPART, 0.0);
+    HIP_TRY(hipMemcpyAsync(v.data(), c->d_mout, sizeof(double) * v.size(), hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    for (int i = 0; i < n; ++i) {
+        const double* w = v.data() + (long long)i * qsro::NPART;
+        totals[i] = w[0] + w[1] + w[2];
+    }
+    const qsr_obs_opts oo = {-1, 0.1, 0.25, -1, -1.0, 0, 0, 1};
+    for (int i : periodic) {
+        qsr_obs o{};
+        if (int rc = qsr_observables(ctx, boxes[i], &oo, &o)) return rc;
+        totals[i] = o.total;
+    }
+    return QSR_OK;
+}
+
 int qsr_observables(void* ctx, int box, const qsr_obs_opts* opts, qsr_obs* out) {
     Ctx* c = as_ctx(ctx);
     if (!c || !out || box < 0 || box >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad argument");
@@ -861,21 +957,8 @@ int qsr_observables(void* ctx, int box, const qsr_obs_opts* opts, qsr_obs* out) 
     const int xz = std::min(std::max(o.xi_z_slice < 0 ? zs : o.xi_z_slice, 0), p.Nz - 1);
     const double xthr = o.xi_S_threshold < 0 ? o.S_threshold : o.xi_S_threshold;
 
-    qsro::ObsParams P{};
-    P.Nx = p.Nx; P.Ny = p.Ny; P.Nz = p.Nz; P.periodic = p.periodic;
-    P.off = b.off; P.rows_off = b.rows_off;
-    P.dx = p.dx; P.dy = p.dy; P.dz = p.dz;
-    P.cx = b.g.cx; P.cy = b.g.cy; P.cz = b.g.cz; P.rx = b.g.rx; P.ry = b.g.ry; P.rz = b.g.rz;
-    P.min_spacing = b.g.min_spacing;
-    const double T = b.last_T, dT = T - p.T_star;
-    if (p.modechoice == 1) { P.c2 = (3.0 * p.a * dT) / 2.0; P.c3 = (3.0 * p.b) / 2.0; P.c4 = (9.0 * p.c) / 8.0; }
-    else { P.c2 = (3.0 * p.a * dT) / 2.0; P.c3 = (9.0 * p.b) / 2.0; P.c4 = (9.0 * p.c) / 4.0; }
-    P.no_L = (p.L1 == 0.0 && p.L2 == 0.0 && p.L3 == 0.0);
-    P.kappa = p.periodic && !P.no_L ? p.L1 : p.kappa;
-    P.L1 = p.L1; P.L2 = p.L2; P.L3 = p.L3;
-    P.W_over_delta = (p.W != 0.0 && b.thickness > 0.0) ? p.W / b.thickness : 0.0;
-    P.S_shell = b.last_S;
-    P.skip_order = o.skip_order ? 1 : 0;
+    qsro::ObsParams P = obs_params(*c, box, o.skip_order != 0);
+    const double T = b.last_T;
     const float* q = c->q[c->cur];
 
     qsr_obs r{};
@@ -1136,6 +1219,32 @@ int box_residual(void* ctx, int box, double* res, long long* count) {
     HIP_TRY(hipStreamSynchronize(c->st));
     *res = std::sqrt(r[0]);
     if (count) *count = (long long)r[1];
+    return QSR_OK;
+}
+
+int box_residuals(void* ctx, const int* boxes, int n, double* res, long long* counts) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !boxes || !res || n < 0) return fail(QSR_E_ARG, "bad argument");
+    if (!c->q_rhs) return fail(QSR_E_STATE, "no explicit step computed");
+    if (n == 0) return QSR_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    constexpr long long kMaxBlk = 2048;  // launch_residual's block cap
+    if (int rc = ensure_many(*c, 2 * kMaxBlk * n, 2LL * n)) return rc;
+    for (int i = 0; i < n; ++i) {
+        if (boxes[i] < 0 || boxes[i] >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad box");
+        long long nblk = 0;
+        double* part = c->d_mpart + 2 * kMaxBlk * i;
+        HIP_TRY(qsrs::launch_residual(c->st, c->q[c->cur], c->q_rhs, c->cs, c->h_boxes[boxes[i]], c->d_step_rows,
+                                      part, &nblk));
+        HIP_TRY(qsrs::launch_sum_slots(c->st, part, nblk, 2, c->d_mout + 2LL * i));
+    }
+    std::vector<double> v(2 * size_t(n));
+    HIP_TRY(hipMemcpyAsync(v.data(), c->d_mout, sizeof(double) * v.size(), hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    for (int i = 0; i < n; ++i) {
+        res[i] = std::sqrt(v[2 * i]);
+        if (counts) counts[i] = (long long)v[2 * i + 1];
+    }
     return QSR_OK;
 }
 

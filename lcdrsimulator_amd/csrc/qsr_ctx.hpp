@@ -41,6 +41,8 @@ int euler_residual(void* ctx, int box, double* res, long long* count = nullptr);
 // FieldView::rhs, then the residual of one box from it.
 int euler_rhs(void* ctx);
 int box_residual(void* ctx, int box, double* res, long long* count = nullptr);
+// box_residual of n boxes with one host synchronisation (counts may be null).
+int box_residuals(void* ctx, const int* boxes, int n, double* res, long long* counts);
 // Batched contexts: swap the ping-pong buffers of every box (the caller makes
 // sure both hold what it wants for every box); copy the current field into
 // the other buffer; copy one box's region of the other buffer into the current.

@@ -257,6 +257,7 @@ int check_net(Net* n) {
 // Network pass over one box of a context, on the context's stream: the proposal
 // q_old + out_scale * (y, or u (1 + y) under increment and gain input) into
 // v.other on the voxels a step can change, from v.cur (and v.rhs = E(v.cur)).
+// With ms_out the pass is timed (and waited for).
 int propose_box(Net& n, const qsrc::FieldView& v, double s_inc, float out_scale, double* ms_out) {
     const bool incr = n.input_kind != QSR_SURR_INPUT_FIELD;
     const long long V = v.nvox;
@@ -273,7 +274,7 @@ int propose_box(Net& n, const qsrc::FieldView& v, double s_inc, float out_scale,
     }
     const hipStream_t st = v.st;
     ConvArgs a = conv_dims(1, v.box.Nx, v.box.Ny, v.box.Nz);
-    HIP_TRY(hipEventRecord(n.ev[0], st));
+    if (ms_out) HIP_TRY(hipEventRecord(n.ev[0], st));
     if (n.input_kind == QSR_SURR_INPUT_INCREMENT)
         HIP_TRY(launch_pack_increment(st, v.cur, v.rhs, v.cs, v.box, v.rows, float(1.0 / s_inc), n.ax));
     else
@@ -300,11 +301,13 @@ int propose_box(Net& n, const qsrc::FieldView& v, double s_inc, float out_scale,
     a.box = v.box;
     a.rows = v.rows;
     HIP_TRY(launch_conv(32, 1, EPI_APPLY, st, a));
-    HIP_TRY(hipEventRecord(n.ev[1], st));
-    HIP_TRY(hipEventSynchronize(n.ev[1]));
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, n.ev[0], n.ev[1]));
-    *ms_out = ms;
+    if (ms_out) {
+        HIP_TRY(hipEventRecord(n.ev[1], st));
+        HIP_TRY(hipEventSynchronize(n.ev[1]));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, n.ev[0], n.ev[1]));
+        *ms_out = ms;
+    }
     return QSR_OK;
 }
 
@@ -802,31 +805,34 @@ int qsr_surr_apply_boxes(void* net, void* ctx, int mode, const double* scales, i
     HIP_TRY(hipSetDevice(n->device));
     HIP_TRY(hipStreamSynchronize(n->st));  // parameters last written on the network's stream
     const bool incr = n->input_kind != QSR_SURR_INPUT_FIELD;
-    const qsr_obs_opts oo = {-1, 0.1, 0.25, -1, -1.0, 0, 0, 1};  // energies only
-    qsr_obs o{};
     std::vector<qsr_surr_result> R(static_cast<size_t>(nsel));
-    std::vector<double> s_inc(static_cast<size_t>(nsel), 0.0);
+    std::vector<double> s_inc(static_cast<size_t>(nsel), 0.0), F(static_cast<size_t>(nsel)),
+        res(static_cast<size_t>(nsel));
+    std::vector<long long> cnt(static_cast<size_t>(nsel), 0);
     // energies and residuals before, from one explicit step of every active box
     if (int rc = qsrc::euler_rhs(ctx)) return rc;
+    if (int rc = qsr_energies(ctx, boxes, nsel, F.data())) return rc;
+    if (int rc = qsrc::box_residuals(ctx, boxes, nsel, res.data(), cnt.data())) return rc;
     for (int i = 0; i < nsel; ++i) {
-        if (int rc = qsr_observables(ctx, boxes[i], &oo, &o)) return rc;
-        R[i].F_before = o.total;
-        long long cnt = 0;
-        if (int rc = qsrc::box_residual(ctx, boxes[i], &R[i].res_before, &cnt)) return rc;
-        s_inc[i] = cnt > 0 ? R[i].res_before / std::sqrt(5.0 * double(cnt)) : 0.0;
-        R[i].F_after = R[i].F_before;
-        R[i].res_after = R[i].res_before;
+        R[i].F_before = R[i].F_after = F[i];
+        R[i].res_before = R[i].res_after = res[i];
+        s_inc[i] = cnt[i] > 0 ? res[i] / std::sqrt(5.0 * double(cnt[i])) : 0.0;
     }
     // proposals into the other buffer, which holds a copy of the current field elsewhere
     if (int rc = qsrc::copy_current_to_other(ctx)) return rc;
     std::vector<int> pending;
+    HIP_TRY(hipEventRecord(n->ev[0], v.st));
     for (int i = 0; i < nsel; ++i) {
         if (incr && !(s_inc[i] > 0.0 && std::isfinite(s_inc[i]))) continue;  // nothing to propose
         if (int rc = qsrc::field_view(ctx, boxes[i], &v)) return rc;
-        if (int rc = propose_box(*n, v, s_inc[i], float(scales[0] * (incr ? n->horizon : 1.0)), &R[i].ms_network))
-            return rc;
+        if (int rc = propose_box(*n, v, s_inc[i], float(scales[0] * (incr ? n->horizon : 1.0)), nullptr)) return rc;
         pending.push_back(i);
     }
+    HIP_TRY(hipEventRecord(n->ev[1], v.st));
+    HIP_TRY(hipEventSynchronize(n->ev[1]));
+    float phase_ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&phase_ms, n->ev[0], n->ev[1]));
+    for (int i : pending) R[i].ms_network = phase_ms;  // the device time of all listed boxes' passes
     if (!pending.empty()) {
         // current: the proposals (every other box unchanged); other: the previous field
         if (int rc = qsrc::swap_buffers(ctx)) return rc;
@@ -839,13 +845,17 @@ int qsr_surr_apply_boxes(void* net, void* ctx, int mode, const double* scales, i
                                                 float(scales[r] / scales[r - 1])));
                 }
             if (int rc = qsrc::euler_rhs(ctx)) return rc;
+            std::vector<int> pb;
+            for (int i : pending) pb.push_back(boxes[i]);
+            if (int rc = qsr_energies(ctx, pb.data(), int(pb.size()), F.data())) return rc;
+            if (int rc = qsrc::box_residuals(ctx, pb.data(), int(pb.size()), res.data(), nullptr)) return rc;
             std::vector<int> still;
-            for (int i : pending) {
+            for (size_t m = 0; m < pending.size(); ++m) {
+                const int i = pending[m];
                 R[i].scale = scales[r];
                 R[i].tried = r + 1;
-                if (int rc = qsr_observables(ctx, boxes[i], &oo, &o)) return rc;
-                R[i].F_after = o.total;
-                if (int rc = qsrc::box_residual(ctx, boxes[i], &R[i].res_after)) return rc;
+                R[i].F_after = F[m];
+                R[i].res_after = res[m];
                 bool keep = mode == QSR_SURR_FORCE;
                 if (mode == QSR_SURR_CHECKED)
                     keep = std::isfinite(R[i].F_after) && std::isfinite(R[i].res_after) &&

@@ -189,14 +189,14 @@ def relax_batch(s: q.Solver, n_boxes: int, net: q.Surrogate | None = None, tol: 
     done, acc, att, net_ms = 0, 0, 0, 0.0
     t0 = time.perf_counter()
 
-    def energy(b):
-        return s.observables(box=b, want_defects=False, want_xi=False, want_order=False)["total"]
+    def energies(boxes):
+        return dict(zip(boxes, s.energies(boxes)))
 
     while active and done < max_steps:
         if net is None:
             s.step(every)
             done += every
-            F = {b: energy(b) for b in active}
+            F = energies(active)
         else:
             sel = [b for b in active if wait[b] == 0]
             for b in active:
@@ -212,10 +212,10 @@ def relax_batch(s: q.Solver, n_boxes: int, net: q.Surrogate | None = None, tol: 
             acc += sum(r["accepted"] for r in res)
             net_ms += sum(r["ms_network"] for r in res)
             s.step(every)
-            Fa = {b: energy(b) for b in active}
+            Fa = energies(active)
             s.step(every)
             done += 2 * every
-            F = {b: energy(b) for b in active}
+            F = energies(active)
         still = []
         for b in active:
             ref = prev[b] if net is None else Fa[b]

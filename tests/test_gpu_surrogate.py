@@ -474,6 +474,20 @@ def test_apply_boxes_equals_the_single_box_apply(qsr, ladder):
                 assert r[k] == r1[k], (b, k, r[k], r1[k])
 
 
+def test_energies_equal_the_observables_totals(qsr):
+    params = _ladder_boxes(qsr)
+    with qsr.Solver(params) as s:
+        for b, p in enumerate(params):
+            s.upload(qsr.init_field(p, b % 2, 0.5, seed=b), box=b)
+        s.step(7)
+        ref = [s.observables(box=b, want_defects=False, want_xi=False, want_order=False)["total"]
+               for b in range(len(params))]
+        np.testing.assert_array_equal(s.energies([3, 0, 2, 1]), [ref[3], ref[0], ref[2], ref[1]])
+        assert s.energies([]).size == 0
+        with pytest.raises(qsr.QSRError):
+            s.energies([4])
+
+
 def test_apply_boxes_rejects_bad_lists(qsr):
     params = _ladder_boxes(qsr)[:2]
     with qsr.Surrogate(seed=5) as net, qsr.Solver(params) as s:
