@@ -651,7 +651,7 @@ int qsr_autotune(void* ctx, int trial_steps, int* best_variant, int* best_ch, do
     double best = 1e300;
     int bv = c->variant, bch = c->ch;
     const int nvar = 2;  // tile shapes 64 x 4 and 32 x 8
-    for (int ch : {8, 16, 32, 64}) {
+    for (int ch : {8, 10, 12, 16, 32, 64}) {
         if (ch > 8 && ch / 2 >= maxNx) break;
         c->ch = ch;
         for (int v = 0; v < nvar; ++v) {
