@@ -210,7 +210,7 @@ def relax_batch(s: q.Solver, n_boxes: int, net: q.Surrogate | None = None, tol: 
                     wait[b] = gap[b]
             att += len(res)
             acc += sum(r["accepted"] for r in res)
-            net_ms += sum(r["ms_network"] for r in res)
+            net_ms += res[0]["ms_network"] if res else 0.0  # each result holds the whole phase's time
             s.step(every)
             Fa = energies(active)
             s.step(every)
