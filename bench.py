@@ -340,7 +340,7 @@ def run_ladder_relax(q, args):
         return {"seconds": r["seconds"], "converged": r["converged"], "steps_per_droplet": r["steps"],
                 "speedup_vs_sequential": t_seq / r["seconds"], "speedup_vs_batched_plain": t_batched / r["seconds"],
                 "steps_kept": r["jumps_accepted"], "steps_refused": r["jumps_rejected"],
-                "network_phase_ms": r["network_ms"],
+                "network_phase_ms": r["network_ms"], "apply_s": r["apply_s"],
                 "same_defects_as_batched_plain": int(sum(int(a["defects"]) == int(c["defects"])
                                                          for a, c in zip(fin, final_b))),
                 "max_rel_dF_vs_batched_plain": max(abs(a["total"] - c["total"]) / abs(c["total"])

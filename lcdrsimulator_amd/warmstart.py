@@ -186,7 +186,7 @@ def relax_batch(s: q.Solver, n_boxes: int, net: q.Surrogate | None = None, tol: 
     steps = [0] * n_boxes
     prev = [None] * n_boxes
     wait, gap = [0] * n_boxes, [0] * n_boxes
-    done, acc, att, net_ms = 0, 0, 0, 0.0
+    done, acc, att, net_ms, apply_s = 0, 0, 0, 0.0, 0.0
     t0 = time.perf_counter()
 
     def energies(boxes):
@@ -201,7 +201,9 @@ def relax_batch(s: q.Solver, n_boxes: int, net: q.Surrogate | None = None, tol: 
             sel = [b for b in active if wait[b] == 0]
             for b in active:
                 wait[b] = max(0, wait[b] - 1)
+            ta = time.perf_counter()
             res = net.apply_boxes(s, sel, q.SURR_CHECKED, scales) if sel else []
+            apply_s += time.perf_counter() - ta
             for b, r in zip(sel, res):
                 if r["accepted"]:
                     gap[b] = 0
@@ -228,7 +230,7 @@ def relax_batch(s: q.Solver, n_boxes: int, net: q.Surrogate | None = None, tol: 
         active = still
     return {"seconds": time.perf_counter() - t0, "steps": steps, "total_steps": done,
             "converged": n_boxes - len(active), "jumps_accepted": acc, "jumps_rejected": att - acc,
-            "network_ms": net_ms}
+            "network_ms": net_ms, "apply_s": apply_s}
 
 
 def zero_output_layer(net: q.Surrogate):
