@@ -143,7 +143,8 @@ int qsr_step_imex(void* ctx, int nsteps, double L_stab, int jacobi_iters);
 int qsr_observables(void* ctx, int box, const qsr_obs_opts* opts, qsr_obs* out);
 /* The total free energy of each listed box (qsr_observables' `total` with
  * energies only, bit for bit), all reductions enqueued before one host
- * synchronisation: the early-stop check of a batched context. */
+ * synchronisation: the energy sum of the early-stop test (reduce_energy,
+ * QSR.cu:2323-2341; the test, QSR.cu:3322-3335) for a batched context. */
 int qsr_energies(void* ctx, const int* boxes, int n, double* totals);
 int qsr_box_info_get(void* ctx, int box, qsr_box_info* out);
 /* Iteration counter of a box (T(iter) follows the protocol). */
