@@ -66,3 +66,109 @@ def test_snapshot_pairs_cover_every_gap_apart_pair():
     assert ws.snapshot_pairs(2, 250, 500)[2] == []
     with pytest.raises(ValueError):
         ws.snapshot_pairs(7, 300, 500)
+
+
+# ---------------------------------------------------------------------------
+# The warm-start drivers' host logic against stand-ins for the solver and the
+# network (no GPU): schedule, retirement, back-off, step accounting.
+# ---------------------------------------------------------------------------
+class _FakeSolver:
+    """F_b(t) = -1 - a_b exp(-t / tau_b) decays with the plain steps a box takes while active; a
+    kept learned step multiplies a_b by `jump`.  Every call is logged."""
+
+    def __init__(self, taus, a0=1e-2):
+        self.t = [0] * len(taus)
+        self.a = [a0] * len(taus)
+        self.tau = list(taus)
+        self.active = [True] * len(taus)
+        self.log = []
+
+    def F(self, b):
+        import math as m
+        return -1.0 - self.a[b] * m.exp(-self.t[b] / self.tau[b])
+
+    def step(self, n):
+        self.log.append(("step", n))
+        for b, on in enumerate(self.active):
+            if on:
+                self.t[b] += n
+
+    def observables(self, box=0, **kw):
+        self.log.append(("energy", box))
+        return {"total": self.F(box), "defects": 0, "plaquettes": 0}
+
+    def energies(self, boxes):
+        self.log.append(("energies", tuple(boxes)))
+        return np.array([self.F(b) for b in boxes])
+
+    def set_active(self, b, on):
+        self.active[b] = bool(on)
+
+
+class _FakeNet:
+    """Keeps a box's proposal when decide(box, call) says so, shrinking its amplitude."""
+
+    def __init__(self, decide, jump=0.25):
+        self.decide, self.jump, self.calls = decide, jump, []
+
+    def _one(self, s, b):
+        ok = self.decide(b, len([c for c in self.calls if c[0] == b]))
+        self.calls.append((b, ok))
+        if ok:
+            s.a[b] *= self.jump
+        return {"accepted": int(ok), "scale": 1.0, "ms_network": 1.0}
+
+    def apply(self, s, mode, scales=None):
+        s.log.append(("apply", 0))
+        return self._one(s, 0)
+
+    def apply_boxes(self, s, boxes, mode, scales):
+        s.log.append(("apply_boxes", tuple(boxes)))
+        return [self._one(s, b) for b in boxes]
+
+
+def test_relax_warm_cycles_never_test_across_a_jump(ws):
+    s = _FakeSolver([300.0])
+    r = ws.relax_warm(s, _FakeNet(lambda b, k: True), tol=1e-6, every=100, max_steps=100000)
+    assert r["converged"] and r["steps"] % 200 == 0
+    kinds = [e[0] for e in s.log]
+    cycles = [kinds[i:i + 5] for i in range(0, len(kinds), 5)]
+    assert all(c == ["apply", "step", "energy", "step", "energy"] for c in cycles)
+    assert len(cycles) == r["jumps_accepted"] + r["jumps_rejected"] == r["steps"] // 200
+    plain = ws.relax_plain(_FakeSolver([300.0]), tol=1e-6, every=100, max_steps=100000)
+    assert plain["converged"] and r["steps"] < plain["steps"]
+
+
+def test_relax_batch_plain_retires_each_box_at_its_own_test(ws):
+    taus = [100.0, 300.0, 900.0]
+    s = _FakeSolver(taus)
+    r = ws.relax_batch(s, 3, None, tol=1e-6, every=100, max_steps=100000)
+    assert r["converged"] == 3 and r["steps"][0] < r["steps"][1] < r["steps"][2] == r["total_steps"]
+    for b, tau in enumerate(taus):  # the same box alone, with relax_plain's test
+        alone = ws.relax_plain(_FakeSolver([tau]), tol=1e-6, every=100, max_steps=100000)
+        assert alone["steps"] == r["steps"][b]
+    assert not any(s.active)
+
+
+def test_relax_batch_backs_off_after_refusals(ws):
+    """Box 0's steps are always refused, box 1's always kept: box 0 is tried at cycles 0, 2, 5,
+    10, ... (it sits out 1, 2, 4, ... cycles), box 1 every cycle until it is retired."""
+    s = _FakeSolver([2000.0, 2000.0])
+    net = _FakeNet(lambda b, k: b == 1)
+    r = ws.relax_batch(s, 2, net, tol=1e-6, every=100, max_steps=6000)
+    def attempts(log, box):  # cycle index (two plain steps per cycle) of each try of `box`
+        out, nsteps = [], 0
+        for e in log:
+            if e[0] == "step":
+                nsteps += 1
+            elif e[0] == "apply_boxes" and box in e[1]:
+                out.append(nsteps // 2)
+        return out
+    cyc0, cyc1 = attempts(s.log, 0), attempts(s.log, 1)
+    assert cyc0[:5] == [0, 2, 5, 10, 19]
+    assert cyc1 == list(range(len(cyc1))) and r["steps"][1] == 200 * len(cyc1)
+    assert r["jumps_accepted"] == len(cyc1) and r["jumps_rejected"] == len(cyc0)
+    without = _FakeSolver([2000.0, 2000.0])
+    ws.relax_batch(without, 2, _FakeNet(lambda b, k: b == 1), tol=1e-6, every=100, max_steps=6000,
+                   backoff=False)
+    assert attempts(without.log, 0) == list(range(30))  # every cycle of the 6000 steps
