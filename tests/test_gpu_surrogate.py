@@ -530,3 +530,37 @@ def test_batched_warm_start_relaxes_every_box(qsr):
     assert pl["converged"] == wa["converged"] == len(params)
     assert sum(wa["steps"]) < sum(pl["steps"])
     assert max(rel) <= 1e-3
+
+
+def test_increment_training_loss_matches_the_host(qsr):
+    """The increment snapshots, their RMS s, the target scaling 1 / (N s), the pair weight and the
+    modulated prediction, pinned together: with a zero network (y = 0, prediction u / s) and one
+    crop covering the whole box, the first training iteration's loss is
+    mean over the voxels a step can change of w |u/s - (Q_b - Q_a)/(N s)|^2 / 5,
+    w = 1 / max(ms, 1), ms the mean square of the scaled target, computed here from downloads.
+    (u comes from a plain step here and from the residual pass in the library: equal to rounding.)"""
+    p = _droplet(qsr, n=24, R=9.0)
+    with qsr.Solver(p) as s, qsr.Surrogate(seed=None) as net:
+        net.params = np.zeros(S.NPARAM, dtype=np.float32)
+        net.set_input(qsr.SURR_INPUT_INCREMENT, HORIZON)
+        s.upload(qsr.init_field(p, 0, 0.5, seed=11))
+        s.step(30)
+        Qa = s.download()
+        a = net.dataset_add(s, increment=True)
+        s.step(1)
+        Q1 = s.download()
+        s.upload(Qa)
+        s.step(500)
+        Qb = s.download()
+        b = net.dataset_add(s)
+        hist = net.train([(a, b)], iters=1, batch=1, crop=24, lr=1e-12, seed=1)
+    u, srms, step = _increment(Qa, Q1)
+    mask = np.any(Q1 != Qa, axis=-1)  # the voxels a step can change (faces of a droplet box are static)
+    assert not np.any(mask & ~step)
+    x = (u.astype(np.float64) / srms)
+    t = (Qb - Qa) / (HORIZON * srms)
+    ms = float((t[mask] ** 2).sum()) / (5.0 * mask.sum())
+    w = 1.0 / max(ms, 1.0)
+    want = w * float(((x - t)[mask] ** 2).sum()) / (5.0 * mask.sum())
+    print(f"  s = {srms:.4e}, ms = {ms:.4e}, loss gpu {hist[0]:.8e} host {want:.8e}")
+    assert abs(hist[0] - want) <= 1e-4 * want
