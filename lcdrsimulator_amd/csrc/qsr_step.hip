@@ -260,8 +260,11 @@ __device__ __forceinline__ void decode_row4(int4 m, const BoxDev& B, int k, bool
 // ---------------------------------------------------------------------------
 // L1 / kappa kernel: x-register queue + one LDS plane (halo 1), double-buffered.
 // ---------------------------------------------------------------------------
+// Without noise the kernel fits 7 waves per SIMD (72 VGPRs, no spills; 6 waves uncapped): the
+// step is latency bound, so the extra wave pays.  The noisy variant (Philox + Box-Muller) keeps
+// its registers (5 waves); so do 64-bit-offset contexts, which would spill under the cap.
 template <int TK, int TJ, bool NOISE, bool BUF>
-__global__ __launch_bounds__(TK* TJ) void step_l1_kernel(const float* __restrict__ qin, float* __restrict__ qout,
+__global__ __launch_bounds__(TK* TJ, (!NOISE && BUF) ? 7 : 1) void step_l1_kernel(const float* __restrict__ qin, float* __restrict__ qout,
                                                           long long cs, const BoxDev* __restrict__ boxes,
                                                           const int4* __restrict__ tiles,
                                                           const StepDyn* __restrict__ dyn, int nboxes,
