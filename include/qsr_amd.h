@@ -206,7 +206,8 @@ int qsr_init_field(const qsr_params* p, int mode, double S0, double noise_amplit
  * from Q(k) (field input), or modulates the increment u(k) = E(Q(k)) - Q(k)
  * of one explicit Euler step E: dQ = N u (1 + y) per voxel and component, the
  * network reading x = u / s (increment input; s the RMS of u per component over
- * the voxels a step can change, box faces excluded) or x = Q(k) (gain input).
+ * the voxels a step can change, box faces excluded) or x = Q(k) (gain input);
+ * step-size input reads u / s and applies one gain per voxel, N u (1 + y_0).
  * A zero network then is the plain step extrapolated N times.  It runs on BF16
  * matrix cores with FP32 accumulation, and is
  * trained with its own backward and Adam kernels on pairs of fields the plain
@@ -223,7 +224,8 @@ enum {
 enum {
     QSR_SURR_INPUT_FIELD = 0,     /* x = Q(k), dQ = y (the default) */
     QSR_SURR_INPUT_INCREMENT = 1, /* x = u(k) / s, dQ = N u (1 + y) */
-    QSR_SURR_INPUT_GAIN = 2       /* x = Q(k), dQ = N u (1 + y) */
+    QSR_SURR_INPUT_GAIN = 2,      /* x = Q(k), dQ = N u (1 + y) */
+    QSR_SURR_INPUT_STEP = 3       /* x = u(k) / s, dQ = N u (1 + y_0): one gain per voxel */
 };
 typedef struct qsr_surr_result {
     double F_before, F_after;     /* total free energy (J) before / with the step */
@@ -252,7 +254,8 @@ int qsr_surr_load(void* net, const char* path);
 int qsr_surr_eval(void* net, int nbatch, int X, int Y, int Z, const float* x5, const float* target5,
                   const float* mask, double* loss, float* pred5, float* grad);
 /* The same with the modulated prediction of increment and gain input:
- * pred = m5 (1 + y), m5 (nbatch, X, Y, Z, 5) FP32 (the normalised increment u / s). */
+ * pred = m5 (1 + y), m5 (nbatch, X, Y, Z, 5) FP32 (the normalised increment u / s);
+ * under step-size input pred = m5 (1 + y_0), one gain per voxel. */
 int qsr_surr_eval_gain(void* net, int nbatch, int X, int Y, int Z, const float* x5, const float* m5,
                        const float* target5, const float* mask, double* loss, float* pred5, float* grad);
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected) with a given gradient. */
@@ -272,8 +275,8 @@ int qsr_surr_dataset_clear(void* net);
  * of the pairs (input snapshot, target snapshot) pairs[2p], pairs[2p+1]; the
  * target is their difference (field input), or the target is the difference
  * / (N s) and the prediction (u / s)(1 + y), u the input snapshot's recorded
- * increment, with the network reading u / s (increment input) or the input
- * snapshot's field (gain input).  A crop face inside its box leaves its 3-voxel
+ * increment, with the network reading u / s (increment and step-size input) or
+ * the input snapshot's field (gain input).  A crop face inside its box leaves its 3-voxel
  * band unscored.  Each pair's voxels are weighted by 1 / max(ms, weight_floor),
  * ms the mean square of its target over the voxels a step can change, so the
  * small changes late in a relaxation count more (weight_floor >= 1 weights

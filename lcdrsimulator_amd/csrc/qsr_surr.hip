@@ -215,7 +215,19 @@ __global__ __launch_bounds__(64 * TJ) void conv_kernel(ConvArgs a, int ch, int n
                             *reinterpret_cast<f32x4*>(a.pred + v * 8 + co) = c + bz;
                         }
                     } else if constexpr (EPI == EPI_APPLY) {
-                        if (co < 8 && changeable(a.box, a.rows, i, j, k)) {
+                        if (a.scalar_gain) {  // one gain per voxel: u (1 + y_0) in every component
+                            if (co == 0 && changeable(a.box, a.rows, i, j, k)) {
+                                const bool face = i == 0 || j == 0 || k == 0 || i == a.box.Nx - 1 ||
+                                                  j == a.box.Ny - 1 || k == a.box.Nz - 1;
+                                const float g = 1.0f + (c[0] + a.bias[0]);
+#pragma unroll
+                                for (int chn = 0; chn < CH_Q; ++chn) {
+                                    const long long idx = (long long)chn * a.cs + a.box.off + v;
+                                    const float d = face ? 0.0f : (a.rhs[idx] - a.q_old[idx]) * g;
+                                    a.q_new[idx] = __builtin_fmaf(a.out_scale, d, a.q_old[idx]);
+                                }
+                            }
+                        } else if (co < 8 && changeable(a.box, a.rows, i, j, k)) {
 #pragma unroll
                             for (int r = 0; r < 4; ++r) {
                                 const int chn = co + r;
@@ -474,7 +486,7 @@ __global__ __launch_bounds__(256) void crop_kernel(const Snap* __restrict__ snap
         const float qi = si.q[c * si.nvox + src];
         d[c] = (so.q[c * so.nvox + src] - qi) * J.out_scale;
         mi[c] = J.kind ? si.inc[c * si.nvox + src] * J.in_scale : 0.0f;
-        xi[c] = J.kind == 1 ? mi[c] : qi;
+        xi[c] = (J.kind == 1 || J.kind == 3) ? mi[c] : qi;  // increment or step-size input read u / s
     }
     d[5] = d[6] = d[7] = 0.0f;
     store_x(w, v, xi);
@@ -533,7 +545,7 @@ __global__ __launch_bounds__(256) void unpack_pred_kernel(Work w, float* __restr
     if (v >= w.V) return;
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
-        const float y = w.pred[v * 8 + c];
+        const float y = w.pred[v * 8 + (w.gain == 2 ? 0 : c)];
         out5[v * 5 + c] = w.gain ? w.mod[v * 8 + c] * (1.0f + y) : y;
     }
 }
@@ -566,20 +578,35 @@ __global__ __launch_bounds__(256) void loss_kernel(Work w, float scale) {
          v += (long long)gridDim.x * blockDim.x) {
         const float mk = w.mask[v];
         uint16_t h[8];
+        if (w.gain == 2) {  // prediction m (1 + y_0) in every component: dL/dy_0 = sum_c dL/dp_c m_c
+            const float y0 = w.pred[v * 8];
+            float g0 = 0.0f;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            float d = 0.0f, g = 1.0f;
-            if (c < CH_Q) {
-                const float y = w.pred[v * 8 + c];
-                if (w.gain) {  // prediction m (1 + y): dL/dy = dL/dp * m
-                    g = w.mod[v * 8 + c];
-                    d = g * (1.0f + y) - w.tgt[v * 8 + c];
-                } else {
-                    d = y - w.tgt[v * 8 + c];
-                }
+            for (int c = 0; c < CH_Q; ++c) {
+                const float m = w.mod[v * 8 + c];
+                const float d = m * (1.0f + y0) - w.tgt[v * 8 + c];
+                acc[0] += double(mk) * double(d) * double(d);
+                g0 = __builtin_fmaf(d, m, g0);
             }
-            acc[0] += double(mk) * double(d) * double(d);
-            h[c] = bf16_bits(scale * mk * d * g);
+            h[0] = bf16_bits(scale * mk * g0);
+#pragma unroll
+            for (int c = 1; c < 8; ++c) h[c] = 0;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                float d = 0.0f, g = 1.0f;
+                if (c < CH_Q) {
+                    const float y = w.pred[v * 8 + c];
+                    if (w.gain) {  // prediction m (1 + y): dL/dy = dL/dp * m
+                        g = w.mod[v * 8 + c];
+                        d = g * (1.0f + y) - w.tgt[v * 8 + c];
+                    } else {
+                        d = y - w.tgt[v * 8 + c];
+                    }
+                }
+                acc[0] += double(mk) * double(d) * double(d);
+                h[c] = bf16_bits(scale * mk * d * g);
+            }
         }
         acc[1] += mk != 0.0f ? 1.0 : 0.0;  // scored voxels (mask values may be weights)
         u32x4 pk;

@@ -87,6 +87,7 @@ struct ConvArgs {
     float* q_new;
     float out_scale;
     const float* rhs;       // EPI_APPLY under increment input: one explicit step of q_old (else null)
+    int scalar_gain;        // EPI_APPLY with rhs: u (1 + y_0) for every component (step-size input)
     long long cs;
     qsrk::BoxDev box;
     const qsrk::RowMask* rows;
@@ -105,8 +106,8 @@ struct Snap {
 // One training crop: pair (input snapshot, target snapshot), its origin, the
 // weight of its scored voxels (the inverse of the pair's target mean square)
 // and the input kind.  Field input: x = q_in, target = q_out - q_in.
-// Increment input: x = inc_in * in_scale, target = (q_out - q_in) * out_scale,
-// modulator inc_in * in_scale.  Gain input: x = q_in, the rest as increment input.
+// Increment and step-size input: x = inc_in * in_scale, target = (q_out - q_in) *
+// out_scale, modulator inc_in * in_scale.  Gain input: x = q_in, the rest as increment input.
 struct CropJob {
     int s_in, s_out, ox, oy, oz;
     float weight;
@@ -123,7 +124,7 @@ struct Work {
     uint16_t *a1, *a1_cm, *a2, *a2_cm;
     float *pred, *tgt, *mask;    // 8 ch, 8 ch, 1
     float* mod;                  // 8 ch FP32: the modulator m of a modulated prediction
-    int gain;                    // 1: the prediction is m (1 + y), else y
+    int gain;                    // 1: the prediction is m (1 + y); 2: m (1 + y_0) (one gain per voxel); 0: y
     uint16_t *dy, *dy_cm;        // 8 ch
     uint16_t *dz2, *dz2_cm, *dz1, *dz1_cm;
     float* part;                 // weight-gradient partials
@@ -173,7 +174,7 @@ hipError_t launch_pack_increment(hipStream_t st, const float* q, const float* rh
 // other = cur + ratio (other - cur) on the voxels a step can change.
 hipError_t launch_rescale_step(hipStream_t st, const float* cur, float* other, long long cs, const qsrk::BoxDev& box,
                                const qsrk::RowMask* rows, float ratio);
-// (nbatch, X, Y, Z, 5) output: y, or m (1 + y) when w.gain is set.
+// (nbatch, X, Y, Z, 5) output: y, or m (1 + y) (w.gain 1), or m (1 + y_0) (w.gain 2).
 hipError_t launch_unpack_pred(hipStream_t st, const Work& w, float* out5);
 
 }  // namespace qsrs

@@ -275,7 +275,7 @@ int propose_box(Net& n, const qsrc::FieldView& v, double s_inc, float out_scale,
     const hipStream_t st = v.st;
     ConvArgs a = conv_dims(1, v.box.Nx, v.box.Ny, v.box.Nz);
     if (ms_out) HIP_TRY(hipEventRecord(n.ev[0], st));
-    if (n.input_kind == QSR_SURR_INPUT_INCREMENT)
+    if (n.input_kind == QSR_SURR_INPUT_INCREMENT || n.input_kind == QSR_SURR_INPUT_STEP)
         HIP_TRY(launch_pack_increment(st, v.cur, v.rhs, v.cs, v.box, v.rows, float(1.0 / s_inc), n.ax));
     else
         HIP_TRY(launch_pack_field(st, v.cur, v.cs, v.box.off, V, n.ax));
@@ -297,6 +297,7 @@ int propose_box(Net& n, const qsrc::FieldView& v, double s_inc, float out_scale,
     a.q_new = v.other;
     a.out_scale = out_scale;
     a.rhs = incr ? v.rhs : nullptr;
+    a.scalar_gain = n.input_kind == QSR_SURR_INPUT_STEP ? 1 : 0;
     a.cs = v.cs;
     a.box = v.box;
     a.rows = v.rows;
@@ -476,7 +477,7 @@ int eval_batch(Net* n, int nbatch, int X, int Y, int Z, const float* x5, const f
     if (target5) HIP_TRY(hipMemcpyAsync(dt, target5, sizeof(float) * V * 5, hipMemcpyHostToDevice, n->st));
     if (mask) HIP_TRY(hipMemcpyAsync(dm, mask, sizeof(float) * V, hipMemcpyHostToDevice, n->st));
     if (m5) HIP_TRY(hipMemcpyAsync(dg, m5, sizeof(float) * V * 5, hipMemcpyHostToDevice, n->st));
-    n->w.gain = m5 ? 1 : 0;
+    n->w.gain = m5 ? (n->input_kind == QSR_SURR_INPUT_STEP ? 2 : 1) : 0;
     HIP_TRY(launch_pack_batch(n->st, dx, target5 ? dt : nullptr, mask ? dm : nullptr, m5 ? dg : nullptr, V, n->w));
     if (int rc = forward(*n, nbatch, X, Y, Z, grad != nullptr)) return rc;
     if (loss_out || grad) {
@@ -581,7 +582,8 @@ int qsr_surr_dataset_add_increment(void* net, void* ctx, int box, int* snapshot)
 int qsr_surr_set_input(void* net, int kind, double horizon) {
     Net* n = as_net(net);
     if (!n) return fail(QSR_E_ARG, "null network");
-    if (kind != QSR_SURR_INPUT_FIELD && kind != QSR_SURR_INPUT_INCREMENT && kind != QSR_SURR_INPUT_GAIN)
+    if (kind != QSR_SURR_INPUT_FIELD && kind != QSR_SURR_INPUT_INCREMENT && kind != QSR_SURR_INPUT_GAIN &&
+        kind != QSR_SURR_INPUT_STEP)
         return fail(QSR_E_ARG, "unknown input kind");
     if (!(horizon > 0.0) || !std::isfinite(horizon)) return fail(QSR_E_ARG, "horizon must be positive");
     n->input_kind = kind;
@@ -691,7 +693,7 @@ int qsr_surr_train(void* net, const int* pairs, int npairs, int iters, int batch
     }
     HIP_TRY(hipStreamSynchronize(n->st));
     HIP_TRY(hipMemcpy(n->d_jobs, jobs.data(), sizeof(CropJob) * njobs, hipMemcpyHostToDevice));
-    n->w.gain = incr ? 1 : 0;
+    n->w.gain = n->input_kind == QSR_SURR_INPUT_STEP ? 2 : (incr ? 1 : 0);
     for (int it = 0; it < iters; ++it) {
         HIP_TRY(launch_crop(n->st, n->d_snaps, n->d_jobs + (long long)it * batch, batch, crop, n->w));
         if (int rc = forward(*n, batch, crop, crop, crop, true)) return rc;

@@ -82,7 +82,7 @@ EXPORTS = (
 )
 
 SURR_CHECKED, SURR_FORCE, SURR_PROPOSE = 0, 1, 2
-SURR_INPUT_FIELD, SURR_INPUT_INCREMENT, SURR_INPUT_GAIN = 0, 1, 2
+SURR_INPUT_FIELD, SURR_INPUT_INCREMENT, SURR_INPUT_GAIN, SURR_INPUT_STEP = 0, 1, 2, 3
 
 
 class _SurrResult(C.Structure):

@@ -22,6 +22,9 @@ p = g (1 + y), and
 
     loss = sum_v m_v sum_c (g (1 + y) - target)^2 / (5 V),  dL/dy = 2 m (p - target) g / (5 V)
 
+or, under step-size input, one gain per voxel: p = g (1 + y_0) in every component and
+dL/dy_0 = 2 m sum_c (p_c - target_c) g_c / (5 V).
+
 With ``emulate=True`` the values the GPU keeps in BF16 are rounded the same way
 (round to nearest even): x, the weights, a1, a2 and the three output gradients
 dy, dz2, dz1.  Everything else is FP64.
@@ -134,18 +137,24 @@ def gain_prediction(g, y):
     return np.asarray(g, dtype=np.float64) * (1.0 + y)
 
 
-def loss_and_grad(params, x, target, mask=None, emulate=True, mod=None):
-    """loss = sum m |y - target|^2 / (5 V) (or, with a modulator ``mod`` = g, of g (1 + y) - target)
-    and its gradient in the flat layout."""
+def loss_and_grad(params, x, target, mask=None, emulate=True, mod=None, scalar=False):
+    """loss = sum m |y - target|^2 / (5 V) (or, with a modulator ``mod`` = g, of g (1 + y) - target;
+    with ``scalar`` of g (1 + y_0) - target, one gain per voxel) and its gradient in the flat layout."""
     f = forward(params, x, emulate)
     r = bf16 if emulate else (lambda v: v)
     V = np.prod(x.shape[:4])
     m = np.ones(x.shape[:4]) if mask is None else np.asarray(mask, dtype=np.float64)
     g = None if mod is None else np.asarray(mod, dtype=np.float64)
-    d = (f["y"] if g is None else gain_prediction(g, f["y"])) - target
+    y = f["y"][..., :1] if scalar else f["y"]
+    d = (f["y"] if g is None else gain_prediction(g, y)) - target
     loss = float(np.sum(m[..., None] * d * d) / (5.0 * V))
     scale = np.float32(2.0 / (5.0 * V)) if emulate else 2.0 / (5.0 * V)
-    dy = r(np.float64(scale) * m[..., None] * d * (1.0 if g is None else g))
+    if scalar:  # dL/dy_0 = sum_c dL/dp_c g_c; the other outputs do not enter the loss
+        dy = np.zeros_like(f["y"])
+        dy[..., 0] = np.sum(d * g, axis=-1)
+        dy = r(np.float64(scale) * m[..., None] * dy)
+    else:
+        dy = r(np.float64(scale) * m[..., None] * d * (1.0 if g is None else g))
     dW3, db3 = conv_weight_grad(f["a2"], dy)
     dz2 = r(conv_input_grad(dy, f["W3"]) * gelu_grad(f["z2"]))
     dW2, db2 = conv_weight_grad(f["a1"], dz2)

@@ -277,14 +277,15 @@ def _increment(Q, Q1):
     return u, s, step
 
 
-@pytest.mark.parametrize("kind", ["increment", "gain"])
+@pytest.mark.parametrize("kind", ["increment", "gain", "step"])
 def test_modulated_apply_force_writes_q_plus_modulated_increment(qsr, kind):
     """dQ = N u (1 + y) on the voxels a step changes, y the network's output for u / s
-    (increment input) or for Q (gain input)."""
+    (increment input) or for Q (gain input); N u (1 + y_0) under step-size input."""
     p = _droplet(qsr)
     Q0 = qsr.init_field(p, 0, 0.5, seed=3).astype(np.float32).astype(np.float64)
     with qsr.Solver(p) as s, qsr.Surrogate(seed=9) as net:
-        net.set_input(qsr.SURR_INPUT_INCREMENT if kind == "increment" else qsr.SURR_INPUT_GAIN, HORIZON)
+        net.set_input({"increment": qsr.SURR_INPUT_INCREMENT, "gain": qsr.SURR_INPUT_GAIN,
+                       "step": qsr.SURR_INPUT_STEP}[kind], HORIZON)
         s.upload(Q0)
         s.step(10)
         Q = s.download()
@@ -294,9 +295,10 @@ def test_modulated_apply_force_writes_q_plus_modulated_increment(qsr, kind):
         r = net.apply(s, qsr.SURR_FORCE)
         Qn = s.download()
         u, srms, step = _increment(Q, Q1)
-        x = (u * np.float32(1.0 / srms)) if kind == "increment" else Q.astype(np.float32)
+        x = (u * np.float32(1.0 / srms)) if kind != "gain" else Q.astype(np.float32)
         _, y, _ = net.eval(x[None])
-    dq = np.float32(HORIZON) * u * (np.float32(1) + y[0])
+    g = y[0][..., :1] if kind == "step" else y[0]
+    dq = np.float32(HORIZON) * u * (np.float32(1) + g)
     expect = np.where(step[..., None], (Q.astype(np.float32) + dq).astype(np.float64), Q)
     err = np.abs(Qn - expect).max()
     print(f"  {kind}: s = {srms:.3e}, max |dQ| {np.abs(dq).max():.3e}, max err {err:.3e}")
@@ -308,21 +310,28 @@ def test_modulated_apply_force_writes_q_plus_modulated_increment(qsr, kind):
 
 
 @pytest.mark.parametrize("shape", [(1, 8, 8, 16), (2, 5, 6, 24)])
-def test_modulated_loss_and_gradient_match_oracle(qsr, shape):
-    """Increment and gain input: the prediction is g (1 + y) and the loss is taken on it."""
+@pytest.mark.parametrize("scalar", [False, True])
+def test_modulated_loss_and_gradient_match_oracle(qsr, shape, scalar):
+    """Increment and gain input: the prediction is g (1 + y) and the loss is taken on it;
+    step-size input: g (1 + y_0), one gain per voxel."""
     p = S.random_params(12, scale=2.0).astype(np.float32)
     x, t, m = _batch(5, shape)
     g5 = np.random.default_rng(6).normal(scale=0.5, size=x.shape).astype(np.float32)
     with qsr.Surrogate(seed=None) as net:
         net.params = p
+        if scalar:
+            net.set_input(qsr.SURR_INPUT_STEP, HORIZON)
         loss, pred, g = net.eval(x, t, m, want_pred=True, want_grad=True, modulator=g5)
     lo, go, f = S.loss_and_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64),
-                                m.astype(np.float64), emulate=True, mod=g5)
-    ref = S.gain_prediction(g5, f["y"])
+                                m.astype(np.float64), emulate=True, mod=g5, scalar=scalar)
+    ref = S.gain_prediction(g5, f["y"][..., :1] if scalar else f["y"])
     print(f"  loss gpu {loss:.6e} oracle {lo:.6e}; pred max err {np.abs(pred - ref).max():.3e}")
     assert np.abs(pred - ref).max() <= FWD_TOL * np.abs(ref).max()
     assert abs(loss - lo) <= 1e-4 * lo
     for name, a, b in BLOCKS:
+        if np.linalg.norm(go[a:b]) == 0.0:  # one gain per voxel: outputs 1-4 get no gradient
+            assert np.all(g[a:b] == 0.0), name
+            continue
         rel = np.linalg.norm(g[a:b] - go[a:b]) / np.linalg.norm(go[a:b])
         assert rel <= GRAD_TOL, (name, rel)
 

@@ -79,16 +79,17 @@ def test_gelu_derivative():
     assert S.gelu(0.0) == 0.0 and abs(S.gelu_grad(0.0) - 0.5) < 1e-15 and abs(S.gelu(10.0) - 10.0) < 1e-12
 
 
-@pytest.mark.parametrize("gain", [False, True])
+@pytest.mark.parametrize("gain", [False, True, "scalar"])
 def test_gradient_matches_finite_differences(gain):
-    """Both losses: y against the target, and the modulated g (1 + y) against it."""
+    """The three losses: y against the target, the modulated g (1 + y) and g (1 + y_0) against it."""
     rng = np.random.default_rng(8)
     p = S.random_params(1, scale=1.5)
     x = rng.normal(scale=0.3, size=(1, 4, 5, 6, 5))
     t = rng.normal(scale=0.1, size=x.shape)
     m = (rng.random(x.shape[:4]) < 0.7).astype(np.float64)
     mod = rng.normal(scale=0.5, size=x.shape) if gain else None
-    loss, g, f = S.loss_and_grad(p, x, t, m, emulate=False, mod=mod)
+    sc = gain == "scalar"
+    loss, g, f = S.loss_and_grad(p, x, t, m, emulate=False, mod=mod, scalar=sc)
     idx = np.concatenate([rng.choice(np.arange(a, b), min(8, b - a), replace=False) for a, b in
                           ((S.OFF_W1, S.OFF_B1), (S.OFF_B1, S.OFF_W2), (S.OFF_W2, S.OFF_B2),
                            (S.OFF_B2, S.OFF_W3), (S.OFF_W3, S.OFF_B3), (S.OFF_B3, S.NPARAM))])
@@ -97,8 +98,8 @@ def test_gradient_matches_finite_differences(gain):
         pp, pm = p.copy(), p.copy()
         pp[i] += h
         pm[i] -= h
-        fd = (S.loss_and_grad(pp, x, t, m, emulate=False, mod=mod)[0] -
-              S.loss_and_grad(pm, x, t, m, emulate=False, mod=mod)[0]) / (2 * h)
+        fd = (S.loss_and_grad(pp, x, t, m, emulate=False, mod=mod, scalar=sc)[0] -
+              S.loss_and_grad(pm, x, t, m, emulate=False, mod=mod, scalar=sc)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(abs(fd), 1e-3 * abs(loss)), (i, fd, g[i])
 
 
