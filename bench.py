@@ -308,7 +308,7 @@ def run_ladder_relax(q, args):
                 for b, Q in enumerate(fields):
                     s.upload(Q, box=b)
                 s.set_tuning(tune["variant"], tune["planes_per_tile"])
-                r = ws.relax_batch(s, n, nt, tol, every, nmax)
+                r = ws.relax_batch(s, n, nt, tol, every, nmax, tail_tol=args.warm_tail_tol or None)
                 r["final"] = [s.observables(box=b, want_xi=False) for b in range(n)]
             warm[name] = r
             log(f"ladder relaxation batched + {name} network: {r['seconds']:.3f} s, {r['converged']} of {n} converged, "
@@ -344,7 +344,16 @@ def run_ladder_relax(q, args):
                 "same_defects_as_batched_plain": int(sum(int(a["defects"]) == int(c["defects"])
                                                          for a, c in zip(fin, final_b))),
                 "max_rel_dF_vs_batched_plain": max(abs(a["total"] - c["total"]) / abs(c["total"])
-                                                   for a, c in zip(fin, final_b))}
+                                                   for a, c in zip(fin, final_b)),
+                "worst_droplet": worst(r)}
+
+    def worst(r):
+        fin = r["final"]
+        rel = [(a["total"] - c["total"]) / abs(c["total"]) for a, c in zip(fin, final_b)]
+        b = int(np.argmax(np.abs(rel)))
+        return {"droplet": b, "radius": params[b].radius_index, "W": params[b].W,
+                "F_plain": final_b[b]["total"], "F_warm": fin[b]["total"], "rel": rel[b],
+                "steps_plain": steps_b[b], "steps_warm": r["steps"][b]}
     tw = warm["trained"]["seconds"]
     out = {
         "metric": METRIC, "value": t_seq / tw, "unit": "x (time to |dF/F|<1e-6: one context per droplet, plain "
@@ -550,6 +559,9 @@ def main():
                     help="surrogate_192: time only the network pass (--steps passes after --warmup)")
     ap.add_argument("--surr-matched", action="store_true",
                     help="surrogate_192: train at the application's T and W instead of their ranges")
+    ap.add_argument("--warm-tail-tol", type=float, default=1e-5,
+                    help="ladder_relax_64: no learned steps for a droplet once its |dF/F| over a cycle is below this, "
+                         "so the plain solver makes the last approach (0: learned steps until convergence)")
     ap.add_argument("--surr-iters", type=int, default=1000,
                     help="surrogate_192: Adam iterations per group of 16 trajectories")
     ap.add_argument("--workload", default=WORKLOADS[0], choices=WORKLOADS,

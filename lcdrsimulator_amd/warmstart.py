@@ -144,23 +144,26 @@ def relax_plain(s: q.Solver, tol: float = 1e-6, every: int = 100, max_steps: int
 
 
 def relax_warm(s: q.Solver, net: q.Surrogate, tol: float = 1e-6, every: int = 100, max_steps: int = 200000,
-               scales=(1.0, 0.5, 0.25)) -> dict:
+               scales=(1.0, 0.5, 0.25), tail_tol: float | None = None) -> dict:
     """Cycles of one checked learned step (the ladder ``scales``: the first proposal that
     raises neither F nor the residual is kept, else the field is left as it was) and 2 x ``every``
     plain steps with the energy checked after each half; converged when |dF/F| < tol between
     the two checks of a cycle, which no jump separates (the energy part of QSR.cu:3322-3335).
+    With ``tail_tol``, once |dF/F| over a cycle falls below it no more learned steps are taken.
     ``steps`` counts plain steps only."""
     n, att, acc, net_ms = 0, 0, 0, 0.0
     kept = []
     conv, rel = False, math.nan
     t0 = time.perf_counter()
+    tail = False
     while n < max_steps:
-        r = net.apply(s, q.SURR_CHECKED, scales=list(scales))
-        att += 1
-        net_ms += r["ms_network"]
-        if r["accepted"]:
-            acc += 1
-            kept.append(r["scale"])
+        if not tail:
+            r = net.apply(s, q.SURR_CHECKED, scales=list(scales))
+            att += 1
+            net_ms += r["ms_network"]
+            if r["accepted"]:
+                acc += 1
+                kept.append(r["scale"])
         s.step(every)
         Fa = _energy(s)
         s.step(every)
@@ -170,22 +173,27 @@ def relax_warm(s: q.Solver, net: q.Surrogate, tol: float = 1e-6, every: int = 10
         if rel < tol:
             conv = True
             break
+        if tail_tol is not None and rel < tail_tol:
+            tail = True
     return {"steps": n, "seconds": time.perf_counter() - t0, "converged": conv, "last_rel_dF": rel,
             "jumps_accepted": acc, "jumps_rejected": att - acc, "scales_kept": kept, "network_ms": net_ms}
 
 
 def relax_batch(s: q.Solver, n_boxes: int, net: q.Surrogate | None = None, tol: float = 1e-6, every: int = 100,
-                max_steps: int = 200000, scales=(1.0, 0.5, 0.25), backoff: bool = True) -> dict:
+                max_steps: int = 200000, scales=(1.0, 0.5, 0.25), backoff: bool = True,
+                tail_tol: float | None = None) -> dict:
     """Relax every box of a batched solver, retiring each (qsr_set_box_active) once its energy
     test passes.  Plain (no ``net``): the test of :func:`relax_plain` per box, checks every
     ``every`` steps.  Warm-started: the cycles of :func:`relax_warm`, with one checked learned step
     per active box per cycle (:meth:`Surrogate.apply_boxes`); with ``backoff`` a box whose step was
-    refused sits out the next 1, 2, 4, ... cycles (reset when one is kept).  ``steps`` per box
-    counts plain steps."""
+    refused sits out the next 1, 2, 4, ... cycles (reset when one is kept).  With ``tail_tol`` a box
+    whose |dF/F| over a cycle's second half fell below it takes no more learned steps: the plain
+    solver alone makes the last approach to the test.  ``steps`` per box counts plain steps."""
     active = list(range(n_boxes))
     steps = [0] * n_boxes
     prev = [None] * n_boxes
     wait, gap = [0] * n_boxes, [0] * n_boxes
+    tail = [False] * n_boxes
     done, acc, att, net_ms, apply_s = 0, 0, 0, 0.0, 0.0
     t0 = time.perf_counter()
 
@@ -198,7 +206,7 @@ def relax_batch(s: q.Solver, n_boxes: int, net: q.Surrogate | None = None, tol: 
             done += every
             F = energies(active)
         else:
-            sel = [b for b in active if wait[b] == 0]
+            sel = [b for b in active if wait[b] == 0 and not tail[b]]
             for b in active:
                 wait[b] = max(0, wait[b] - 1)
             ta = time.perf_counter()
@@ -221,6 +229,8 @@ def relax_batch(s: q.Solver, n_boxes: int, net: q.Surrogate | None = None, tol: 
         still = []
         for b in active:
             ref = prev[b] if net is None else Fa[b]
+            if net is not None and tail_tol is not None and abs((F[b] - ref) / ref) < tail_tol:
+                tail[b] = True
             if ref is not None and abs((F[b] - ref) / ref) < tol:
                 steps[b] = done
                 s.set_active(b, False)
