@@ -172,3 +172,25 @@ def test_relax_batch_backs_off_after_refusals(ws):
     ws.relax_batch(without, 2, _FakeNet(lambda b, k: b == 1), tol=1e-6, every=100, max_steps=6000,
                    backoff=False)
     assert attempts(without.log, 0) == list(range(30))  # every cycle of the 6000 steps
+
+
+def test_plain_tail_stops_learned_steps(ws):
+    """With tail_tol, once a cycle's |dF/F| is below it no learned step is proposed again: the
+    plain solver alone makes the last approach, in the single-box and the batched driver."""
+    s = _FakeSolver([400.0])
+    r = ws.relax_warm(s, _FakeNet(lambda b, k: False), tol=1e-6, every=100, max_steps=100000, tail_tol=1e-4)
+    s0 = _FakeSolver([400.0])
+    r0 = ws.relax_warm(s0, _FakeNet(lambda b, k: False), tol=1e-6, every=100, max_steps=100000)
+    assert r["converged"] and r["steps"] == r0["steps"]  # refused steps change nothing
+    assert r["jumps_rejected"] < r0["jumps_rejected"]
+    # the proposals stop at the first cycle whose |dF/F| is below 1e-4 and never resume
+    kinds = [e[0] for e in s.log]
+    last_apply = max(i for i, k in enumerate(kinds) if k == "apply")
+    assert "apply" not in kinds[last_apply + 1:]
+    sb = _FakeSolver([400.0, 800.0])
+    rb = ws.relax_batch(sb, 2, _FakeNet(lambda b, k: False), tol=1e-6, every=100, max_steps=100000,
+                        backoff=False, tail_tol=1e-4)
+    sb0 = _FakeSolver([400.0, 800.0])
+    rb0 = ws.relax_batch(sb0, 2, _FakeNet(lambda b, k: False), tol=1e-6, every=100, max_steps=100000,
+                         backoff=False)
+    assert rb["steps"] == rb0["steps"] and rb["jumps_rejected"] < rb0["jumps_rejected"]
