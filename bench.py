@@ -308,7 +308,8 @@ def run_ladder_relax(q, args):
                 for b, Q in enumerate(fields):
                     s.upload(Q, box=b)
                 s.set_tuning(tune["variant"], tune["planes_per_tile"])
-                r = ws.relax_batch(s, n, nt, tol, every, nmax, tail_tol=args.warm_tail_tol or None)
+                tail = 1e-5 if args.warm_tail_tol is None else (args.warm_tail_tol or None)
+                r = ws.relax_batch(s, n, nt, tol, every, nmax, tail_tol=tail)
                 r["final"] = [s.observables(box=b, want_xi=False) for b in range(n)]
             warm[name] = r
             log(f"ladder relaxation batched + {name} network: {r['seconds']:.3f} s, {r['converged']} of {n} converged, "
@@ -491,8 +492,9 @@ def run_surrogate(q, args):
         zero.params = np.zeros(zero.nparams, dtype=np.float32)
         zero.set_input(q.SURR_INPUT_INCREMENT, float(horizon))
         p = ws.droplet_params(88.0, 300.0, 1e-4, N=192)
-        res = {m: ws.compare(p, net, m, max_steps=200000, log=log) for m in (1, 0)}
-        res_zero = {m: ws.compare(p, zero, m, max_steps=200000, log=log) for m in (1, 0)}
+        tail = args.warm_tail_tol or None  # off unless asked for
+        res = {m: ws.compare(p, net, m, max_steps=200000, tail_tol=tail, log=log) for m in (1, 0)}
+        res_zero = {m: ws.compare(p, zero, m, max_steps=200000, tail_tol=tail, log=log) for m in (1, 0)}
         with q.Solver(p) as s:
             s.upload(q.init_field(p, 1, 0.5, 0.0, 12345))
             s.step(10)
@@ -559,9 +561,11 @@ def main():
                     help="surrogate_192: time only the network pass (--steps passes after --warmup)")
     ap.add_argument("--surr-matched", action="store_true",
                     help="surrogate_192: train at the application's T and W instead of their ranges")
-    ap.add_argument("--warm-tail-tol", type=float, default=1e-5,
-                    help="ladder_relax_64: no learned steps for a droplet once its |dF/F| over a cycle is below this, "
-                         "so the plain solver makes the last approach (0: learned steps until convergence)")
+    ap.add_argument("--warm-tail-tol", type=float, default=None,
+                    help="ladder_relax_64 and surrogate_192: no learned step in a cycle after one whose |dF/F| was "
+                         "below this, so the plain solver makes the last approach (0: off; default 1e-5 for the "
+                         "ladder's radial starts, off for surrogate_192, whose random-director start coarsens through "
+                         "long stretches of small |dF/F| where the learned steps are what speeds it up)")
     ap.add_argument("--surr-iters", type=int, default=1000,
                     help="surrogate_192: Adam iterations per group of 16 trajectories")
     ap.add_argument("--workload", default=WORKLOADS[0], choices=WORKLOADS,

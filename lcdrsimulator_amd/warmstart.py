@@ -149,7 +149,8 @@ def relax_warm(s: q.Solver, net: q.Surrogate, tol: float = 1e-6, every: int = 10
     raises neither F nor the residual is kept, else the field is left as it was) and 2 x ``every``
     plain steps with the energy checked after each half; converged when |dF/F| < tol between
     the two checks of a cycle, which no jump separates (the energy part of QSR.cu:3322-3335).
-    With ``tail_tol``, once |dF/F| over a cycle falls below it no more learned steps are taken.
+    With ``tail_tol``, a cycle whose predecessor's |dF/F| was below it takes no learned step (the
+    plain solver alone makes the last approach; a plateau of a coarsening run only pauses them).
     ``steps`` counts plain steps only."""
     n, att, acc, net_ms = 0, 0, 0, 0.0
     kept = []
@@ -173,8 +174,7 @@ def relax_warm(s: q.Solver, net: q.Surrogate, tol: float = 1e-6, every: int = 10
         if rel < tol:
             conv = True
             break
-        if tail_tol is not None and rel < tail_tol:
-            tail = True
+        tail = tail_tol is not None and rel < tail_tol
     return {"steps": n, "seconds": time.perf_counter() - t0, "converged": conv, "last_rel_dF": rel,
             "jumps_accepted": acc, "jumps_rejected": att - acc, "scales_kept": kept, "network_ms": net_ms}
 
@@ -187,8 +187,9 @@ def relax_batch(s: q.Solver, n_boxes: int, net: q.Surrogate | None = None, tol: 
     ``every`` steps.  Warm-started: the cycles of :func:`relax_warm`, with one checked learned step
     per active box per cycle (:meth:`Surrogate.apply_boxes`); with ``backoff`` a box whose step was
     refused sits out the next 1, 2, 4, ... cycles (reset when one is kept).  With ``tail_tol`` a box
-    whose |dF/F| over a cycle's second half fell below it takes no more learned steps: the plain
-    solver alone makes the last approach to the test.  ``steps`` per box counts plain steps."""
+    whose |dF/F| over the last cycle's second half was below it takes no learned step in the next
+    cycle: the plain solver alone makes the last approach to the test.  ``steps`` per box counts
+    plain steps."""
     active = list(range(n_boxes))
     steps = [0] * n_boxes
     prev = [None] * n_boxes
@@ -229,8 +230,8 @@ def relax_batch(s: q.Solver, n_boxes: int, net: q.Surrogate | None = None, tol: 
         still = []
         for b in active:
             ref = prev[b] if net is None else Fa[b]
-            if net is not None and tail_tol is not None and abs((F[b] - ref) / ref) < tail_tol:
-                tail[b] = True
+            if net is not None and tail_tol is not None:
+                tail[b] = abs((F[b] - ref) / ref) < tail_tol
             if ref is not None and abs((F[b] - ref) / ref) < tol:
                 steps[b] = done
                 s.set_active(b, False)
@@ -253,7 +254,7 @@ def zero_output_layer(net: q.Surrogate):
 
 
 def compare(p: q.Params, net: q.Surrogate, init_mode: int, seed: int = 12345, tol: float = 1e-6, every: int = 100,
-            max_steps: int = 200000, scales=(1.0, 0.5, 0.25), log=None) -> dict:
+            max_steps: int = 200000, scales=(1.0, 0.5, 0.25), tail_tol: float | None = None, log=None) -> dict:
     """Plain relaxation and warm-started relaxation of the same start on one context."""
     Q0 = q.init_field(p, init_mode, 0.5, 0.0, seed)
     out = {}
@@ -266,7 +267,7 @@ def compare(p: q.Params, net: q.Surrogate, init_mode: int, seed: int = 12345, to
             if name == "plain":
                 r = relax_plain(s, tol, every, max_steps)
             else:
-                r = relax_warm(s, net, tol, every, max_steps, scales)
+                r = relax_warm(s, net, tol, every, max_steps, scales, tail_tol)
             o = s.observables()
             r.update({"final_F": o["total"], "defects": int(o["defects"]), "plaquettes": int(o["plaquettes"])})
             out[name] = r
