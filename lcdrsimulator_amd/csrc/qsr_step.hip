@@ -243,7 +243,18 @@ __device__ __forceinline__ int4 load_row4(const RowMask* rows, const BoxDev& B, 
     return *reinterpret_cast<const int4*>(rows + B.rows_off + (long long)i * B.Ny + j);
 }
 
-__device__ __forceinline__ void decode_row4(int4 m, const BoxDev& B, int k, bool& evolve, bool& shell) {
+// The first 12 bytes of the record (the fourth word is padding), for kernels
+// short of registers; decoded by the same decode_row4.
+struct Row3 {
+    int x, y, z;
+};
+__device__ __forceinline__ Row3 load_row3(const RowMask* rows, const BoxDev& B, int i, int j) {
+    const int* r = reinterpret_cast<const int*>(rows + B.rows_off + (long long)i * B.Ny + j);
+    return Row3{r[0], r[1], r[2]};
+}
+
+template <class M>
+__device__ __forceinline__ void decode_row4(M m, const BoxDev& B, int k, bool& evolve, bool& shell) {
     if (B.flags & F_PERIODIC) {
         evolve = true;
         shell = false;
@@ -289,12 +300,19 @@ __global__ __launch_bounds__(TK* TJ, (!NOISE && BUF) ? 7 : 1) void step_l1_kerne
     const int j = t.z + tj, k = t.w + tk;
     const bool out_ok = per ? (j < Ny && k < Nz) : (j >= 1 && j <= Ny - 2 && k >= 1 && k <= Nz - 2);
     const int jc = wrap_or_clamp(j, Ny, per), kc = wrap_or_clamp(k, Nz, per);
-    const bool rowh = (tj == 0) || (tj == TJ - 1);
-    const bool colh = (tk == 0) || (tk == TK - 1);
-    const int jh = wrap_or_clamp(tj == 0 ? t.z - 1 : t.z + TJ, Ny, per);
-    const int kh = wrap_or_clamp(tk == 0 ? t.w - 1 : t.w + TK, Nz, per);
+    // Halo ring of the plane (rows j0-1 and j0+TJ, columns k0-1 and k0+TK, no
+    // corners): thread tid < 2 TK + 2 TJ owns ring point tid, so a thread holds at
+    // most one halo point and the row halos are loaded by whole coalesced waves.
+    const int tid = threadIdx.x;
+    const bool halo = tid < 2 * TK + 2 * TJ;
+    int hj = 0, hk = 0;  // ring point in tile coordinates (-1 .. TJ, -1 .. TK)
+    if (tid < TK) { hj = -1; hk = tid; }
+    else if (tid < 2 * TK) { hj = TJ; hk = tid - TK; }
+    else if (tid < 2 * TK + TJ) { hj = tid - 2 * TK; hk = -1; }
+    else if (halo) { hj = tid - 2 * TK - TJ; hk = TK; }
     const long long plane = (long long)Ny * Nz;
-    const long long o_own = (long long)jc * Nz + kc, o_row = (long long)jh * Nz + kc, o_col = (long long)jc * Nz + kh;
+    const long long o_own = (long long)jc * Nz + kc;
+    const long long o_halo = (long long)wrap_or_clamp(t.z + hj, Ny, per) * Nz + wrap_or_clamp(t.w + hk, Nz, per);
     const Field<BUF> F(qin, qout, cs, B.off);
 
     auto ld5 = [&](int p, long long o, float v[5]) {
@@ -304,37 +322,33 @@ __global__ __launch_bounds__(TK* TJ, (!NOISE && BUF) ? 7 : 1) void step_l1_kerne
     };
     // own queue Q(p-1), Q(p), Q(p+1) plus Q(p+2) in flight: every global load is
     // consumed one whole iteration after it is issued
-    float qm[5], qc[5], qn[5], hr[5] = {}, hcol[5] = {}, hrn[5] = {}, hcoln[5] = {};
+    float qm[5], qc[5], qn[5], hc[5] = {}, hn[5] = {};
     ld5(wrap_or_clamp(ib - 1, Nx, per), o_own, qm);
     ld5(ib, o_own, qc);
     ld5(wrap_or_clamp(ib + 1, Nx, per), o_own, qn);
-    if (rowh) ld5(ib, o_row, hr);
-    if (colh) ld5(ib, o_col, hcol);
+    if (halo) ld5(ib, o_halo, hc);
     const int lrow = (tj + 1) * W + tk + 1;
-    const int hrow = (tj == 0 ? 0 : (TJ + 1)) * W + tk + 1;
-    const int hcolx = (tj + 1) * W + (tk == 0 ? 0 : TK + 1);
+    const int lhalo = (hj + 1) * W + hk + 1;
 
     const bool pre_rows = out_ok && !per;  // rows with a mask record to prefetch
-    int4 mcur = make_int4(0, 0, 0, 0);
-    if (pre_rows) mcur = load_row4(rows, B, ib, j);
+    Row3 mcur{0, 0, 0};
+    if (pre_rows) mcur = load_row3(rows, B, ib, j);
     for (int p = ib; p < ie; ++p) {
         // next plane's halo and row mask, and own Q(p+2), a whole iteration ahead of use
         const bool more = p + 1 < ie;
         const int pn = wrap_or_clamp(p + 1, Nx, per);
-        int4 mnext = make_int4(0, 0, 0, 0);
+        Row3 mnext{0, 0, 0};
         float qf[5];
         if (more) {
-            if (pre_rows) mnext = load_row4(rows, B, p + 1, j);
+            if (pre_rows) mnext = load_row3(rows, B, p + 1, j);
             ld5(wrap_or_clamp(p + 2, Nx, per), o_own, qf);
-            if (rowh) ld5(pn, o_row, hrn);
-            if (colh) ld5(pn, o_col, hcoln);
+            if (halo) ld5(pn, o_halo, hn);
         }
         float* L = lds + (p & 1) * SLOT;
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
             L[c * PL + lrow] = qc[c];
-            if (rowh) L[c * PL + hrow] = hr[c];
-            if (colh) L[c * PL + hcolx] = hcol[c];
+            if (halo) L[c * PL + lhalo] = hc[c];
         }
         lds_barrier();
         if (out_ok) {
@@ -366,8 +380,7 @@ __global__ __launch_bounds__(TK* TJ, (!NOISE && BUF) ? 7 : 1) void step_l1_kerne
             qm[c] = qc[c];
             qc[c] = qn[c];
             qn[c] = more ? qf[c] : qn[c];
-            hr[c] = hrn[c];
-            hcol[c] = hcoln[c];
+            hc[c] = hn[c];
         }
         mcur = mnext;
     }
@@ -623,7 +636,10 @@ __global__ __launch_bounds__(TK* TJv, 3) void step_aniso_kernel(const float* __r
             dcol_ring(qsl, r0, r2, axs(t + 1), Dr);
             store_d(dsl, ld_ring, Dr);
         }
-        lds_barrier();
+        // No barrier here: (d) reads only the Dcol(t) slot, written in (b) of the
+        // previous plane before its closing barrier, and (b) writes the other D
+        // slot, whose last reader (d of the previous plane) is behind that same
+        // barrier.  One barrier per plane orders every LDS hand-over.
         // (d) output plane t
         if (out_ok) {
             bool evolve, shell;
