@@ -910,12 +910,17 @@ __global__ __launch_bounds__(TK* TJ) void jacobi_kernel(const float* __restrict_
     const int j = t.z + tj, k = t.w + tk;
     const bool out_ok = per ? (j < Ny && k < Nz) : (j >= 1 && j <= Ny - 2 && k >= 1 && k <= Nz - 2);
     const int jc = wrap_or_clamp(j, Ny, per), kc = wrap_or_clamp(k, Nz, per);
-    const bool rowh = (tj == 0) || (tj == TJ - 1);
-    const bool colh = (tk == 0) || (tk == TK - 1);
-    const int jh = wrap_or_clamp(tj == 0 ? t.z - 1 : t.z + TJ, Ny, per);
-    const int kh = wrap_or_clamp(tk == 0 ? t.w - 1 : t.w + TK, Nz, per);
+    // halo ring of the plane, one point per thread tid < 2 TK + 2 TJ (as in step_l1_kernel)
+    const int tid = threadIdx.x;
+    const bool halo = tid < 2 * TK + 2 * TJ;
+    int hj = 0, hk = 0;
+    if (tid < TK) { hj = -1; hk = tid; }
+    else if (tid < 2 * TK) { hj = TJ; hk = tid - TK; }
+    else if (tid < 2 * TK + TJ) { hj = tid - 2 * TK; hk = -1; }
+    else if (halo) { hj = tid - 2 * TK - TJ; hk = TK; }
     const long long plane = (long long)Ny * Nz;
-    const long long o_own = (long long)jc * Nz + kc, o_row = (long long)jh * Nz + kc, o_col = (long long)jc * Nz + kh;
+    const long long o_own = (long long)jc * Nz + kc;
+    const long long o_halo = (long long)wrap_or_clamp(t.z + hj, Ny, per) * Nz + wrap_or_clamp(t.w + hk, Nz, per);
     const Field<BUF> F(qin, qout, cs, B.off);    // iterate in, iterate out
     const Field<BUF> Fr(rhs, qout, cs, B.off);   // right-hand side
     const Field<BUF> Fo(qout, qout, cs, B.off);  // what the output buffer holds (iterate two sweeps back)
@@ -926,20 +931,18 @@ __global__ __launch_bounds__(TK* TJ) void jacobi_kernel(const float* __restrict_
 #pragma unroll
         for (int c = 0; c < 5; ++c) v[c] = A.ld(c, pb, o);
     };
-    float qm[5], qc[5], qn[5], hr[5] = {}, hcol[5] = {}, hrn[5] = {}, hcoln[5] = {};
+    float qm[5], qc[5], qn[5], hc[5] = {}, hn[5] = {};
     float rc[5] = {}, oc[5] = {};
     ld5(F, wrap_or_clamp(ib - 1, Nx, per), o_own, qm);
     ld5(F, ib, o_own, qc);
     ld5(F, wrap_or_clamp(ib + 1, Nx, per), o_own, qn);
-    if (rowh) ld5(F, ib, o_row, hr);
-    if (colh) ld5(F, ib, o_col, hcol);
+    if (halo) ld5(F, ib, o_halo, hc);
     if (out_ok) {
         ld5(Fr, ib, o_own, rc);
         if (detect) ld5(Fo, ib, o_own, oc);
     }
     const int lrow = (tj + 1) * W + tk + 1;
-    const int hrow = (tj == 0 ? 0 : (TJ + 1)) * W + tk + 1;
-    const int hcolx = (tj + 1) * W + (tk == 0 ? 0 : TK + 1);
+    const int lhalo = (hj + 1) * W + hk + 1;
     const bool pre_rows = out_ok && !per;
     int4 mcur = make_int4(0, 0, 0, 0);
     if (pre_rows) mcur = load_row4(rows, B, ib, j);
@@ -953,8 +956,7 @@ __global__ __launch_bounds__(TK* TJ) void jacobi_kernel(const float* __restrict_
         if (more) {
             if (pre_rows) mnext = load_row4(rows, B, p + 1, j);
             ld5(F, wrap_or_clamp(p + 2, Nx, per), o_own, qf);
-            if (rowh) ld5(F, pn, o_row, hrn);
-            if (colh) ld5(F, pn, o_col, hcoln);
+            if (halo) ld5(F, pn, o_halo, hn);
             if (out_ok) {
                 ld5(Fr, pn, o_own, rn);
                 if (detect) ld5(Fo, pn, o_own, on);
@@ -964,8 +966,7 @@ __global__ __launch_bounds__(TK* TJ) void jacobi_kernel(const float* __restrict_
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
             L[c * PL + lrow] = qc[c];
-            if (rowh) L[c * PL + hrow] = hr[c];
-            if (colh) L[c * PL + hcolx] = hcol[c];
+            if (halo) L[c * PL + lhalo] = hc[c];
         }
         lds_barrier();
         if (out_ok) {
@@ -995,8 +996,7 @@ __global__ __launch_bounds__(TK* TJ) void jacobi_kernel(const float* __restrict_
             qm[c] = qc[c];
             qc[c] = qn[c];
             qn[c] = more ? qf[c] : qn[c];
-            hr[c] = hrn[c];
-            hcol[c] = hcoln[c];
+            hc[c] = hn[c];
             rc[c] = rn[c];
             oc[c] = on[c];
         }
