@@ -155,8 +155,9 @@ int qsr_set_iteration(void* ctx, int box, int64_t iteration);
 int qsr_last_step_ms(void* ctx, double* ms_per_step);
 /* Autotuner: times every stencil-kernel variant and planes-per-tile choice
  * for trial_steps launches on the resident field (the trial launches write the
- * spare buffer only, so the state and iteration count do not change) and keeps
- * the fastest.  Results are bit-identical across choices. */
+ * spare buffer only, so the state and iteration count do not change), three
+ * sweeps over all choices, and keeps the one whose fastest sweep is fastest
+ * (best_ms).  Results are bit-identical across choices. */
 int qsr_autotune(void* ctx, int trial_steps, int* best_variant, int* best_ch, double* best_ms);
 /* Hold (active = 0) or resume one box of a batch: a held box takes no part in
  * later steps (its tiles leave the launch, its iteration count and field stay
