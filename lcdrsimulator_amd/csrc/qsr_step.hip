@@ -391,10 +391,12 @@ __global__ __launch_bounds__(TK* TJ, (!NOISE && BUF) ? 7 : 1) void step_l1_kerne
 //   * one "lead" plane (i = t+1) in LDS with an in-plane halo of 2,
 //     double-buffered, from which every thread takes its own first derivatives
 //     d_y Q, d_z Q, second differences and the mixed d_y d_z Q;
-//   * per-thread register queues along i for Q(t-1..t+2), d_yQ/d_zQ(t-1..t+1)
+//   * per-thread register queues along i for Q(t-1..t+2), d_yQ/d_zQ(t..t+1)
 //     and Dcol(t-1..t+1), so x-derivatives need no LDS: the mixed
-//     d_x d_y Q = (d_yQ(t+1) - d_yQ(t-1)) / (2 dx) is exactly the reference's
-//     4-point formula (QSR.cu:583-588);
+//     d_x d_y Q = (d_yQ(t+1) - d_yQ(t-1)) / (2 dx) is the reference's 4-point
+//     formula (QSR.cu:583-588); the L3 term needs it only inside
+//     Q_xy d_x d_y Q + Q_xz d_x d_z Q, whose d_yQ/d_zQ(t-1) half is folded one
+//     plane early (pxm), so no register holds d_yQ/d_zQ(t-1);
 //   * Dcol = d_k Q_kj (QSR.cu:325-371) in an LDS plane with a halo of 1: own
 //     points reuse their first derivatives, the halo ring is computed by ring
 //     threads that keep a 3-component x-queue of their ring point.
@@ -573,7 +575,10 @@ __global__ __launch_bounds__(TK* TJv, 3) void step_aniso_kernel(const float* __r
 
     // ---- prologue: lead planes ib-1 and ib --------------------------------
     float q0[5], q1[5], q2[5], q3[5];           // own Q(t-1), Q(t), Q(t+1), Q(t+2)
-    float dy0[5], dz0[5], dy1[5], dz1[5], dy2[5], dz2[5];
+    float dy1[5], dz1[5], dy2[5], dz2[5];
+    // L3 only: (2 hx) (Q_xy(t) d_yQ(t-1) + Q_xz(t) d_zQ(t-1)), the t-1 half of the mixed
+    // x-y and x-z terms, folded one plane early so that d_yQ/d_zQ(t-1) need no registers
+    float pxm[5] = {};
     float syy1[5], szz1[5], dyz1[5];            // in-plane second derivatives of plane t
     float D0[3], D1[3], D2[3];                  // own Dcol(t-1), Dcol(t), Dcol(t+1)
     float r0[3] = {}, r1[3] = {}, r2[3] = {};   // ring comps xx,xy,xz at t, t+1, t+2
@@ -594,8 +599,16 @@ __global__ __launch_bounds__(TK* TJv, 3) void step_aniso_kernel(const float* __r
         fill_slot(0, ib - 1, q0);
         fill_slot(1, ib, q1);
         lds_barrier();
-        own_d1(0, dy0, dz0);
-        dcol_own(qm2, q1, axs(ib - 1), dy0, dz0, D0);
+        {
+            float dym[5], dzm[5];
+            own_d1(0, dym, dzm);
+            dcol_own(qm2, q1, axs(ib - 1), dym, dzm, D0);
+            if constexpr (hasL3) {
+                const float cy = 2.0f * q1[1] * B.hx, cz = 2.0f * q1[2] * B.hx;
+#pragma unroll
+                for (int c = 0; c < 5; ++c) pxm[c] = cy * dym[c] + cz * dzm[c];
+            }
+        }
         own_d1(1, dy1, dz1);
         own_d2(1, syy1, szz1, dyz1);
         dcol_own(q0, q2, axs(ib), dy1, dz1, D1);
@@ -639,7 +652,10 @@ __global__ __launch_bounds__(TK* TJv, 3) void step_aniso_kernel(const float* __r
         // No barrier here: (d) reads only the Dcol(t) slot, written in (b) of the
         // previous plane before its closing barrier, and (b) writes the other D
         // slot, whose last reader (d of the previous plane) is behind that same
-        // barrier.  One barrier per plane orders every LDS hand-over.
+        // barrier.  One barrier per plane orders every LDS hand-over.  The
+        // scheduling fence keeps the compiler from interleaving (b) and (d),
+        // which would hold both passes' values at once and spill.
+        __builtin_amdgcn_sched_barrier(0);
         // (d) output plane t
         if (out_ok) {
             bool evolve, shell;
@@ -699,7 +715,7 @@ __global__ __launch_bounds__(TK* TJv, 3) void step_aniso_kernel(const float* __r
 #pragma unroll
                     for (int c = 0; c < 5; ++c) {
                         const float contract = cxx * sx_[c] + cyy * syy1[c] + czz * szz1[c] +
-                                               cxy * (dy2[c] - dy0[c]) + cxz * (dz2[c] - dz0[c]) + cyz * dyz1[c];
+                                               (cxy * dy2[c] + cxz * dz2[c] - pxm[c]) + cyz * dyz1[c];
                         Bc[c] = D1[0] * d[c][0] + D1[1] * d[c][1] + D1[2] * d[c][2] + contract;
                     }
                     const float hl = 0.5f * B.mL3;
@@ -724,10 +740,15 @@ __global__ __launch_bounds__(TK* TJv, 3) void step_aniso_kernel(const float* __r
         }
         // in-plane second derivatives of plane t+1 while its slot is resident
         own_d2(qsl, syy1, szz1, dyz1);
+        // the t-1 half of plane t+1's mixed terms (Q(t+1) is q2, d_yQ/d_zQ(t) are dy1/dz1)
+        if constexpr (hasL3) {
+            const float cy = 2.0f * q2[1] * B.hx, cz = 2.0f * q2[2] * B.hx;
+#pragma unroll
+            for (int c = 0; c < 5; ++c) pxm[c] = cy * dy1[c] + cz * dz1[c];
+        }
         // rotate the queues: t+1 becomes t
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
-            dy0[c] = dy1[c]; dz0[c] = dz1[c];
             dy1[c] = dy2[c]; dz1[c] = dz2[c];
             q0[c] = q1[c]; q1[c] = q2[c]; q2[c] = q3[c]; q3[c] = q4[c];
         }
