@@ -199,6 +199,16 @@ def chempot(Q, grid: Grid, p: Phys, with_l3=True):
     return mu
 
 
+def update(Q, mu, grid: Grid, g: Geom, shell, dt, gamma, W, cap=0.0):
+    """One pass of the explicit update alone (orc_update, QSR.cu:697-732) on a copy of Q."""
+    Q = _q(Q).copy()
+    mu = _q(mu)
+    shell = np.ascontiguousarray(shell, dtype=np.uint8)
+    lib().orc_update(_p(Q), _p(mu), *grid.dims, C.c_double(dt), shell.ctypes.data_as(_u8p), C.byref(g),
+                     C.c_double(gamma), C.c_double(W), C.c_double(cap))
+    return Q
+
+
 class DropletOracle:
     """Stateful FP64 droplet solver in the QSR.cu step order (oracle)."""
 

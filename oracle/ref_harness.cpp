@@ -79,6 +79,18 @@ void ref_nematic(const double* Q, size_t n, double* out4) {
     }
 }
 
+// QSR.cpp:861-891 updateQTensor on the given field, in place.  shell is the
+// caller's mask in the reference index order (1 = shell voxel).
+void ref_update(double* Q, const double* mu, int Nx, int Ny, int Nz, double dt, double gamma, double W,
+                const unsigned char* shell) {
+    const size_t n = (size_t)Nx * Ny * Nz;
+    std::vector<QTensor> q = to_vec(Q, n), m = to_vec(mu, n);
+    std::vector<bool> sh(n);
+    for (size_t t = 0; t < n; ++t) sh[t] = shell[t] != 0;
+    updateQTensor(q, m, Nx, Ny, Nz, dt, sh, gamma, W);
+    from_vec(q, Q);
+}
+
 // Time `steps` reference CPU iterations (QSR.cpp loop body: laplacian, chemical
 // potential incl. L2/L3, weak anchoring, update, strong BC) on the given field.
 // The field is advanced in place.  Returns wall seconds.

@@ -4,8 +4,11 @@ Runs in the build container only (it needs oracle/_ref/libqsr_ref.so, which
 oracle/build_ref.sh compiles from the reference's QSR.cpp).  Inputs are seeded
 synthetic fields; outputs are what QSR.cpp's computeLaplacian (QSR.cpp:178-282),
 computeChemicalPotential (QSR.cpp:559-826), computeEnergyComponents
-(QSR.cpp:410-557) and calculateNematicField (QSR.cpp:10-61) return for them.
-The .npz holds data only (inputs and expected outputs).
+(QSR.cpp:410-557), calculateNematicField (QSR.cpp:10-61) and updateQTensor
+(QSR.cpp:861-891) return for them.
+The .npz holds data only (inputs and expected outputs).  computeEnergyComponents
+sums with OpenMP reductions, so regenerated energies may differ from the
+committed ones in the last bit; every other array is reproduced exactly.
 
     python tests/golden/make_golden.py
 """
@@ -26,6 +29,17 @@ GRIDS = [((9, 8, 7), (1e-8, 1e-8, 1e-8)), ((11, 10, 12), (1e-8, 1.1e-8, 0.9e-8))
 # (kappa, L1, L2, L3)
 ELASTIC = [(6.5e-12, 0, 0, 0), (0, 6e-12, 0, 0), (0, 6e-12, 3e-12, 0), (0, 6e-12, 3e-12, 1e-12)]
 BULK = dict(a=0.044e6, b=1.413e6, c=1.153e6, T=300.0, T_star=307.0)
+# (dims, W): strong (W = 0, shell skipped) and weak anchoring (evolve band R + 1.5)
+UPDATE = [((12, 11, 10), 0.0), ((12, 11, 10), 1e-4), ((9, 13, 11), 0.0)]
+UPDATE_DT, UPDATE_GAMMA = 1.60372e-10, 0.1
+
+
+def reference_shell(dims):
+    """QSR.cpp's shell mask: |r - R| < 1 about (N/2), R = min(N)/2 (QSR.cpp:1088-1102)."""
+    Nx, Ny, Nz = dims
+    I, J, K = np.meshgrid(np.arange(Nx), np.arange(Ny), np.arange(Nz), indexing="ij")
+    r = np.sqrt((I - Nx / 2.0) ** 2 + (J - Ny / 2.0) ** 2 + (K - Nz / 2.0) ** 2)
+    return (np.abs(r - min(dims) / 2.0) < 1.0).astype(np.uint8)
 
 
 def nematic_inputs(rng):
@@ -78,6 +92,22 @@ def main():
     R.ref_nematic(nq.ctypes.data_as(dp), C.c_size_t(nq.shape[0]), nf.ctypes.data_as(dp))
     data["nematic_in"] = nq
     data["nematic_out"] = nf
+    # updateQTensor: Q -= (dt/gamma) mu inside the reference's evolve mask, strong shell skipped.
+    # Drawn after every other input so the earlier vectors do not change.
+    for ui, (dims, W) in enumerate(UPDATE):
+        Q = rng.normal(0, 0.3, dims + (5,))
+        mu = rng.normal(0, 1e6, dims + (5,))
+        shell = reference_shell(dims)
+        out = Q.copy()
+        R.ref_update(out.ctypes.data_as(dp), mu.ctypes.data_as(dp), *dims, C.c_double(UPDATE_DT),
+                     C.c_double(UPDATE_GAMMA), C.c_double(W), shell.ctypes.data_as(C.POINTER(C.c_ubyte)))
+        data[f"u{ui}_dims"] = np.array(dims)
+        data[f"u{ui}_W"] = np.array(W)
+        data[f"u{ui}_Q"] = Q
+        data[f"u{ui}_mu"] = mu
+        data[f"u{ui}_shell"] = shell
+        data[f"u{ui}_out"] = out
+    data["update_dt_gamma"] = np.array([UPDATE_DT, UPDATE_GAMMA])
     np.savez_compressed(OUT, **data)
     print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes)")
 

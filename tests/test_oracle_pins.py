@@ -516,3 +516,199 @@ def test_xi_proxy_linear_twist():
     assert ne.value == nx_edges + ny_edges
     mean_sq = (2 * c) ** 2 * nx_edges / (nx_edges + ny_edges)
     assert xi.value == pytest.approx(1 / math.sqrt(mean_sq), rel=1e-5)
+
+
+# ---- the explicit update and the periodic step --------------------------------------------
+# Q <- Q - (dt/gamma) mu (QSR.cu:697-732, QSR.cpp:861-891) and the fused periodic step
+# (KZM_bulk_ldg.cu:351-469) are pinned by QSR.cpp's own updateQTensor and by closed forms whose
+# every term is fixed: a wrong mobility (dt*gamma for dt/gamma), a wrong sign or factor of the
+# kappa / L1 Laplacian term, a transposed direction or spacing, or a dropped bulk term fails them.
+
+def _ref_radius_mask(dims, W):
+    """QSR.cpp's evolve mask: r <= R (strong) or r <= R + 1.5 (weak) about N/2 (QSR.cpp:875-877)."""
+    Nx, Ny, Nz = dims
+    I, J, K = np.meshgrid(np.arange(Nx), np.arange(Ny), np.arange(Nz), indexing="ij")
+    r = np.sqrt((I - Nx / 2.0) ** 2 + (J - Ny / 2.0) ** 2 + (K - Nz / 2.0) ** 2)
+    return r, r <= (min(dims) / 2.0 + (1.5 if W > 0 else 0.0))
+
+
+@pytest.mark.parametrize("ui", [0, 1, 2])
+def test_golden_update_matches_qsr_cpp(gold, ui):
+    """orc_update equals QSR.cpp's updateQTensor bit for bit on every voxel that both evolve masks
+    contain.  The two codes place the droplet differently (centre (N-1)/2 and sd <= band here, N/2 and
+    r <= R there), so voxels in only one mask lie within two voxels of the surface; every voxel outside
+    both, and every strong-shell voxel, is left untouched by both."""
+    dims = tuple(int(v) for v in gold[f"u{ui}_dims"])
+    W = float(gold[f"u{ui}_W"])
+    dt, gamma = gold["update_dt_gamma"]
+    Q, mu, shell, ref = gold[f"u{ui}_Q"], gold[f"u{ui}_mu"], gold[f"u{ui}_shell"], gold[f"u{ui}_out"]
+    g = O.Grid(*dims)
+    out = O.update(Q, mu, g, O.geometry(g), shell, dt, gamma, W)
+    ch_ref = (ref != Q).any(-1)
+    ch_orc = (out != Q).any(-1)
+    both = ch_ref & ch_orc
+    assert both.sum() >= 100
+    assert np.array_equal(out[both], ref[both])
+    r, _ = _ref_radius_mask(dims, W)
+    R0 = min(dims) / 2.0
+    assert np.all(np.abs(r[ch_ref ^ ch_orc] - R0) <= 2.6)
+    assert not (ch_ref | ch_orc)[shell.astype(bool)].any() if W == 0 else True
+    # the reference's mask, restated, is what changed there
+    _, inside = _ref_radius_mask(dims, W)
+    assert np.array_equal(ch_ref, inside & ~(shell.astype(bool) if W == 0 else False))
+
+
+def test_live_reference_update_when_built():
+    R = O.ref_lib()
+    if R is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    import ctypes as C
+    rng = np.random.default_rng(17)
+    dims = (10, 12, 11)
+    g = O.Grid(*dims)
+    Q = rng.normal(0, 0.2, dims + (5,))
+    mu = rng.normal(0, 3e5, dims + (5,))
+    shell = np.zeros(dims, dtype=np.uint8)
+    ref = Q.copy()
+    dp = C.POINTER(C.c_double)
+    R.ref_update(ref.ctypes.data_as(dp), mu.ctypes.data_as(dp), *dims, C.c_double(1.3e-10), C.c_double(0.07),
+                 C.c_double(2e-4), shell.ctypes.data_as(C.POINTER(C.c_ubyte)))
+    out = O.update(Q, mu, g, O.geometry(g), shell, 1.3e-10, 0.07, 2e-4)
+    both = (ref != Q).any(-1) & (out != Q).any(-1)
+    assert both.sum() > 0.15 * both.size and np.array_equal(out[both], ref[both])
+
+
+def _uniaxial(S, n):
+    n = np.asarray(n, float) / np.linalg.norm(n)
+    M = S * (np.outer(n, n) - np.eye(3) / 3)
+    return np.array([M[0, 0], M[0, 1], M[0, 2], M[1, 1], M[1, 2]])
+
+
+@pytest.mark.parametrize("T,S0,W", [(300.0, 0.5, 0.0), (300.0, 0.5, 1e-4), (312.0, 1e-6, 0.0), (295.0, 0.95, 1e-4)])
+def test_uniform_uniaxial_field_relaxes_by_closed_form(T, S0, W):
+    """A uniform uniaxial field S0 (nn - I/3) has no gradients, and for it
+    Q^2 - I trQ^2/3 = (S0/3) Q and trQ^2 = 2 S0^2/3, so the bulk molecular field is
+    (A - B S0/3 + 2 C S0^2/3) Q with A = 3a(T - T*), B = 4.5 b, C = 4.5 c (mode 1, QSR.cu:1363-1378).
+    One explicit step therefore scales Q by 1 - (dt/gamma)(A - B S0/3 + 2 C S0^2/3) at every evolving
+    voxel off the shell, whatever L1/L2/L3 are; for T > T* and S0 -> 0 that is 1 - (dt/gamma) A."""
+    g = O.Grid(14, 13, 12)
+    L1, L2, L3 = 6e-12, 3e-12, 1e-12
+    p = O.phys(T=T, T_star=307.0, kappa=0.0, L1=L1, L2=L2, L3=L3, W=W)
+    d = O.DropletOracle(g, p, gamma=0.1)
+    dt = 1.6e-10
+    q0 = _uniaxial(S0, (0.3, -0.5, 0.8))
+    Q = np.broadcast_to(q0, g.dims + (5,)).copy()
+    Q1 = d.step(Q, 0.7, dt)
+    A, B, Cc = 3 * A5 * (T - 307.0), 4.5 * B5, 4.5 * C5
+    factor = 1.0 - (dt / 0.1) * (A - B * S0 / 3 + 2 * Cc * S0 ** 2 / 3)
+    sd = np.array([[[O.sample(d.g, i, j, k)[4] for k in range(g.Nz)] for j in range(g.Ny)] for i in range(g.Nx)])
+    band = d.g.shell_half_thickness if W > 0 else 0.0
+    evolve = (sd <= band) & ~d.shell.astype(bool)
+    assert evolve.sum() > 200
+    want = q0 * factor
+    dec_got = Q[evolve] - Q1[evolve]
+    dec_want = np.broadcast_to(q0 - want, dec_got.shape)
+    assert np.abs(dec_got - dec_want).max() <= 1e-9 * np.abs(dec_want).max()
+    if T > 307.0:  # small-amplitude limit: the linear decay by (dt/gamma) A
+        assert np.allclose(Q1[evolve], q0 * (1 - (dt / 0.1) * A), rtol=1e-5, atol=0)
+    outside = sd > band
+    assert np.array_equal(Q1[outside & ~d.shell.astype(bool)], Q[outside & ~d.shell.astype(bool)])
+
+
+def test_uniform_equilibrium_field_is_stationary():
+    """At S0 = S_eq the bracket A - B S/3 + 2 C S^2/3 vanishes (S_eq solves 2CS^2 - BS + 3A = 0),
+    so one step leaves every interior voxel at rest to rounding."""
+    T = 300.0
+    A, B, Cc = O.bulk_ABC(A5, B5, C5, T, 307.0, 1)
+    S = O.S_eq(A, B, Cc)
+    g = O.Grid(12, 12, 12)
+    d = O.DropletOracle(g, O.phys(T=T, T_star=307.0, kappa=0.0, L1=6e-12))
+    q0 = _uniaxial(S, (1, 2, 2))
+    Q1 = d.step(np.broadcast_to(q0, g.dims + (5,)).copy(), S, 1.6e-10)
+    assert np.abs(Q1[6, 6, 6] - q0).max() < 1e-14
+
+
+def _wave_expect(comp, q, lam, A, B, Cc, kap, m):
+    """One explicit step of a single-component field q(x) whose discrete Laplacian is -lam q.
+
+    Off-diagonal component (a, b), third axis c: Q^2 - I trQ^2/3 is diag with q^2/3 at a and b and
+    -2q^2/3 at c, trQ^2 = 2q^2, so mu_ab = (A + 2C q^2 + kap lam) q, mu_aa = mu_bb = -B q^2/3,
+    mu_cc = 2B q^2/3 (traceless, no projection).  Diagonal Qxx = q (Qzz = -q): mu_xx =
+    (A + 2C q^2 + kap lam) q - B q^2/3, mu_yy = 2B q^2/3.  Returns the 5 stored components after
+    Q -= m mu."""
+    out = np.zeros(q.shape + (5,))
+    lin = (A + 2 * Cc * q ** 2 + kap * lam) * q
+    if comp == 0:  # Qxx = q, Qyy = 0, Qzz = -q
+        out[..., 0] = q - m * (lin - B * q ** 2 / 3)
+        out[..., 3] = -m * (2 * B * q ** 2 / 3)
+        return out
+    axes = {1: (0, 1, 2), 2: (0, 2, 1), 4: (1, 2, 0)}[comp]
+    diag = np.zeros(q.shape + (3,))
+    diag[..., axes[0]] = -B * q ** 2 / 3
+    diag[..., axes[1]] = -B * q ** 2 / 3
+    diag[..., axes[2]] = 2 * B * q ** 2 / 3
+    out[..., comp] = q - m * lin
+    out[..., 0] = -m * diag[..., 0]
+    out[..., 3] = -m * diag[..., 1]
+    return out
+
+
+WAVES = [(1, 0), (2, 1), (4, 2), (0, 2), (0, 0)]  # (component, direction)
+
+
+@pytest.mark.parametrize("comp,axis", WAVES)
+@pytest.mark.parametrize("T", [310.0, 300.0])
+def test_periodic_plane_wave_closed_form(comp, axis, T):
+    """A periodic plane wave eps sin(2 pi x_axis / N_axis) in one component: its discrete Laplacian is
+    -4 sin^2(pi/N)/h^2 times the wave along that axis only, so one fused periodic step
+    (KZM_bulk_ldg.cu:351-469) gives every component in closed form (see _wave_expect).  Spacings
+    differ per axis, so a transposed direction or spacing fails as well."""
+    g = O.Grid(16, 12, 10, 2e-9, 3e-9, 2.5e-9)
+    A, B, Cc = O.bulk_ABC(A5, B5, C5, T, 307.0, 1)
+    kap, gamma, dt = 6e-12, 0.1, 1.6e-10
+    N = g.dims[axis]
+    h = (g.dx, g.dy, g.dz)[axis]
+    x = np.arange(N)
+    wave = 1e-3 * np.sin(2 * np.pi * x / N + 0.3)
+    shape = [1, 1, 1]
+    shape[axis] = N
+    q = np.broadcast_to(wave.reshape(shape), g.dims).copy()
+    Q = np.zeros(g.dims + (5,))
+    Q[..., comp] = q
+    lam = 4 * np.sin(np.pi / N) ** 2 / h ** 2
+    out = O.periodic_step(Q, g, A, B, Cc, kap, gamma, dt)
+    want = _wave_expect(comp, q, lam, A, B, Cc, kap, dt / gamma)
+    dec_got, dec_want = Q - out, Q - want
+    assert np.abs(dec_got - dec_want).max() <= 1e-9 * np.abs(dec_want).max()
+    # the elastic part is a sizeable share of the decrement, so its sign and factor are resolved
+    assert kap * lam > 0.05 * abs(A)
+
+
+@pytest.mark.parametrize("comp,axis", WAVES[:4])
+def test_droplet_step_plane_wave_closed_form(comp, axis):
+    """The same closed form holds for the droplet step (orc_step, L1 only: mu carries -L1 lap Q) at
+    every evolving voxel away from the box faces and the strong shell, where the 7-point Laplacian of
+    the wave is -lam times the wave whatever the boundary does."""
+    g = O.Grid(20, 18, 16, 2e-9, 3e-9, 2.5e-9)
+    T, L1, gamma, dt = 310.0, 6e-12, 0.1, 1.6e-10
+    p = O.phys(T=T, T_star=307.0, kappa=0.0, L1=L1)
+    d = O.DropletOracle(g, p, gamma=gamma)
+    A, B, Cc = O.bulk_ABC(A5, B5, C5, T, 307.0, 1)
+    N = g.dims[axis]
+    h = (g.dx, g.dy, g.dz)[axis]
+    wave = 1e-3 * np.sin(2 * np.pi * np.arange(N) / 7.0 + 0.3)
+    shape = [1, 1, 1]
+    shape[axis] = N
+    q = np.broadcast_to(wave.reshape(shape), g.dims).copy()
+    Q = np.zeros(g.dims + (5,))
+    Q[..., comp] = q
+    lam = 4 * np.sin(np.pi / 7.0) ** 2 / h ** 2
+    out = d.step(Q, 0.0, dt)
+    want = _wave_expect(comp, q, lam, A, B, Cc, L1, dt / gamma)
+    sd = np.array([[[O.sample(d.g, i, j, k)[4] for k in range(g.Nz)] for j in range(g.Ny)] for i in range(g.Nx)])
+    inner = np.zeros(g.dims, bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    sel = inner & (sd <= 0) & ~d.shell.astype(bool)
+    assert sel.sum() > 300
+    dec_got, dec_want = (Q - out)[sel], (Q - want)[sel]
+    assert np.abs(dec_got - dec_want).max() <= 1e-9 * np.abs(dec_want).max()
