@@ -9,11 +9,19 @@ window; seeded random-director start (S0 = 0.5, seed 12345).  A "step" is one
 explicit-Euler iteration of the whole box (fused stencil launch + box-face
 launch).  Inputs are resident in HBM when the timed region starts.
 
-Prints one JSON line: value = box voxel-updates/s; roofline = algorithmic
-bytes of the fused stencil (40 B per voxel of the launched tiles: read 5 FP32,
-write 5 FP32) / its average launch time from hipEvents on the context stream;
-cpu_baseline = the reference's own QSR.cpp step functions (oracle/_ref,
-OpenMP) timed on this host on a bounded sample of the same box.
+Prints one JSON line:
+* value = droplet voxel-updates/s (the voxels a step changes, over the whole
+  timed step); the box-equivalent and launched-tile rates are separate fields;
+* roofline = algorithmic bytes of the fused stencil (40 B per droplet voxel:
+  read 5 FP32, write 5 FP32) / its average launch time from hipEvents on the
+  context stream, so value x 40 B / 8 TB/s ~ roofline.frac; tile_frac counts
+  the launched tiles' voxels instead; traffic and the compute figure come from
+  the committed rocprofv3 summaries named beside them;
+* config.time_to_solution = the second half of the metric: the 64 quench-ladder
+  droplets relaxed to |dF/F| < 1e-6 batched and warm-started against one plain
+  context per droplet;
+* cpu_baseline = the reference's own QSR.cpp step functions (oracle/_ref,
+  OpenMP) timed on this host on a bounded sample of the same box.
 """
 from __future__ import annotations
 
@@ -39,6 +47,7 @@ def log(msg):
 WORKLOADS = ("droplet_l1l2l3_256", "periodic_noise_256", "periodic_noise_256x8", "droplet_l1_64", "droplet_l1_256",
              "ladder_64", "ladder_relax_64", "surrogate_192")
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+FP32_VALU_PEAK_TFLOPS = 78.6  # 256 CUs x 4 SIMD x 16 lanes x 2 flop x 2.4 GHz: FP32 without packed issue
 NET_FLOP_PER_VOXEL = 72576.0  # 2 * 27 * (5*32 + 32*32 + 32*5) multiply-adds of the 5-32-32-5 network
 
 
@@ -260,18 +269,23 @@ def run_ladder(q, args):
     return 0
 
 
-def run_ladder_relax(q, args):
+STOP_RULE = ("energy part of QSR.cu:3322-3335: |dF/F| < tol between FP64 energy checks `every` steps apart; "
+             "the defect-density and 0.5 tau_align gates are not applied")
+
+
+def ladder_relax(q, args, surr_traj=None, trained=True):
     """Time to |dF/F| < relax_tol (FP64 energy of every droplet every relax_every steps) for the 64 ladder
     droplets relaxed at T = 300 K from radial starts (S0 = 0.5): one batched context that retires each droplet
     once its test passes (qsr_set_box_active), plain and warm-started (one checked learned step per active
-    droplet per cycle, qsr_surr_apply_boxes; the network trained on the warm-start training set, and the zero
-    network beside it), against one context per droplet relaxed plainly in turn; value = sequential plain
-    time / batched warm-started time."""
+    droplet per cycle, qsr_surr_apply_boxes; the network trained on ``surr_traj`` trajectories of the warm-start
+    training set, and the zero network beside it), against one context per droplet relaxed plainly in turn.
+    Returns the figures of every leg."""
     import dataclasses
 
     import numpy as np
 
     from lcdrsimulator_amd import warmstart as ws
+    surr_traj = args.surr_traj if surr_traj is None else surr_traj
     params = []
     for p in ladder_params(q):
         w = q.dt_estimate(p.dx, p.dy, p.dz, p.gamma, p.a, p.b, p.c, p.T_star, 1, 0.0, p.L1, 0, 0, math.nan, 0.5,
@@ -295,15 +309,19 @@ def run_ladder_relax(q, args):
         final_b = [s.observables(box=b, want_xi=False) for b in range(n)]
     log(f"ladder relaxation batched: {t_batched:.3f} s, {done} steps, {n_conv} of {n} converged")
     # batched and warm-started: the trained network and the zero network
-    warm = {}
+    warm, tr = {}, None
     with q.Surrogate(seed=3) as net, q.Surrogate(seed=None) as zero:
-        ws.zero_output_layer(net)
-        net.set_input(q.SURR_INPUT_INCREMENT, 500.0)
-        tr = ws.train(net, ws.training_set(args.surr_traj), gap=500, n_snap=13, stride=250, group=16,
-                      iters_per_group=args.surr_iters, lr=5e-4, increment=True, log=log)
+        nets = []
+        if trained and surr_traj > 0:
+            ws.zero_output_layer(net)
+            net.set_input(q.SURR_INPUT_INCREMENT, 500.0)
+            tr = ws.train(net, ws.training_set(surr_traj), gap=500, n_snap=13, stride=250, group=16,
+                          iters_per_group=args.surr_iters, lr=5e-4, increment=True, log=log)
+            nets.append(("trained", net))
         zero.params = np.zeros(zero.nparams, dtype=np.float32)
         zero.set_input(q.SURR_INPUT_INCREMENT, 500.0)
-        for name, nt in (("trained", net), ("zero", zero)):
+        nets.append(("zero", zero))
+        for name, nt in nets:
             with q.Solver(params) as s:
                 for b, Q in enumerate(fields):
                     s.upload(Q, box=b)
@@ -336,6 +354,15 @@ def run_ladder_relax(q, args):
         same_defects += int(o["defects"] == final_b[b]["defects"])
         max_rel_dF = max(max_rel_dF, abs(o["total"] - final_b[b]["total"]) / abs(o["total"]))
     log(f"ladder relaxation one context per droplet: {t_seq:.3f} s")
+
+    def worst(r):
+        fin = r["final"]
+        rel = [(a["total"] - c["total"]) / abs(c["total"]) for a, c in zip(fin, final_b)]
+        b = int(np.argmax(np.abs(rel)))
+        return {"droplet": b, "radius": params[b].radius_index, "W": params[b].W,
+                "F_plain": final_b[b]["total"], "F_warm": fin[b]["total"], "rel": rel[b],
+                "steps_plain": steps_b[b], "steps_warm": r["steps"][b]}
+
     def vs_plain(r):
         fin = r["final"]
         return {"seconds": r["seconds"], "converged": r["converged"], "steps_per_droplet": r["steps"],
@@ -347,32 +374,34 @@ def run_ladder_relax(q, args):
                 "max_rel_dF_vs_batched_plain": max(abs(a["total"] - c["total"]) / abs(c["total"])
                                                    for a, c in zip(fin, final_b)),
                 "worst_droplet": worst(r)}
+    out = {"workload": "relaxation of the 64 quench-ladder droplets (R=16..48 in (2R+8)^3 boxes, strong/weak "
+                       "anchoring, L1=6e-12) at T=300 K from radial starts to |dF/F|<" f"{tol:g} between checks every "
+                       f"{every} steps", "stop_rule": STOP_RULE, "droplets": n,
+           "sequential_plain_s": t_seq, "batched_plain_s": t_batched, "autotune": tune,
+           "converged_batched": n_conv, "steps_per_droplet_batched": steps_b,
+           "steps_per_droplet_sequential": steps_s, "total_steps_batched": done,
+           "same_steps": int(sum(a == c for a, c in zip(steps_b, steps_s))),
+           "same_defects": same_defects, "max_rel_dF_final": max_rel_dF,
+           "batched_plain_speedup": t_seq / t_batched}
+    for name, r in warm.items():
+        out["warm_trained" if name == "trained" else "warm_zero_network"] = vs_plain(r)
+    if tr is not None:
+        out["training"] = {k: v for k, v in tr.items() if k != "loss"}
+    return out
 
-    def worst(r):
-        fin = r["final"]
-        rel = [(a["total"] - c["total"]) / abs(c["total"]) for a, c in zip(fin, final_b)]
-        b = int(np.argmax(np.abs(rel)))
-        return {"droplet": b, "radius": params[b].radius_index, "W": params[b].W,
-                "F_plain": final_b[b]["total"], "F_warm": fin[b]["total"], "rel": rel[b],
-                "steps_plain": steps_b[b], "steps_warm": r["steps"][b]}
-    tw = warm["trained"]["seconds"]
+
+def run_ladder_relax(q, args):
+    """The ladder time-to-solution legs as their own line: value = sequential plain time / batched and
+    warm-started (trained network) time."""
+    r = ladder_relax(q, args)
+    lead = r.get("warm_trained", r["warm_zero_network"])
     out = {
-        "metric": METRIC, "value": t_seq / tw, "unit": "x (time to |dF/F|<1e-6: one context per droplet, plain "
-        "/ batched and warm-started)", "n_gpus": 1, "steps": done, "warmup": 0,
-        "ms_per_step": 1e3 * t_batched / max(done, 1),
+        "metric": METRIC, "value": lead["speedup_vs_sequential"],
+        "unit": "x (time to |dF/F|<1e-6: one context per droplet, plain / batched and warm-started)",
+        "n_gpus": 1, "steps": r["total_steps_batched"], "warmup": 0,
+        "ms_per_step": 1e3 * r["batched_plain_s"] / max(r["total_steps_batched"], 1),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
-        "data": "synthetic (radial start S0 = 0.5 per droplet)",
-        "config": {"workload": "relaxation of the 64 quench-ladder droplets (R=16..48 in (2R+8)^3 boxes, strong/weak "
-                               "anchoring, L1=6e-12) at T=300 K to |dF/F|<1e-6 between checks every "
-                               f"{every} steps",
-                   "batched_s": t_batched, "sequential_s": t_seq, "autotune": tune,
-                   "converged_batched": n_conv, "steps_per_droplet_batched": steps_b,
-                   "steps_per_droplet_sequential": steps_s,
-                   "same_steps": int(sum(a == c for a, c in zip(steps_b, steps_s))),
-                   "same_defects": same_defects, "max_rel_dF_final": max_rel_dF,
-                   "batched_plain_speedup": t_seq / t_batched,
-                   "warm_trained": vs_plain(warm["trained"]), "warm_zero_network": vs_plain(warm["zero"]),
-                   "training": {k: v for k, v in tr.items() if k != "loss"}},
+        "data": "synthetic (radial start S0 = 0.5 per droplet)", "config": r,
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -549,13 +578,15 @@ def main():
                     help="time-to-solution leg (headline workload): relax from the start state until the free "
                          "energy changes by less than --relax-tol between log points, at most this many steps "
                          "(0 skips the leg)")
+    ap.add_argument("--tts-traj", type=int, default=16,
+                    help="headline line: run the ladder time-to-solution legs (batched and warm-started vs one "
+                         "plain context per droplet) with a network trained on this many trajectories "
+                         "(0: zero network only; -1 skips the legs)")
     ap.add_argument("--relax-every", type=int, default=100)
     ap.add_argument("--relax-tol", type=float, default=1e-6)
     ap.add_argument("--imex", type=int, default=0, metavar="J",
                     help="time the semi-implicit quench step (RHS + J Jacobi sweeps, L_stab = L1 or kappa) "
                          "instead of the explicit step")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="only run a plain device copy of this many MiB (counter calibration) and exit")
     ap.add_argument("--surr-traj", type=int, default=256, help="surrogate_192: training trajectories")
     ap.add_argument("--surr-net-only", action="store_true",
                     help="surrogate_192: time only the network pass (--steps passes after --warmup)")
@@ -576,16 +607,6 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU slab decomposition is a design note only)", file=sys.stderr)
         return 2
-    if args.copy_probe:
-        import torch
-        n = args.copy_probe * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": 4 * n, "launches": 5}), flush=True)
-        return 0
     import numpy as np
 
     import lcdrsimulator_amd as q
@@ -641,30 +662,52 @@ def main():
     relax = None
     if args.workload == WORKLOADS[0] and args.relax_max_steps > 0 and not args.imex:
         relax = time_to_converge(q, p, args)
+    tts = None
+    if args.workload == WORKLOADS[0] and args.tts_traj >= 0 and not args.imex:
+        tts = ladder_relax(q, args, surr_traj=args.tts_traj)
     ms_per_step = 1e3 * wall / args.steps
-    value = work["box_voxels"] * args.steps / wall
-    traffic_bytes = BYTES_PER_VOXEL * work["tile_voxels"]
-    achieved = traffic_bytes / (main_ms * 1e-3) / 1e9
+    # one voxel update = one droplet voxel stepped (what the reference's update kernel changes per step)
+    value = work["active_voxels"] * args.steps / wall
+    algo_bytes = BYTES_PER_VOXEL * work["active_voxels"]
+    tile_bytes = BYTES_PER_VOXEL * work["tile_voxels"]
+    achieved = algo_bytes / (main_ms * 1e-3) / 1e9
     hbm_bytes, hbm_src = measured_traffic(args.workload, int(tune["variant"]), int(tune["planes_per_tile"]))
-    valu_busy, _ = profiled("bench-valu:", "busy", args.workload, int(tune["variant"]), int(tune["planes_per_tile"]))
+    valu_busy, valu_src = profiled("bench-valu:", "busy", args.workload, int(tune["variant"]),
+                                   int(tune["planes_per_tile"]))
+    flop_vox, flop_src = profiled("bench-flop:", "per_tile_voxel", args.workload, int(tune["variant"]),
+                                  int(tune["planes_per_tile"]))
+    compute = None
+    if flop_vox is not None:
+        tflops = flop_vox * work["tile_voxels"] / (main_ms * 1e-3) / 1e12
+        compute = {"bound": "valu", "achieved": tflops, "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                   "frac": tflops / FP32_VALU_PEAK_TFLOPS, "flop_per_tile_voxel": flop_vox,
+                   "flop_per_droplet_voxel": flop_vox * work["tile_voxels"] / work["active_voxels"],
+                   "source": flop_src + " (executed FP32 lane operations, FMA = 2; peak = non-packed FP32 issue)"}
     cpu = None if (args.no_cpu or args.workload != WORKLOADS[0]) else cpu_baseline(p)
     if relax is not None and cpu is not None:
         # the CPU figure is a rate on a bounded sample; the full relaxation is not run on the CPU
         relax["cpu_seconds_estimate"] = relax["steps"] * work["box_voxels"] / cpu["value"]
     out = {
-        "metric": METRIC, "value": value, "unit": "voxel-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32",
+        "metric": METRIC, "value": value, "unit": "voxel-updates/s (droplet voxels)", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic (seeded random-director droplet, reference init_mode 0)",
         "config": {"workload": describe(p, args.workload) + (f", semi-implicit J={args.imex}" if args.imex else ""),
                    "box_voxels": work["box_voxels"], "active_voxels": work["active_voxels"],
                    "tile_voxels": work["tile_voxels"], "graph_chunk": args.graph, "autotune": tune,
-                   "droplet_voxel_updates_per_s": work["active_voxels"] * args.steps / wall,
+                   "box_voxel_updates_per_s": work["box_voxels"] * args.steps / wall,
+                   "tile_voxel_updates_per_s": work["tile_voxels"] * args.steps / wall,
                    "fused_stencil_ms": main_ms, "face_ms": face_ms, "event_ms_per_step": ev_ms,
-                   "final_total_F": obs["total"], "algorithmic_bytes_per_launch": traffic_bytes,
-                   "traffic_source": hbm_src, "valu_busy_frac": valu_busy, "relaxation": relax},
+                   "final_total_F": obs["total"], "algorithmic_bytes_per_launch": algo_bytes,
+                   "tile_bytes_per_launch": tile_bytes,
+                   "valu_busy": None if valu_busy is None else {"frac": valu_busy, "source": valu_src},
+                   "relaxation": relax, "time_to_solution": tts},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": hbm_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": hbm_bytes,
+                     "traffic_source": None if hbm_src is None else
+                     hbm_src + " (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, same workload and tuning)",
+                     "tile_frac": tile_bytes / (main_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "compute": compute},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
