@@ -373,6 +373,7 @@ def ladder_relax(q, args, surr_traj=None, trained=True):
                                                          for a, c in zip(fin, final_b))),
                 "max_rel_dF_vs_batched_plain": max(abs(a["total"] - c["total"]) / abs(c["total"])
                                                    for a, c in zip(fin, final_b)),
+                "defects_per_droplet": [int(a["defects"]) for a in fin],
                 "worst_droplet": worst(r)}
     out = {"workload": "relaxation of the 64 quench-ladder droplets (R=16..48 in (2R+8)^3 boxes, strong/weak "
                        "anchoring, L1=6e-12) at T=300 K from radial starts to |dF/F|<" f"{tol:g} between checks every "
@@ -382,6 +383,7 @@ def ladder_relax(q, args, surr_traj=None, trained=True):
            "steps_per_droplet_sequential": steps_s, "total_steps_batched": done,
            "same_steps": int(sum(a == c for a, c in zip(steps_b, steps_s))),
            "same_defects": same_defects, "max_rel_dF_final": max_rel_dF,
+           "defects_per_droplet_batched": [int(o["defects"]) for o in final_b],
            "batched_plain_speedup": t_seq / t_batched}
     for name, r in warm.items():
         out["warm_trained" if name == "trained" else "warm_zero_network"] = vs_plain(r)
