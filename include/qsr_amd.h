@@ -146,6 +146,17 @@ int qsr_observables(void* ctx, int box, const qsr_obs_opts* opts, qsr_obs* out);
  * synchronisation: the energy sum of the early-stop test (reduce_energy,
  * QSR.cu:2323-2341; the test, QSR.cu:3322-3335) for a batched context. */
 int qsr_energies(void* ctx, const int* boxes, int n, double* totals);
+/* qsr_energies for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): no
+ * counterpart in the reference.  The same kernels, so the same totals bit for
+ * bit, but the reduction partials live in a fixed device array of the library
+ * (about 550 KB per GPU, held until the process ends) instead of buffers the
+ * context allocates on first use; nothing is allocated or freed.  Limits: at
+ * most 4096 boxes per call, droplet boxes only (a periodic box returns
+ * QSR_E_ARG; use qsr_energies).  Every call on one GPU shares that array, so
+ * calls that overlap on one GPU (several host threads, several contexts on
+ * one device) race; serialise them.  totals is a host array of n doubles. */
+int qsr_energies_noalloc(void* ctx, const int* boxes, int n, double* totals);
 int qsr_box_info_get(void* ctx, int box, qsr_box_info* out);
 /* Iteration counter of a box (T(iter) follows the protocol). */
 int qsr_set_iteration(void* ctx, int box, int64_t iteration);

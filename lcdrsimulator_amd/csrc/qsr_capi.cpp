@@ -17,9 +17,29 @@
 #include "qsr_kernels.hpp"
 #include "qsr_surr.hpp"
 
+// Scratch of qsr_energies_noalloc: the volume partials of one box (the volume
+// passes launch at most 4096 blocks of NPART slots), then NPART finalized sums
+// and one total per listed box.  One copy per GPU, held until the process ends;
+// host code reaches it only through the address hipGetSymbolAddress returns.
+constexpr int kNoallocMaxBoxes = 4096;
+constexpr long long kNoallocPart = 4096LL * qsro::NPART;
+constexpr long long kNoallocDoubles = kNoallocPart + (long long)kNoallocMaxBoxes * (qsro::NPART + 1);
+static __device__ __align__(256) double scratch[kNoallocDoubles];
+
+__global__ void energy_totals_kernel(const double* sums, int n, double* totals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double* w = sums + (long long)i * qsro::NPART;
+        totals[i] = (w[0] + w[1]) + w[2];  // bulk + elastic + anchoring, as the host sums them
+    }
+}
+
 namespace {
 
 thread_local std::string g_err;
+
+constexpr int kMaxDevices = 64;
+double* g_scratch[kMaxDevices] = {};  // device address of `scratch` on each GPU, set at context creation
 
 int fail(int code, const std::string& msg) {
     g_err = msg;
@@ -309,6 +329,11 @@ int create_impl(const qsr_params* params, int n, int device, void** out) {
     auto c = std::make_unique<Ctx>();
     c->device = device;
     HIP_TRY(hipSetDevice(device));
+    if (device < kMaxDevices && !g_scratch[device]) {
+        void* addr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)));
+        g_scratch[device] = static_cast<double*>(addr);
+    }
     HIP_TRY(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&c->ev[0]));
     HIP_TRY(hipEventCreate(&c->ev[1]));
@@ -952,6 +977,44 @@ int qsr_energies(void* ctx, const int* boxes, int n, double* totals) {
         if (int rc = qsr_observables(ctx, boxes[i], &oo, &o)) return rc;
         totals[i] = o.total;
     }
+    return QSR_OK;
+}
+
+int qsr_energies_noalloc(void* ctx, const int* boxes, int n, double* totals) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !boxes || !totals || n < 0) return fail(QSR_E_ARG, "bad argument");
+    if (n > kNoallocMaxBoxes)
+        return fail(QSR_E_ARG, "qsr_energies_noalloc: at most " + std::to_string(kNoallocMaxBoxes) + " boxes per call");
+    for (int i = 0; i < n; ++i) {
+        if (boxes[i] < 0 || boxes[i] >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad box");
+        if (c->boxes[boxes[i]].p.periodic)
+            return fail(QSR_E_ARG, "qsr_energies_noalloc: droplet boxes only (box " + std::to_string(boxes[i]) +
+                                       " is periodic; use qsr_energies)");
+    }
+    if (n == 0) return QSR_OK;
+    if (int rc = check_uploaded(*c)) return rc;
+    if (c->device >= kMaxDevices || !g_scratch[c->device])
+        return fail(QSR_E_STATE, "qsr_energies_noalloc: no scratch address for this device");
+    HIP_TRY(hipSetDevice(c->device));
+    double* part = g_scratch[c->device];
+    double* sums = part + kNoallocPart;
+    double* tot = sums + (long long)kNoallocMaxBoxes * qsro::NPART;
+    const float* q = c->q[c->cur];
+    for (int i = 0; i < n; ++i) {
+        const Box& b = c->boxes[boxes[i]];
+        if (qsro::obs_volume_blocks(b.nvox) * qsro::NPART > kNoallocPart)
+            return fail(QSR_E_ARG, "qsr_energies_noalloc: partials exceed the scratch");
+        const qsro::ObsParams P = obs_params(*c, boxes[i], true);
+        long long nblk = 0;
+        // the partial slots are reused box after box: the stream orders each box's pass after the
+        // previous box's finalize
+        HIP_TRY(qsro::launch_obs_droplet(c->st, q, c->cs, P, c->d_step_rows, c->d_obs_rows, part, &nblk));
+        HIP_TRY(qsro::launch_obs_finalize(c->st, part, nblk, qsro::NPART, 1u << 7, sums + (long long)i * qsro::NPART));
+    }
+    hipLaunchKernelGGL(energy_totals_kernel, dim3((n + 255) / 256), dim3(256), 0, c->st, sums, n, tot);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(totals, tot, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
     return QSR_OK;
 }
 

@@ -115,6 +115,7 @@ hipError_t launch_soa_to_aos(hipStream_t st, const float* q, double* aos, long l
 }  // namespace qsrk
 
 namespace qsro {
+long long obs_volume_blocks(long long n_voxels);  // blocks (= partial records) of the volume passes
 hipError_t launch_obs_droplet(hipStream_t st, const float* q, long long cs, const ObsParams& P,
                               const RowMask* step_rows, const RowMask* obs_rows, double* partials, long long* nblk);
 hipError_t launch_obs_periodic(hipStream_t st, const float* q, long long cs, const ObsParams& P, double* nf,

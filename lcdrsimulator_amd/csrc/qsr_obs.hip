@@ -473,6 +473,7 @@ namespace qsro {
 static unsigned blocks_for(long long n) { return unsigned((n + 255) / 256); }
 // grid of the volume passes (grid-stride loops): enough blocks to fill the chip, few partials
 static unsigned volume_blocks(long long n) { return std::min(blocks_for(n), 4096u); }
+long long obs_volume_blocks(long long n) { return volume_blocks(n); }
 
 hipError_t launch_obs_droplet(hipStream_t st, const float* q, long long cs, const ObsParams& P,
                               const RowMask* step_rows, const RowMask* obs_rows, double* partials, long long* nblk) {

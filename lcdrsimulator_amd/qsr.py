@@ -78,7 +78,7 @@ EXPORTS = (
     "qsr_surr_get_params", "qsr_surr_save", "qsr_surr_load", "qsr_surr_eval", "qsr_surr_adam_step",
     "qsr_surr_dataset_add", "qsr_surr_dataset_size", "qsr_surr_dataset_clear", "qsr_surr_train", "qsr_surr_apply",
     "qsr_surr_dataset_add_increment", "qsr_surr_set_input", "qsr_surr_apply_ladder", "qsr_surr_eval_gain",
-    "qsr_surr_apply_boxes", "qsr_energies",
+    "qsr_surr_apply_boxes", "qsr_energies", "qsr_energies_noalloc",
 )
 
 SURR_CHECKED, SURR_FORCE, SURR_PROPOSE = 0, 1, 2
@@ -149,6 +149,7 @@ def load():
     L.qsr_surr_apply_ladder.argtypes = [vp, vp, C.c_int, dp, C.c_int, C.POINTER(_SurrResult)]
     L.qsr_surr_eval_gain.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, fp, fp, fp, fp, dp, fp, fp]
     L.qsr_energies.argtypes = [vp, C.POINTER(C.c_int), C.c_int, dp]
+    L.qsr_energies_noalloc.argtypes = [vp, C.POINTER(C.c_int), C.c_int, dp]
     L.qsr_surr_apply_boxes.argtypes = [vp, vp, C.c_int, dp, C.c_int, C.POINTER(C.c_int), C.c_int,
                                        C.POINTER(_SurrResult)]
     _lib = L
@@ -302,6 +303,15 @@ class Solver:
         out = np.zeros(bx.size)
         _check(load().qsr_energies(self._h, bx.ctypes.data_as(C.POINTER(C.c_int)), bx.size,
                                    out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def energies_noalloc(self, boxes) -> np.ndarray:
+        """:meth:`energies` without device allocation (droplet boxes only, at most 4096 per call):
+        the library's fixed scratch array holds the partials."""
+        bx = np.ascontiguousarray(boxes, dtype=np.int32).ravel()
+        out = np.zeros(bx.size)
+        _check(load().qsr_energies_noalloc(self._h, bx.ctypes.data_as(C.POINTER(C.c_int)), bx.size,
+                                           out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
     def info(self, box: int = 0) -> dict:
