@@ -230,26 +230,33 @@ qsrk::BoxDev make_boxdev(const Box& b) {
 }
 
 // Does any interior output voxel of the tile change in a step?
-bool tile_active(const Box& b, int i0, int j0, int k0, int ch, int tj, int tk) {
+// Smallest and largest k of the voxels a step changes (or re-imposes) in rows
+// i in [ib, ie), j in [jb, je), within the interior k range [1, Nz-2]; false if
+// there are none.
+bool live_k_bounds(const Box& b, int ib, int ie, int jb, int je, int& lo, int& hi) {
     const qsr_params& p = b.p;
-    if (p.periodic) return true;
-    const int ib = std::max(i0, 1), ie = std::min(i0 + ch, p.Nx - 1);
-    const int jb = std::max(j0, 1), je = std::min(j0 + tj, p.Ny - 1);
-    const int kb = std::max(k0, 1), ke = std::min(k0 + tk, p.Nz - 1) - 1;
     const bool bc = !(p.W > 0.0);
+    lo = p.Nz;
+    hi = -1;
     for (int i = ib; i < ie; ++i)
         for (int j = jb; j < je; ++j) {
             const int16_t* s = &b.masks.step[(size_t(i) * p.Ny + j) * 8];
-            // evolving interval, and the shell (where BC or the penalty acts)
-            const int lo = std::max<int>(kb, s[0]), hi = std::min<int>(ke, s[1]);
-            if (lo <= hi) return true;
-            if (bc) {
-                const int olo = std::max<int>(kb, s[2]), ohi = std::min<int>(ke, s[3]);
-                for (int k = olo; k <= ohi; ++k)
-                    if (!(k >= s[4] && k <= s[5])) return true;
+            int a = std::max<int>(1, s[0]), e = std::min<int>(p.Nz - 2, s[1]);
+            if (a <= e) {
+                lo = std::min(lo, a);
+                hi = std::max(hi, e);
+            }
+            if (bc) {  // shell voxels outside the inner interval [s4, s5]
+                const int o0 = std::max<int>(1, s[2]), o1 = std::min<int>(p.Nz - 2, s[3]);
+                if (o0 > o1) continue;
+                const bool inner = s[4] <= s[5];
+                const int first = (inner && o0 >= s[4] && o0 <= s[5]) ? s[5] + 1 : o0;
+                const int last = (inner && o1 >= s[4] && o1 <= s[5]) ? s[4] - 1 : o1;
+                if (first <= o1) lo = std::min(lo, first);
+                if (last >= o0) hi = std::max(hi, last);
             }
         }
-    return false;
+    return lo <= hi;
 }
 
 int build_tiles(Ctx& c) {
@@ -260,14 +267,34 @@ int build_tiles(Ctx& c) {
     for (size_t bi = 0; bi < c.boxes.size(); ++bi) {
         const Box& b = c.boxes[bi];
         if (!b.active) continue;
-        for (int i0 = 0; i0 < b.p.Nx; i0 += c.ch)
-            for (int j0 = 0; j0 < b.p.Ny; j0 += tj)
-                for (int k0 = 0; k0 < b.p.Nz; k0 += tk)
-                    if (tile_active(b, i0, j0, k0, c.ch, tj, tk)) {
+        const int Nx = b.p.Nx, Ny = b.p.Ny, Nz = b.p.Nz;
+        for (int i0 = 0; i0 < Nx; i0 += c.ch)
+            for (int j0 = 0; j0 < Ny; j0 += tj) {
+                const long long ij = (long long)std::min(c.ch, Nx - i0) * std::min(tj, Ny - j0);
+                if (b.p.periodic) {  // every voxel steps: tiles on the k grid
+                    for (int k0 = 0; k0 < Nz; k0 += tk) {
                         nat.push_back(make_int4(int(bi), i0, j0, k0));
-                        c.tile_voxels += (long long)std::min(c.ch, b.p.Nx - i0) * std::min(tj, b.p.Ny - j0) *
-                                         std::min(tk, b.p.Nz - k0);
+                        c.tile_voxels += ij * std::min(tk, Nz - k0);
                     }
+                    continue;
+                }
+                // droplet: cover only the k range the rows of this (slab, block) can change,
+                // with the leftover of the last tile split over both ends (the kernels clamp
+                // loads at the box faces and write interior voxels only)
+                int lo, hi;
+                if (!live_k_bounds(b, std::max(i0, 1), std::min(i0 + c.ch, Nx - 1), std::max(j0, 1),
+                                   std::min(j0 + tj, Ny - 1), lo, hi))
+                    continue;
+                const int span = hi - lo + 1, n = (span + tk - 1) / tk;
+                int start = lo - (n * tk - span) / 2;
+                start = n * tk >= Nz ? 0 : std::min(std::max(start, 0), Nz - n * tk);
+                for (int t = 0; t < n; ++t) {
+                    const int k0 = start + t * tk;
+                    if (k0 >= Nz) break;
+                    nat.push_back(make_int4(int(bi), i0, j0, k0));
+                    c.tile_voxels += ij * std::min(tk, Nz - k0);
+                }
+            }
     }
     // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs,
     // so launch position 8*r + x carries tile x*q + r: each XCD (and its L2)
