@@ -387,20 +387,27 @@ __global__ __launch_bounds__(TK* TJ, (!NOISE && BUF) ? 7 : 1) void step_l1_kerne
 }
 
 // ---------------------------------------------------------------------------
-// Anisotropic kernel (L2 and/or L3).  Streams along i with
-//   * one "lead" plane (i = t+1) in LDS with an in-plane halo of 2,
-//     double-buffered, from which every thread takes its own first derivatives
-//     d_y Q, d_z Q, second differences and the mixed d_y d_z Q;
-//   * per-thread register queues along i for Q(t-1..t+2), d_yQ/d_zQ(t..t+1)
-//     and Dcol(t-1..t+1), so x-derivatives need no LDS: the mixed
+// Anisotropic kernel (L2 and/or L3).  Streams along i; output plane t is
+// produced after lead plane t+1 has been processed.
+//   * Three Q planes t, t+1 and (written at the end of the iteration) t+2 sit
+//     in LDS with an in-plane halo of 2 (a ring of three slots, so plane t is
+//     still resident while plane t+2 is written).  Every in-plane derivative
+//     of the own point (d_y Q, d_z Q, second differences, mixed d_y d_z Q) and
+//     the own values Q(t), Q(t+1) are read from the slots, not carried in
+//     registers.
+//   * Registers carry only Q(t-1) and Q(t+2) (plus the prefetch of Q(t+3)),
+//     Dcol(t-1) and, for ring threads, Q(t+2) of their ring point, so the
+//     kernel fits four waves per SIMD.
+//   * x-derivatives: central differences of the register values; the mixed
 //     d_x d_y Q = (d_yQ(t+1) - d_yQ(t-1)) / (2 dx) is the reference's 4-point
-//     formula (QSR.cu:583-588); the L3 term needs it only inside
-//     Q_xy d_x d_y Q + Q_xz d_x d_z Q, whose d_yQ/d_zQ(t-1) half is folded one
-//     plane early (pxm), so no register holds d_yQ/d_zQ(t-1);
-//   * Dcol = d_k Q_kj (QSR.cu:325-371) in an LDS plane with a halo of 1: own
-//     points reuse their first derivatives, the halo ring is computed by ring
-//     threads that keep a 3-component x-queue of their ring point.
-// Output plane t is produced after lead plane t+1 has been processed.
+//     formula (QSR.cu:583-588).  The L3 term needs it only inside
+//     Q_xy d_x d_y Q + Q_xz d_x d_z Q: its d_yQ/d_zQ(t-1) half is folded one
+//     plane early (pxm), while plane t-1 is still resident, and its t+1 half
+//     (pxp) while plane t+1's derivatives are formed.
+//   * Dcol = d_k Q_kj (QSR.cu:325-371) in an LDS plane with a halo of 1
+//     (two slots): own points reuse their first derivatives, the halo ring is
+//     computed by ring threads.
+// One barrier per plane orders every LDS hand-over (see the loop).
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -419,16 +426,16 @@ struct AnisoGeom {
 // EL: elastic terms compiled in (1 = L2, 2 = L3, 3 = both), the union over the
 // context's boxes; a box without a term has a zero coefficient for it, which
 // adds exactly nothing, so the whole molecular field is one basic block.
+// Four waves per SIMD (128 VGPRs) for the instances the headline runs use; the
+// noisy and 64-bit-offset instances keep three.
 template <int TK, int TJv, bool NOISE, bool BUF, int EL>
-__global__ __launch_bounds__(TK* TJv, 3) void step_aniso_kernel(const float* __restrict__ qin, float* __restrict__ qout,
-                                                             long long cs, const BoxDev* __restrict__ boxes,
-                                                             const int4* __restrict__ tiles,
-                                                             const StepDyn* __restrict__ dyn, int nboxes,
-                                                             const RowMask* __restrict__ rows, int ch,
-                                                          const float4* __restrict__ imex_tab) {
+__global__ __launch_bounds__(TK* TJv, (!NOISE && BUF) ? 4 : 3) void step_aniso_kernel(
+    const float* __restrict__ qin, float* __restrict__ qout, long long cs, const BoxDev* __restrict__ boxes,
+    const int4* __restrict__ tiles, const StepDyn* __restrict__ dyn, int nboxes, const RowMask* __restrict__ rows,
+    int ch, const float4* __restrict__ imex_tab) {
     using G = AnisoGeom<TK, TJv>;
     constexpr int W2 = G::W2, P2 = G::P2, QSLOT = G::QSLOT, W1 = G::W1, P1 = G::P1, DSLOT = G::DSLOT;
-    __shared__ float qs[2 * QSLOT];
+    __shared__ float qs[3 * QSLOT];
     __shared__ float ds[2 * DSLOT];
     const int4 t4 = tiles[blockIdx.x];
     if (t4.x < 0) return;  // padding tile of the XCD-ordered list (uniform over the block)
@@ -511,9 +518,11 @@ __global__ __launch_bounds__(TK* TJv, 3) void step_aniso_kernel(const float* __r
                 for (int c = 0; c < 5; ++c) hv[s][c] = F.ld(c, pb, hoff[s]);
             }
     };
-    // write a plane (own value `own` + halo `hv`) into Q slot `slot`
-    auto write_slot = [&](int slot, const float own[5], const float hv[G::NHPT][5]) {
-        float* Q = qs + slot * QSLOT;
+    // plane p lives in Q slot (p - ib + 1) mod 3
+    auto qslot = [&](int p) { return qs + ((p - ib + 1) % 3) * QSLOT; };
+    // write a plane (own value `own` + halo `hv`) into its Q slot
+    auto write_slot = [&](int p, const float own[5], const float hv[G::NHPT][5]) {
+        float* Q = qslot(p);
 #pragma unroll
         for (int c = 0; c < 5; ++c) Q[c * P2 + lq] = own[c];
 #pragma unroll
@@ -523,14 +532,18 @@ __global__ __launch_bounds__(TK* TJv, 3) void step_aniso_kernel(const float* __r
                 for (int c = 0; c < 5; ++c) Q[c * P2 + hpos[s]] = hv[s][c];
             }
     };
-    auto fill_slot = [&](int slot, int p, const float own[5]) {
+    auto fill_slot = [&](int p, const float own[5]) {
         float hv[G::NHPT][5];
         load_halo(p, hv);
-        write_slot(slot, own, hv);
+        write_slot(p, own, hv);
     };
-    // own first derivatives of a lead plane (face-aware scales)
-    auto own_d1 = [&](int slot, float dy[5], float dz[5]) {
-        const float* Q = qs + slot * QSLOT;
+    // own values of a resident plane
+    auto own_q = [&](const float* Q, float v[5]) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) v[c] = Q[c * P2 + lq];
+    };
+    // own first derivatives of a resident plane (face-aware scales)
+    auto own_d1 = [&](const float* Q, float dy[5], float dz[5]) {
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
             const float* A = Q + c * P2;
@@ -538,32 +551,36 @@ __global__ __launch_bounds__(TK* TJv, 3) void step_aniso_kernel(const float* __r
             dz[c] = (A[lq + 1] - A[lq - 1]) * az_own;
         }
     };
-    // own in-plane second differences and mixed d_y d_z of a lead plane
-    auto own_d2 = [&](int slot, float syy[5], float szz[5], float dyz[5]) {
-        const float* Q = qs + slot * QSLOT;
+    // own in-plane second differences and mixed d_y d_z of a resident plane (own values q)
+    auto own_d2 = [&](const float* Q, const float q[5], float syy[5], float szz[5], float dyz[5]) {
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
             const float* A = Q + c * P2;
-            const float q2 = 2.0f * A[lq];
+            const float q2 = 2.0f * q[c];
             syy[c] = A[lq + W2] + A[lq - W2] - q2;
             szz[c] = A[lq + 1] + A[lq - 1] - q2;
             dyz[c] = (A[lq + W2 + 1] - A[lq + W2 - 1] - A[lq - W2 + 1] + A[lq - W2 - 1]) * B.i4yz;
         }
     };
-    // Dcol from x-part (three comps xx, xy, xz differenced along x) and in-plane derivatives
+    // Dcol from x-part (three comps xx, xy, xz differenced along x) and in-plane derivatives.
+    // The quantities formed both in the prologue and in the loop (Dcol, the folded mixed
+    // terms) spell out their fused multiply-adds, so every call site rounds them alike and
+    // the result does not depend on where a tile's planes start (planes per tile, tile shape).
     auto dcol_own = [&](const float xm[5], const float xp[5], float ax, const float dy[5], const float dz[5],
                         float Dv[3]) {
-        Dv[0] = (xp[0] - xm[0]) * ax + dy[1] + dz[2];
-        Dv[1] = (xp[1] - xm[1]) * ax + dy[3] + dz[4];
-        Dv[2] = (xp[2] - xm[2]) * ax + dy[4] - (dz[0] + dz[3]);
+        Dv[0] = __builtin_fmaf(xp[0] - xm[0], ax, dy[1]) + dz[2];
+        Dv[1] = __builtin_fmaf(xp[1] - xm[1], ax, dy[3]) + dz[4];
+        Dv[2] = __builtin_fmaf(xp[2] - xm[2], ax, dy[4]) - (dz[0] + dz[3]);
     };
-    auto dcol_ring = [&](int slot, const float xm[3], const float xp[3], float ax, float Dv[3]) {
-        const float* Q = qs + slot * QSLOT;
+    // ring Dcol of resident plane Q; the x-difference uses the ring point's values of the plane
+    // before (read from slot Qm) and after (xp)
+    auto dcol_ring = [&](const float* Q, const float* Qm, const float xp[3], float ax, float Dv[3]) {
         auto dyv = [&](int c) { return (Q[c * P2 + lq_ring + W2] - Q[c * P2 + lq_ring - W2]) * ay_ring; };
         auto dzv = [&](int c) { return (Q[c * P2 + lq_ring + 1] - Q[c * P2 + lq_ring - 1]) * az_ring; };
-        Dv[0] = (xp[0] - xm[0]) * ax + dyv(1) + dzv(2);
-        Dv[1] = (xp[1] - xm[1]) * ax + dyv(3) + dzv(4);
-        Dv[2] = (xp[2] - xm[2]) * ax + dyv(4) - (dzv(0) + dzv(3));
+        const float y1 = dyv(1), y3 = dyv(3), y4 = dyv(4);
+        Dv[0] = __builtin_fmaf(xp[0] - Qm[0 * P2 + lq_ring], ax, y1) + dzv(2);
+        Dv[1] = __builtin_fmaf(xp[1] - Qm[1 * P2 + lq_ring], ax, y3) + dzv(4);
+        Dv[2] = __builtin_fmaf(xp[2] - Qm[2 * P2 + lq_ring], ax, y4) - (dzv(0) + dzv(3));
     };
     auto axs = [&](int p) { return face(p, Nx) ? B.fx : B.hx; };
     auto store_d = [&](int dslot, int pos, const float Dv[3]) {
@@ -573,89 +590,96 @@ __global__ __launch_bounds__(TK* TJv, 3) void step_aniso_kernel(const float* __r
         Dd[2 * P1 + pos] = Dv[2];
     };
 
-    // ---- prologue: lead planes ib-1 and ib --------------------------------
-    float q0[5], q1[5], q2[5], q3[5];           // own Q(t-1), Q(t), Q(t+1), Q(t+2)
-    float dy1[5], dz1[5], dy2[5], dz2[5];
+    // ---- prologue: planes ib-1, ib, ib+1 resident; Dcol(ib-1) own, Dcol(ib) own and ring --
+    float q0[5], q3[5];          // own Q(t-1), Q(t+2)
     // L3 only: (2 hx) (Q_xy(t) d_yQ(t-1) + Q_xz(t) d_zQ(t-1)), the t-1 half of the mixed
-    // x-y and x-z terms, folded one plane early so that d_yQ/d_zQ(t-1) need no registers
+    // x-y and x-z terms, folded one plane early while plane t-1 is resident
     float pxm[5] = {};
-    float syy1[5], szz1[5], dyz1[5];            // in-plane second derivatives of plane t
-    float D0[3], D1[3], D2[3];                  // own Dcol(t-1), Dcol(t), Dcol(t+1)
-    float r0[3] = {}, r1[3] = {}, r2[3] = {};   // ring comps xx,xy,xz at t, t+1, t+2
+    float D0[3];                 // own Dcol(t-1)
+    float r2[3] = {};            // ring comps xx, xy, xz at t+2
     {
-        float qm2[5];
+        float qm2[5], q1[5], q2[5];
         ld5(ib - 2, o_own, qm2);
         ld5(ib - 1, o_own, q0);
         ld5(ib, o_own, q1);
         ld5(ib + 1, o_own, q2);
         ld5(ib + 2, o_own, q3);
-        float rm[3] = {};
+        float rp[3] = {};
         if (ring) {
-            ld3(ib - 1, o_ring, rm);
-            ld3(ib, o_ring, r0);
-            ld3(ib + 1, o_ring, r1);
+            ld3(ib + 1, o_ring, rp);
             ld3(ib + 2, o_ring, r2);
         }
-        fill_slot(0, ib - 1, q0);
-        fill_slot(1, ib, q1);
+        fill_slot(ib - 1, q0);
+        fill_slot(ib, q1);
+        fill_slot(ib + 1, q2);
         lds_barrier();
         {
             float dym[5], dzm[5];
-            own_d1(0, dym, dzm);
+            own_d1(qslot(ib - 1), dym, dzm);
             dcol_own(qm2, q1, axs(ib - 1), dym, dzm, D0);
             if constexpr (hasL3) {
                 const float cy = 2.0f * q1[1] * B.hx, cz = 2.0f * q1[2] * B.hx;
 #pragma unroll
-                for (int c = 0; c < 5; ++c) pxm[c] = cy * dym[c] + cz * dzm[c];
+                for (int c = 0; c < 5; ++c) pxm[c] = __builtin_fmaf(cz, dzm[c], cy * dym[c]);
             }
         }
-        own_d1(1, dy1, dz1);
-        own_d2(1, syy1, szz1, dyz1);
+        float dy1[5], dz1[5], D1[3];
+        own_d1(qslot(ib), dy1, dz1);
         dcol_own(q0, q2, axs(ib), dy1, dz1, D1);
         store_d(ib & 1, ld, D1);
         if (ring) {
             float Dr[3];
-            dcol_ring(1, rm, r1, axs(ib), Dr);
+            dcol_ring(qslot(ib), qslot(ib - 1), rp, axs(ib), Dr);
             store_d(ib & 1, ld_ring, Dr);
         }
-        // r-queue now holds planes ib, ib+1, ib+2 as r0, r1, r2
-        lds_barrier();
-        fill_slot(0, ib + 1, q2);
         lds_barrier();
     }
 
     const bool pre_rows = out_ok && !per;  // rows with a mask record to prefetch
     int4 mcur = make_int4(0, 0, 0, 0);
     if (pre_rows) mcur = load_row4(rows, B, ib, j);
+    float hv[G::NHPT][5];  // halo of the next lead plane to write, loaded an iteration ahead
+    if (ib + 1 < ie) load_halo(ib + 2, hv);
     for (int t = ib; t < ie; ++t) {
-        const int qsl = (t - ib) & 1;  // Q slot holding lead plane t+1 (plane L sits in slot (L - ib + 1) & 1)
+        // LDS hand-over, one barrier per plane (at the end of the iteration):
+        //   Q slot of plane t+2 (written in (e)) last held plane t-1, last read before the
+        //   previous barrier; planes t and t+1 (read here) were written before it.
+        //   D slot (t+1)&1 (written in (b)) was last read in (d) of iteration t-1; D slot t&1
+        //   (read in (d)) was written in (b) of iteration t-1.
+        const float* Qt = qslot(t);
+        const float* Qn = qslot(t + 1);
         const int dsl = (t + 1) & 1;   // D slot of Dcol(t+1) (Dcol(L) sits in slot L & 1)
-        // prefetch, a whole iteration ahead of use: halo of lead plane t+2 and own/ring Q(t+3)
         const bool more = t + 1 < ie;
-        float hv[G::NHPT][5], q4[5], r3[3] = {};
-        int4 mnext = make_int4(0, 0, 0, 0);
-        if (more) {
-            if (pre_rows) mnext = load_row4(rows, B, t + 1, j);
-            load_halo(t + 2, hv);
-            ld5(t + 3, o_own, q4);
-            if (ring) ld3(t + 3, o_ring, r3);
+        // (b) lead plane t+1: first derivatives, own and ring Dcol(t+1), and (L3) the t+1 half
+        // of the mixed x-y and x-z terms of plane t, (2 hx)(Q_xy(t) d_yQ(t+1) + Q_xz(t) d_zQ(t+1))
+        float D2[3], pxp[5] = {};
+        {
+            float q1[5], dy2[5], dz2[5];
+            own_q(Qt, q1);
+            own_d1(Qn, dy2, dz2);
+            dcol_own(q1, q3, axs(t + 1), dy2, dz2, D2);
+            if constexpr (hasL3) {
+                const float cxy = 2.0f * q1[1] * B.hx, cxz = 2.0f * q1[2] * B.hx;
+#pragma unroll
+                for (int c = 0; c < 5; ++c) pxp[c] = __builtin_fmaf(cxz, dz2[c], cxy * dy2[c]);
+            }
         }
-        // (b) lead plane t+1: first derivatives, own and ring Dcol(t+1)
-        own_d1(qsl, dy2, dz2);
-        dcol_own(q1, q3, axs(t + 1), dy2, dz2, D2);
         store_d(dsl, ld, D2);
         if (ring) {
             float Dr[3];
-            dcol_ring(qsl, r0, r2, axs(t + 1), Dr);
+            dcol_ring(Qn, Qt, r2, axs(t + 1), Dr);
             store_d(dsl, ld_ring, Dr);
         }
-        // No barrier here: (d) reads only the Dcol(t) slot, written in (b) of the
-        // previous plane before its closing barrier, and (b) writes the other D
-        // slot, whose last reader (d of the previous plane) is behind that same
-        // barrier.  One barrier per plane orders every LDS hand-over.  The
-        // scheduling fence keeps the compiler from interleaving (b) and (d),
-        // which would hold both passes' values at once and spill.
+        // (e) lead plane t+2 into the slot that held plane t-1, which nothing reads after the
+        // previous barrier (its own value is q3, its halo arrived with the previous iteration's
+        // loads), so the halo registers are free before the output pass
+        if (more) write_slot(t + 2, q3, hv);
+        // the scheduling fence keeps the compiler from interleaving (b) and (d), which would
+        // hold both passes' values at once
         __builtin_amdgcn_sched_barrier(0);
+        // plane t's own first derivatives: the output pass and the fold below both need them
+        float dy1[5], dz1[5];
+        own_d1(Qt, dy1, dz1);
         // (d) output plane t
         if (out_ok) {
             bool evolve, shell;
@@ -663,6 +687,10 @@ __global__ __launch_bounds__(TK* TJv, 3) void step_aniso_kernel(const float* __r
             const bool live = (evolve && !((B.flags & F_SKIPSHELL) && shell)) || ((B.flags & F_BC) && shell);
             if (live) {
                 const float* Dc = ds + (t & 1) * DSLOT;
+                float q1[5], q2[5], syy1[5], szz1[5], dyz1[5];
+                own_q(Qt, q1);
+                own_q(Qn, q2);
+                own_d2(Qt, q1, syy1, szz1, dyz1);
                 float sx_[5], h[5];
 #pragma unroll
                 for (int c = 0; c < 5; ++c) sx_[c] = q2[c] + q0[c] - 2.0f * q1[c];
@@ -690,6 +718,7 @@ __global__ __launch_bounds__(TK* TJv, 3) void step_aniso_kernel(const float* __r
                     h[4] -= B.mL2 * (dDz_dy + dDy_dz);
                 }
                 if constexpr (hasL3) {
+                    const float D1[3] = {Dc[0 * P1 + ld], Dc[1 * P1 + ld], Dc[2 * P1 + ld]};
                     // d[c][axis] at plane t (interior: central differences)
                     float d[6][3];
 #pragma unroll
@@ -710,12 +739,12 @@ __global__ __launch_bounds__(TK* TJv, 3) void step_aniso_kernel(const float* __r
                     const float xx = q1[0], xy = q1[1], xz = q1[2], yy = q1[3], yz = q1[4], zz = -(xx + yy);
                     // Q_ij d_i d_j with the grid scales folded into the coefficients
                     const float cxx = xx * B.ix2, cyy = yy * B.iy2, czz = zz * B.iz2;
-                    const float cxy = 2.0f * xy * B.hx, cxz = 2.0f * xz * B.hx, cyz = 2.0f * yz;
+                    const float cyz = 2.0f * yz;
                     float Bc[5];
 #pragma unroll
                     for (int c = 0; c < 5; ++c) {
                         const float contract = cxx * sx_[c] + cyy * syy1[c] + czz * szz1[c] +
-                                               (cxy * dy2[c] + cxz * dz2[c] - pxm[c]) + cyz * dyz1[c];
+                                               (pxp[c] - pxm[c]) + cyz * dyz1[c];
                         Bc[c] = D1[0] * d[c][0] + D1[1] * d[c][1] + D1[2] * d[c][2] + contract;
                     }
                     const float hl = 0.5f * B.mL3;
@@ -738,27 +767,31 @@ __global__ __launch_bounds__(TK* TJv, 3) void step_aniso_kernel(const float* __r
                 }
             }
         }
-        // in-plane second derivatives of plane t+1 while its slot is resident
-        own_d2(qsl, syy1, szz1, dyz1);
-        // the t-1 half of plane t+1's mixed terms (Q(t+1) is q2, d_yQ/d_zQ(t) are dy1/dz1)
+        // the t-1 half of plane t+1's mixed terms: Q(t+1) and d_yQ/d_zQ(t), both resident
         if constexpr (hasL3) {
-            const float cy = 2.0f * q2[1] * B.hx, cz = 2.0f * q2[2] * B.hx;
+            const float cy = 2.0f * Qn[1 * P2 + lq] * B.hx, cz = 2.0f * Qn[2 * P2 + lq] * B.hx;
 #pragma unroll
-            for (int c = 0; c < 5; ++c) pxm[c] = cy * dy1[c] + cz * dz1[c];
+            for (int c = 0; c < 5; ++c) pxm[c] = __builtin_fmaf(cz, dz1[c], cy * dy1[c]);
         }
-        // rotate the queues: t+1 becomes t
+        // loads for the next iteration: row mask of plane t+1, own and ring Q(t+3), halo of
+        // lead plane t+3
+        int4 mnext = make_int4(0, 0, 0, 0);
+        float q4[5], r3[3] = {};
+        const bool more2 = t + 2 < ie;
+        if (more) {
+            if (pre_rows) mnext = load_row4(rows, B, t + 1, j);
+            ld5(t + 3, o_own, q4);
+            if (ring) ld3(t + 3, o_ring, r3);
+            if (more2) load_halo(t + 3, hv);
+        }
+        // advance: t+1 becomes t
+        own_q(Qt, q0);
 #pragma unroll
-        for (int c = 0; c < 5; ++c) {
-            dy1[c] = dy2[c]; dz1[c] = dz2[c];
-            q0[c] = q1[c]; q1[c] = q2[c]; q2[c] = q3[c]; q3[c] = q4[c];
-        }
+        for (int a = 0; a < 3; ++a) D0[a] = ds[(t & 1) * DSLOT + a * P1 + ld];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            D0[a] = D1[a]; D1[a] = D2[a];
-            r0[a] = r1[a]; r1[a] = r2[a]; r2[a] = r3[a];
-        }
-        // (e) lead plane t+2 into the other Q slot (its own value is now q2)
-        if (more) write_slot(qsl ^ 1, q2, hv);
+        for (int c = 0; c < 5; ++c) q3[c] = more ? q4[c] : q3[c];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) r2[a] = r3[a];
         mcur = mnext;
         lds_barrier();
     }

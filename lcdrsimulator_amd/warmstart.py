@@ -80,12 +80,13 @@ def snapshot_pairs(n_snap: int, stride: int, gap: int):
 
 def train(net: q.Surrogate, trajectories, gap: int = 500, n_snap: int = 7, group: int = 32,
           iters_per_group: int = 1000, lr: float = 3e-4, batch: int = 8, crop: int = 32, weight_floor: float = 1.0,
-          stride: int | None = None, increment: bool = False, log=None) -> dict:
+          stride: int | None = None, increment: bool = False, first: int = 0, log=None) -> dict:
     """Generate (Q_k, Q_{k+gap}) pairs with the plain solver, ``group`` trajectories per batched
     context (snapshots every ``stride`` steps, default ``gap``; with ``increment`` each also records
     one explicit step's increment, the network's input under increment input), and train on each
     group's pairs (the dataset is cleared between groups, so device memory holds one group's
-    snapshots).  Returns timings and the loss history."""
+    snapshots).  With ``first`` the snapshots start that many plain steps into each trajectory.
+    Returns timings and the loss history."""
     stride = stride or gap
     t_gen = t_train = 0.0
     hist = []
@@ -97,6 +98,7 @@ def train(net: q.Surrogate, trajectories, gap: int = 500, n_snap: int = 7, group
             for b, (p, mode, seed) in enumerate(chunk):
                 s.upload(q.init_field(p, mode, 0.5, 0.0, seed), box=b)
             steps, inputs, idx_pairs = snapshot_pairs(n_snap, stride, gap)
+            steps = [first + k for k in steps]
             ids, t = [], 0
             for t_next, is_input in zip(steps, inputs):
                 if t_next > t:
@@ -144,21 +146,21 @@ def relax_plain(s: q.Solver, tol: float = 1e-6, every: int = 100, max_steps: int
 
 
 def relax_warm(s: q.Solver, net: q.Surrogate, tol: float = 1e-6, every: int = 100, max_steps: int = 200000,
-               scales=(1.0, 0.5, 0.25), tail_tol: float | None = None) -> dict:
+               scales=(1.0, 0.5, 0.25), tail_tol: float | None = None, hold: int = 0) -> dict:
     """Cycles of one checked learned step (the ladder ``scales``: the first proposal that
     raises neither F nor the residual is kept, else the field is left as it was) and 2 x ``every``
     plain steps with the energy checked after each half; converged when |dF/F| < tol between
     the two checks of a cycle, which no jump separates (the energy part of QSR.cu:3322-3335).
     With ``tail_tol``, a cycle whose predecessor's |dF/F| was below it takes no learned step (the
     plain solver alone makes the last approach; a plateau of a coarsening run only pauses them).
-    ``steps`` counts plain steps only."""
+    No learned step is tried in the first ``hold`` plain steps.  ``steps`` counts plain steps only."""
     n, att, acc, net_ms = 0, 0, 0, 0.0
     kept = []
     conv, rel = False, math.nan
     t0 = time.perf_counter()
     tail = False
     while n < max_steps:
-        if not tail:
+        if not tail and n >= hold:
             r = net.apply(s, q.SURR_CHECKED, scales=list(scales))
             att += 1
             net_ms += r["ms_network"]
@@ -181,15 +183,15 @@ def relax_warm(s: q.Solver, net: q.Surrogate, tol: float = 1e-6, every: int = 10
 
 def relax_batch(s: q.Solver, n_boxes: int, net: q.Surrogate | None = None, tol: float = 1e-6, every: int = 100,
                 max_steps: int = 200000, scales=(1.0, 0.5, 0.25), backoff: bool = True,
-                tail_tol: float | None = None) -> dict:
+                tail_tol: float | None = None, hold: int = 0) -> dict:
     """Relax every box of a batched solver, retiring each (qsr_set_box_active) once its energy
     test passes.  Plain (no ``net``): the test of :func:`relax_plain` per box, checks every
     ``every`` steps.  Warm-started: the cycles of :func:`relax_warm`, with one checked learned step
     per active box per cycle (:meth:`Surrogate.apply_boxes`); with ``backoff`` a box whose step was
     refused sits out the next 1, 2, 4, ... cycles (reset when one is kept).  With ``tail_tol`` a box
     whose |dF/F| over the last cycle's second half was below it takes no learned step in the next
-    cycle: the plain solver alone makes the last approach to the test.  ``steps`` per box counts
-    plain steps."""
+    cycle: the plain solver alone makes the last approach to the test.  No learned step is tried in
+    the first ``hold`` plain steps.  ``steps`` per box counts plain steps."""
     active = list(range(n_boxes))
     steps = [0] * n_boxes
     prev = [None] * n_boxes
@@ -207,7 +209,7 @@ def relax_batch(s: q.Solver, n_boxes: int, net: q.Surrogate | None = None, tol: 
             done += every
             F = energies(active)
         else:
-            sel = [b for b in active if wait[b] == 0 and not tail[b]]
+            sel = [b for b in active if wait[b] == 0 and not tail[b]] if done >= hold else []
             for b in active:
                 wait[b] = max(0, wait[b] - 1)
             ta = time.perf_counter()
@@ -254,7 +256,8 @@ def zero_output_layer(net: q.Surrogate):
 
 
 def compare(p: q.Params, net: q.Surrogate, init_mode: int, seed: int = 12345, tol: float = 1e-6, every: int = 100,
-            max_steps: int = 200000, scales=(1.0, 0.5, 0.25), tail_tol: float | None = None, log=None) -> dict:
+            max_steps: int = 200000, scales=(1.0, 0.5, 0.25), tail_tol: float | None = None, hold: int = 0,
+            log=None) -> dict:
     """Plain relaxation and warm-started relaxation of the same start on one context."""
     Q0 = q.init_field(p, init_mode, 0.5, 0.0, seed)
     out = {}
@@ -267,7 +270,7 @@ def compare(p: q.Params, net: q.Surrogate, init_mode: int, seed: int = 12345, to
             if name == "plain":
                 r = relax_plain(s, tol, every, max_steps)
             else:
-                r = relax_warm(s, net, tol, every, max_steps, scales, tail_tol)
+                r = relax_warm(s, net, tol, every, max_steps, scales, tail_tol, hold)
             o = s.observables()
             r.update({"final_F": o["total"], "defects": int(o["defects"]), "plaquettes": int(o["plaquettes"])})
             out[name] = r

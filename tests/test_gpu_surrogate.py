@@ -237,17 +237,23 @@ def test_trained_jump_gains_over_plain_steps(qsr):
     assert gains["untrained"] < 0.5 * gains["trained"]
 
 
+# The 1e-5 final-energy bar is checked on droplets of R >= 24.  Small droplets with L1/L2/L3 stop
+# (|dF/F| < 1e-6 per 100 steps) inside a slow tail: the R = 16 plain run's energy still falls by
+# 2.8e-5 over the 7,000 steps after its stop, so two runs stopped by the same rule at different
+# points of that tail differ by more than the bar.
+
+
 def test_warm_start_driver_relaxes_and_reports(qsr):
-    """The warm-start protocol end to end on a small droplet with the zero modulating network
-    (the plain step extrapolated): both relaxations reach |dF/F| < 1e-6, the warm-started one
-    ends at the plain run's energy to 1e-5 with the same defect count, and both loops are
-    deterministic."""
+    """The warm-start protocol end to end with the zero modulating network (the plain step
+    extrapolated) on an R = 40 droplet (W = 3e-4, L1/L2/L3): both relaxations reach
+    |dF/F| < 1e-6, the warm-started one in fewer plain steps, at the plain run's energy to 1e-5
+    with the same defect count, and both loops are deterministic."""
     from lcdrsimulator_amd import warmstart as ws
 
     with qsr.Surrogate(seed=None) as net:
         net.params = np.zeros(S.NPARAM, dtype=np.float32)
         net.set_input(qsr.SURR_INPUT_INCREMENT, 500.0)
-        p = ws.droplet_params(16.0, 300.0, 1e-4)
+        p = ws.droplet_params(40.0, 300.0, 3e-4)
         r = ws.compare(p, net, 1, max_steps=40000)
         r2 = ws.compare(p, net, 1, max_steps=40000)
     pl, wa = r["plain"], r["warm"]
@@ -255,8 +261,8 @@ def test_warm_start_driver_relaxes_and_reports(qsr):
           f"{wa['jumps_accepted']} accepted, {wa['jumps_rejected']} rejected; F {pl['final_F']:.9e} / "
           f"{wa['final_F']:.9e} (rel {r['rel_dF_final']:.2e}), defects {pl['defects']} / {wa['defects']}")
     assert pl["converged"] and wa["converged"]
-    assert wa["jumps_accepted"] >= 1
-    assert r["rel_dF_final"] <= 1e-4 and r["same_defects"]
+    assert wa["jumps_accepted"] >= 1 and wa["steps"] < pl["steps"]
+    assert r["rel_dF_final"] <= 1e-5 and r["same_defects"]
     for k in ("steps", "final_F"):
         assert r2["plain"][k] == pl[k] and r2["warm"][k] == wa[k]
 
@@ -366,12 +372,12 @@ def test_increment_training_needs_increment_snapshots(qsr):
             net.set_input(qsr.SURR_INPUT_INCREMENT, 0.0)
 
 
-def _trained_increment_net(qsr, net, kind, iters=1200):
+def _trained_increment_net(qsr, net, kind, iters=1200, radii=(14.0, 15.0, 17.0, 18.0), first=0):
     from lcdrsimulator_amd import warmstart as ws
     net.set_input(kind, HORIZON)
-    trajs = [(ws.droplet_params(R, 300.0, 1e-4), m, 0) for R in (14.0, 15.0, 17.0, 18.0) for m in (0, 1)]
+    trajs = [(ws.droplet_params(R, 300.0, 1e-4), m, 0) for R in radii for m in (0, 1)]
     return ws.train(net, trajs, gap=500, n_snap=9, stride=250, group=8, iters_per_group=iters, lr=1e-3,
-                    batch=4, crop=32, increment=True)
+                    batch=4, crop=32, increment=True, first=first)
 
 
 @pytest.mark.parametrize("kind", ["increment", "gain"])
@@ -422,37 +428,41 @@ def test_modulated_jump_gains_over_plain_steps_and_ladder_rescales(qsr, kind):
 
 
 def test_trained_warm_start_reaches_the_plain_energy_sooner(qsr):
-    """What the trained network gains over the method without it, end to end: a network trained
-    under increment input on eight small droplets relaxes an unseen radial droplet (R = 16) to
-    |dF/F| < 1e-6 in fewer plain steps than the plain solver, at the plain run's energy (1e-5)
-    and with the same defect count.  The zero network (the plain step extrapolated) is printed
-    beside it."""
+    """What the trained network gains over the plain solver, end to end: a network trained under
+    increment input on eight droplets (R = 36, 38, 42, 44; radial and random-director starts;
+    snapshots from step 500 on) relaxes an unseen radial droplet (R = 40, W = 3e-4), with no
+    learned step in the first 400 steps, to |dF/F| < 1e-6 in fewer plain steps than the plain
+    solver, at the plain run's energy (1e-5) and with the same defect count.  The zero network
+    (the plain step extrapolated, the method without learning) runs the same schedule and is
+    printed beside it: on these droplets training does not beat it (it is not asserted).
+    Trained on snapshots from step 0, the network's first jumps from the radial start reshape the
+    core into a 4-plaquette state that outlives the stop rule; from step 500 on, they do not."""
     from lcdrsimulator_amd import warmstart as ws
 
     with qsr.Surrogate(seed=3) as net, qsr.Surrogate(seed=None) as zero:
         ws.zero_output_layer(net)
-        _trained_increment_net(qsr, net, qsr.SURR_INPUT_INCREMENT)
+        _trained_increment_net(qsr, net, qsr.SURR_INPUT_INCREMENT, radii=(36.0, 38.0, 42.0, 44.0), first=500)
         zero.params = np.zeros(S.NPARAM, dtype=np.float32)
         zero.set_input(qsr.SURR_INPUT_INCREMENT, HORIZON)
-        p = ws.droplet_params(16.0, 300.0, 1e-4)
-        r = ws.compare(p, net, 1, max_steps=40000)
-        rz = ws.compare(p, zero, 1, max_steps=40000)
+        p = ws.droplet_params(40.0, 300.0, 3e-4)
+        r = ws.compare(p, net, 1, max_steps=40000, hold=400)
+        rz = ws.compare(p, zero, 1, max_steps=40000, hold=400)
     for name, x in (("trained", r), ("zero network", rz)):
         print(f"  {name}: plain {x['plain']['steps']} steps, warm {x['warm']['steps']} steps "
               f"({x['warm']['jumps_accepted']} kept, {x['warm']['jumps_rejected']} refused), "
               f"F {x['plain']['final_F']:.9e} / {x['warm']['final_F']:.9e} (rel {x['rel_dF_final']:.2e}), "
               f"defects {x['plain']['defects']} / {x['warm']['defects']}")
     assert r["warm"]["converged"] and r["warm"]["steps"] < r["plain"]["steps"]
-    assert r["rel_dF_final"] <= 1e-4 and r["same_defects"]
+    assert r["rel_dF_final"] <= 1e-5 and r["same_defects"]
 
 
 # ---------------------------------------------------------------------------
 # Batched contexts: one checked ladder per listed box.
 # ---------------------------------------------------------------------------
-def _ladder_boxes(qsr):
+def _ladder_boxes(qsr, radii=(10.0, 13.0, 8.0, 11.0)):
     from lcdrsimulator_amd import warmstart as ws
-    return [ws.droplet_params(R, T, W) for R, T, W in ((10.0, 300.0, 1e-4), (13.0, 298.0, 1e-3),
-                                                        (8.0, 303.0, 0.0), (11.0, 300.0, 1e-5))]
+    return [ws.droplet_params(R, T, W) for R, (T, W) in zip(radii, ((300.0, 1e-4), (298.0, 1e-3),
+                                                                   (303.0, 0.0), (300.0, 1e-5)))]
 
 
 @pytest.mark.parametrize("ladder", [[1.0], [8.0, 4.0, 2.0, 1.0, 0.5, 0.25]])
@@ -512,13 +522,13 @@ def test_apply_boxes_rejects_bad_lists(qsr):
 
 
 def test_batched_warm_start_relaxes_every_box(qsr):
-    """Batched relaxation with retirement, plain and warm-started with the zero network: every box
-    converges in both, warm-started in fewer plain steps in all, each near its plain energy (1e-3:
-    on droplets of R = 8-13 the |dF/F| test stops while slow modes still decay, and the two runs
-    can end in different core structures, e.g. 4 charged plaquettes against 2, both printed)."""
+    """Batched relaxation with retirement, plain and warm-started with the zero network, of four
+    droplets R = 24, 30, 36, 40 (T 298-303 K, strong and weak anchoring): every box converges in
+    both, warm-started in fewer plain steps in all, each at its plain energy to 1e-5 with the same
+    defect count."""
     from lcdrsimulator_amd import warmstart as ws
 
-    params = _ladder_boxes(qsr)
+    params = _ladder_boxes(qsr, radii=(24.0, 30.0, 36.0, 40.0))
     fields = [qsr.init_field(p, 1, 0.5, seed=0) for p in params]
     out = {}
     with qsr.Surrogate(seed=None) as zero:
@@ -537,8 +547,9 @@ def test_batched_warm_start_relaxes_every_box(qsr):
           f"defects plain {[int(o['defects']) for o in pl['obs']]} warm {[int(o['defects']) for o in wa['obs']]}; "
           f"rel dF {', '.join(f'{x:.1e}' for x in rel)}")
     assert pl["converged"] == wa["converged"] == len(params)
-    assert sum(wa["steps"]) < sum(pl["steps"])
-    assert max(rel) <= 1e-3
+    assert all(w < p for w, p in zip(wa["steps"], pl["steps"]))
+    assert max(rel) <= 1e-5
+    assert [int(o["defects"]) for o in pl["obs"]] == [int(o["defects"]) for o in wa["obs"]]
 
 
 def test_increment_training_loss_matches_the_host(qsr):
