@@ -316,7 +316,7 @@ def ladder_relax(q, args, surr_traj=None, trained=True):
             ws.zero_output_layer(net)
             net.set_input(q.SURR_INPUT_INCREMENT, 500.0)
             tr = ws.train(net, ws.training_set(surr_traj), gap=500, n_snap=13, stride=250, group=16,
-                          iters_per_group=args.surr_iters, lr=5e-4, increment=True, log=log)
+                          iters_per_group=args.surr_iters, lr=5e-4, increment=True, first=args.surr_first, log=log)
             nets.append(("trained", net))
         zero.params = np.zeros(zero.nparams, dtype=np.float32)
         zero.set_input(q.SURR_INPUT_INCREMENT, 500.0)
@@ -519,7 +519,7 @@ def run_surrogate(q, args):
         ws.zero_output_layer(net)
         net.set_input(q.SURR_INPUT_INCREMENT, float(horizon))
         tr = ws.train(net, trajs, gap=horizon, n_snap=13, stride=250, group=16, iters_per_group=args.surr_iters,
-                      lr=5e-4, increment=True, log=log)
+                      lr=5e-4, increment=True, first=args.surr_first, log=log)
         zero.params = np.zeros(zero.nparams, dtype=np.float32)
         zero.set_input(q.SURR_INPUT_INCREMENT, float(horizon))
         p = ws.droplet_params(88.0, 300.0, 1e-4, N=192)
@@ -599,6 +599,9 @@ def main():
                          "below this, so the plain solver makes the last approach (0: off; default 1e-5 for the "
                          "ladder's radial starts, off for surrogate_192, whose random-director start coarsens through "
                          "long stretches of small |dF/F| where the learned steps are what speeds it up)")
+    ap.add_argument("--surr-first", type=int, default=500,
+                    help="training snapshots start this many plain steps into each trajectory (from step 0, the "
+                         "network's first jumps from radial starts reshape the droplet cores)")
     ap.add_argument("--surr-iters", type=int, default=1000,
                     help="surrogate_192: Adam iterations per group of 16 trajectories")
     ap.add_argument("--workload", default=WORKLOADS[0], choices=WORKLOADS,
