@@ -31,6 +31,11 @@ __device__ __forceinline__ int wrap_or_clamp(int x, int n, bool periodic) {
 
 __device__ __forceinline__ bool in_iv(int k, short lo, short hi) { return k >= lo && k <= hi; }
 
+// A value equal in every lane, moved to a scalar register.
+__device__ __forceinline__ float uni(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+
 // Workgroup barrier that orders LDS only.  The step kernels never exchange data
 // through global memory inside a launch (they read one buffer and write the
 // other), so the barrier need not drain outstanding global loads: prefetches of
@@ -442,6 +447,17 @@ __global__ __launch_bounds__(TK* TJv, (!NOISE && BUF) ? 4 : 3) void step_aniso_k
     const BoxDev B = boxes[t4.x];
     const StepDyn D = dyn[t4.x];
     const float4 imex = imex_tab ? imex_tab[t4.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+    // The molecular field works on raw differences (Q[+1] - Q[-1], Q[+1] + Q[-1] - 2Q, the
+    // 4-point mixed difference); the grid scales and m L2, m L3 sit in these per-box products.
+    //   second differences: L1 (minus the IMEX stabiliser) + m L3 Q_aa / d_a^2
+    const float lxp = uni(B.lx - imex.x), lyp = uni(B.ly - imex.y), lzp = uni(B.lz - imex.z);
+    const float k3x = uni(B.mL3 * B.ix2), k3y = uni(B.mL3 * B.iy2), k3z = uni(B.mL3 * B.iz2);
+    const float k3yz = uni(2.0f * B.mL3 * B.i4yz);                      // 2 Q_yz d_y d_z
+    const float k3hx = uni(B.mL3 * B.hx), k3hy = uni(B.mL3 * B.hy), k3hz = uni(B.mL3 * B.hz);  // Dcol . grad
+    const float kpy = uni(2.0f * B.hx * B.hy * B.mL3), kpz = uni(2.0f * B.hx * B.hz * B.mL3);  // 2 Q_xa d_x d_a
+    const float a2x = uni(B.mL2 * B.hx), a2y = uni(B.mL2 * B.hy), a2z = uni(B.mL2 * B.hz);     // d Dcol
+    const float kxx = uni(B.mL3 * B.hx * B.hx), kyy = uni(B.mL3 * B.hy * B.hy), kzz = uni(B.mL3 * B.hz * B.hz);
+    const float kxy = uni(B.mL3 * B.hx * B.hy), kxz = uni(B.mL3 * B.hx * B.hz), kyz = uni(B.mL3 * B.hy * B.hz);
     const int tid = threadIdx.x, tk = tid & (TK - 1), tj = tid / TK;
     const int Nx = B.Nx, Ny = B.Ny, Nz = B.Nz;
     const bool per = (B.flags & F_PERIODIC) != 0;
@@ -542,45 +558,43 @@ __global__ __launch_bounds__(TK* TJv, (!NOISE && BUF) ? 4 : 3) void step_aniso_k
 #pragma unroll
         for (int c = 0; c < 5; ++c) v[c] = Q[c * P2 + lq];
     };
-    // own first derivatives of a resident plane (face-aware scales)
-    auto own_d1 = [&](const float* Q, float dy[5], float dz[5]) {
+    // own raw first differences Q[+1] - Q[-1] of a resident plane along y and z
+    auto own_r1 = [&](const float* Q, float ry[5], float rz[5]) {
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
             const float* A = Q + c * P2;
-            dy[c] = (A[lq + W2] - A[lq - W2]) * ay_own;
-            dz[c] = (A[lq + 1] - A[lq - 1]) * az_own;
+            ry[c] = A[lq + W2] - A[lq - W2];
+            rz[c] = A[lq + 1] - A[lq - 1];
         }
     };
-    // own in-plane second differences and mixed d_y d_z of a resident plane (own values q)
-    auto own_d2 = [&](const float* Q, const float q[5], float syy[5], float szz[5], float dyz[5]) {
-#pragma unroll
-        for (int c = 0; c < 5; ++c) {
-            const float* A = Q + c * P2;
-            const float q2 = 2.0f * q[c];
-            syy[c] = A[lq + W2] + A[lq - W2] - q2;
-            szz[c] = A[lq + 1] + A[lq - 1] - q2;
-            dyz[c] = (A[lq + W2 + 1] - A[lq + W2 - 1] - A[lq - W2 + 1] + A[lq - W2 - 1]) * B.i4yz;
-        }
-    };
-    // Dcol from x-part (three comps xx, xy, xz differenced along x) and in-plane derivatives.
-    // The quantities formed both in the prologue and in the loop (Dcol, the folded mixed
-    // terms) spell out their fused multiply-adds, so every call site rounds them alike and
-    // the result does not depend on where a tile's planes start (planes per tile, tile shape).
-    auto dcol_own = [&](const float xm[5], const float xp[5], float ax, const float dy[5], const float dz[5],
+    // Dcol from x-part (three comps xx, xy, xz differenced along x) and raw in-plane differences
+    // scaled by the point's face-aware y and z factors.  The quantities formed both in the
+    // prologue and in the loop (Dcol, the folded mixed terms) spell out their fused
+    // multiply-adds, and an own point and the same point as another tile's ring point use one
+    // sequence, so every call site rounds them alike and the result does not depend on where a
+    // tile's planes start (planes per tile, tile shape).
+    auto dcol_own = [&](const float xm[5], const float xp[5], float ax, const float ry[5], const float rz[5],
                         float Dv[3]) {
-        Dv[0] = __builtin_fmaf(xp[0] - xm[0], ax, dy[1]) + dz[2];
-        Dv[1] = __builtin_fmaf(xp[1] - xm[1], ax, dy[3]) + dz[4];
-        Dv[2] = __builtin_fmaf(xp[2] - xm[2], ax, dy[4]) - (dz[0] + dz[3]);
+        Dv[0] = __builtin_fmaf(ay_own, ry[1], __builtin_fmaf(az_own, rz[2], ax * (xp[0] - xm[0])));
+        Dv[1] = __builtin_fmaf(ay_own, ry[3], __builtin_fmaf(az_own, rz[4], ax * (xp[1] - xm[1])));
+        Dv[2] = __builtin_fmaf(ay_own, ry[4], __builtin_fmaf(-az_own, rz[0] + rz[3], ax * (xp[2] - xm[2])));
     };
     // ring Dcol of resident plane Q; the x-difference uses the ring point's values of the plane
     // before (read from slot Qm) and after (xp)
     auto dcol_ring = [&](const float* Q, const float* Qm, const float xp[3], float ax, float Dv[3]) {
-        auto dyv = [&](int c) { return (Q[c * P2 + lq_ring + W2] - Q[c * P2 + lq_ring - W2]) * ay_ring; };
-        auto dzv = [&](int c) { return (Q[c * P2 + lq_ring + 1] - Q[c * P2 + lq_ring - 1]) * az_ring; };
-        const float y1 = dyv(1), y3 = dyv(3), y4 = dyv(4);
-        Dv[0] = __builtin_fmaf(xp[0] - Qm[0 * P2 + lq_ring], ax, y1) + dzv(2);
-        Dv[1] = __builtin_fmaf(xp[1] - Qm[1 * P2 + lq_ring], ax, y3) + dzv(4);
-        Dv[2] = __builtin_fmaf(xp[2] - Qm[2 * P2 + lq_ring], ax, y4) - (dzv(0) + dzv(3));
+        auto ryv = [&](int c) { return Q[c * P2 + lq_ring + W2] - Q[c * P2 + lq_ring - W2]; };
+        auto rzv = [&](int c) { return Q[c * P2 + lq_ring + 1] - Q[c * P2 + lq_ring - 1]; };
+        Dv[0] = __builtin_fmaf(ay_ring, ryv(1), __builtin_fmaf(az_ring, rzv(2), ax * (xp[0] - Qm[0 * P2 + lq_ring])));
+        Dv[1] = __builtin_fmaf(ay_ring, ryv(3), __builtin_fmaf(az_ring, rzv(4), ax * (xp[1] - Qm[1 * P2 + lq_ring])));
+        Dv[2] = __builtin_fmaf(ay_ring, ryv(4),
+                               __builtin_fmaf(-az_ring, rzv(0) + rzv(3), ax * (xp[2] - Qm[2 * P2 + lq_ring])));
+    };
+    // L3: the t-1 or t+1 half of plane t's 2 Q_xy d_x d_y Q + 2 Q_xz d_x d_z Q (m L3 and the grid
+    // scales included), from plane t's Q_xy, Q_xz and the raw differences of the plane beside it
+    auto fold_x = [&](float qxy, float qxz, const float ry[5], const float rz[5], float p[5]) {
+        const float cy = qxy * kpy, cz = qxz * kpz;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) p[c] = __builtin_fmaf(cz, rz[c], cy * ry[c]);
     };
     auto axs = [&](int p) { return face(p, Nx) ? B.fx : B.hx; };
     auto store_d = [&](int dslot, int pos, const float Dv[3]) {
@@ -592,8 +606,8 @@ __global__ __launch_bounds__(TK* TJv, (!NOISE && BUF) ? 4 : 3) void step_aniso_k
 
     // ---- prologue: planes ib-1, ib, ib+1 resident; Dcol(ib-1) own, Dcol(ib) own and ring --
     float q0[5], q3[5];          // own Q(t-1), Q(t+2)
-    // L3 only: (2 hx) (Q_xy(t) d_yQ(t-1) + Q_xz(t) d_zQ(t-1)), the t-1 half of the mixed
-    // x-y and x-z terms, folded one plane early while plane t-1 is resident
+    // L3 only: the t-1 half of plane t's mixed x-y and x-z terms (fold_x), folded one plane
+    // early while plane t-1 is resident
     float pxm[5] = {};
     float D0[3];                 // own Dcol(t-1)
     float r2[3] = {};            // ring comps xx, xy, xz at t+2
@@ -614,18 +628,14 @@ __global__ __launch_bounds__(TK* TJv, (!NOISE && BUF) ? 4 : 3) void step_aniso_k
         fill_slot(ib + 1, q2);
         lds_barrier();
         {
-            float dym[5], dzm[5];
-            own_d1(qslot(ib - 1), dym, dzm);
-            dcol_own(qm2, q1, axs(ib - 1), dym, dzm, D0);
-            if constexpr (hasL3) {
-                const float cy = 2.0f * q1[1] * B.hx, cz = 2.0f * q1[2] * B.hx;
-#pragma unroll
-                for (int c = 0; c < 5; ++c) pxm[c] = __builtin_fmaf(cz, dzm[c], cy * dym[c]);
-            }
+            float rym[5], rzm[5];
+            own_r1(qslot(ib - 1), rym, rzm);
+            dcol_own(qm2, q1, axs(ib - 1), rym, rzm, D0);
+            if constexpr (hasL3) fold_x(q1[1], q1[2], rym, rzm, pxm);
         }
-        float dy1[5], dz1[5], D1[3];
-        own_d1(qslot(ib), dy1, dz1);
-        dcol_own(q0, q2, axs(ib), dy1, dz1, D1);
+        float ry1[5], rz1[5], D1[3];
+        own_r1(qslot(ib), ry1, rz1);
+        dcol_own(q0, q2, axs(ib), ry1, rz1, D1);
         store_d(ib & 1, ld, D1);
         if (ring) {
             float Dr[3];
@@ -650,19 +660,15 @@ __global__ __launch_bounds__(TK* TJv, (!NOISE && BUF) ? 4 : 3) void step_aniso_k
         const float* Qn = qslot(t + 1);
         const int dsl = (t + 1) & 1;   // D slot of Dcol(t+1) (Dcol(L) sits in slot L & 1)
         const bool more = t + 1 < ie;
-        // (b) lead plane t+1: first derivatives, own and ring Dcol(t+1), and (L3) the t+1 half
-        // of the mixed x-y and x-z terms of plane t, (2 hx)(Q_xy(t) d_yQ(t+1) + Q_xz(t) d_zQ(t+1))
+        // (b) lead plane t+1: raw first differences, own and ring Dcol(t+1), and (L3) the t+1 half
+        // of plane t's mixed x-y and x-z terms
         float D2[3], pxp[5] = {};
         {
-            float q1[5], dy2[5], dz2[5];
+            float q1[5], ry2[5], rz2[5];
             own_q(Qt, q1);
-            own_d1(Qn, dy2, dz2);
-            dcol_own(q1, q3, axs(t + 1), dy2, dz2, D2);
-            if constexpr (hasL3) {
-                const float cxy = 2.0f * q1[1] * B.hx, cxz = 2.0f * q1[2] * B.hx;
-#pragma unroll
-                for (int c = 0; c < 5; ++c) pxp[c] = __builtin_fmaf(cxz, dz2[c], cxy * dy2[c]);
-            }
+            own_r1(Qn, ry2, rz2);
+            dcol_own(q1, q3, axs(t + 1), ry2, rz2, D2);
+            if constexpr (hasL3) fold_x(q1[1], q1[2], ry2, rz2, pxp);
         }
         store_d(dsl, ld, D2);
         if (ring) {
@@ -677,9 +683,9 @@ __global__ __launch_bounds__(TK* TJv, (!NOISE && BUF) ? 4 : 3) void step_aniso_k
         // the scheduling fence keeps the compiler from interleaving (b) and (d), which would
         // hold both passes' values at once
         __builtin_amdgcn_sched_barrier(0);
-        // plane t's own first derivatives: the output pass and the fold below both need them
-        float dy1[5], dz1[5];
-        own_d1(Qt, dy1, dz1);
+        // plane t's raw first differences: the output pass and the fold below both need them
+        float ry1[5], rz1[5];
+        own_r1(Qt, ry1, rz1);
         // (d) output plane t
         if (out_ok) {
             bool evolve, shell;
@@ -687,78 +693,95 @@ __global__ __launch_bounds__(TK* TJv, (!NOISE && BUF) ? 4 : 3) void step_aniso_k
             const bool live = (evolve && !((B.flags & F_SKIPSHELL) && shell)) || ((B.flags & F_BC) && shell);
             if (live) {
                 const float* Dc = ds + (t & 1) * DSLOT;
-                float q1[5], q2[5], syy1[5], szz1[5], dyz1[5];
+                float q1[5], q2[5], sxx[5], syy[5], szz[5];
                 own_q(Qt, q1);
                 own_q(Qn, q2);
-                own_d2(Qt, q1, syy1, szz1, dyz1);
-                float sx_[5], h[5];
 #pragma unroll
-                for (int c = 0; c < 5; ++c) sx_[c] = q2[c] + q0[c] - 2.0f * q1[c];
-                bulk_field(q1, D.cA, B.cB, B.cC, h);
+                for (int c = 0; c < 5; ++c) {
+                    const float* A = Qt + c * P2;
+                    sxx[c] = __builtin_fmaf(-2.0f, q1[c], q2[c] + q0[c]);
+                    syy[c] = __builtin_fmaf(-2.0f, q1[c], A[lq + W2] + A[lq - W2]);
+                    szz[c] = __builtin_fmaf(-2.0f, q1[c], A[lq + 1] + A[lq - 1]);
+                }
+                // bulk, m [A Q - B (Q^2 - tr(Q^2)/3 I) + C tr(Q^2) Q] (QSR.cu:399-415), with the
+                // off-diagonal entries of Q^2 reduced by tr Q = 0
+                const float xx = q1[0], xy = q1[1], xz = q1[2], yy = q1[3], yz = q1[4], zz = -(xx + yy);
+                const float q2xx = __builtin_fmaf(xx, xx, __builtin_fmaf(xy, xy, xz * xz));
+                const float q2yy = __builtin_fmaf(xy, xy, __builtin_fmaf(yy, yy, yz * yz));
+                const float q2zz = __builtin_fmaf(xz, xz, __builtin_fmaf(yz, yz, zz * zz));
+                const float g = __builtin_fmaf(B.cC, q2xx + q2yy + q2zz, D.cA);
+                // diagonal entries before the one trace projection of bulk, L2 and L3 (px, py, pz),
+                // off-diagonal entries of h (h1, h2, h4)
+                float px = -B.cB * q2xx, py = -B.cB * q2yy, pz = -B.cB * q2zz;
+                float h1 = __builtin_fmaf(g, xy, -B.cB * __builtin_fmaf(xz, yz, -(xy * zz)));
+                float h2 = __builtin_fmaf(g, xz, -B.cB * __builtin_fmaf(xy, yz, -(xz * yy)));
+                float h4 = __builtin_fmaf(g, yz, -B.cB * __builtin_fmaf(xy, xz, -(yz * xx)));
+                // per component: L1 lap Q (+ IMEX) and L3's Q_ab d_a d_b Q + Dcol . grad Q
+                float e[5];
+                if constexpr (hasL3) {
+                    const float ex = __builtin_fmaf(k3x, xx, lxp), ey = __builtin_fmaf(k3y, yy, lyp);
+                    const float ez = __builtin_fmaf(k3z, zz, lzp), eyz = k3yz * yz;
+                    const float Dx = k3hx * Dc[0 * P1 + ld], Dy = k3hy * Dc[1 * P1 + ld], Dz = k3hz * Dc[2 * P1 + ld];
+                    float rx[5];
 #pragma unroll
-                for (int c = 0; c < 5; ++c) h[c] -= B.lx * sx_[c] + B.ly * syy1[c] + B.lz * szz1[c];
-                if constexpr (hasL2) {
-                    float dyp[3], dym[3], dzp[3], dzm[3];
+                    for (int c = 0; c < 5; ++c) {
+                        const float* A = Qt + c * P2;
+                        rx[c] = q2[c] - q0[c];
+                        const float syz = (A[lq + W2 + 1] - A[lq + W2 - 1]) - (A[lq - W2 + 1] - A[lq - W2 - 1]);
+                        float v = __builtin_fmaf(Dz, rz1[c], pxp[c] - pxm[c]);
+                        v = __builtin_fmaf(Dy, ry1[c], v);
+                        v = __builtin_fmaf(Dx, rx[c], v);
+                        v = __builtin_fmaf(eyz, syz, v);
+                        v = __builtin_fmaf(ez, szz[c], v);
+                        v = __builtin_fmaf(ey, syy[c], v);
+                        e[c] = __builtin_fmaf(ex, sxx[c], v);
+                    }
+                    // (1/2) m L3 d_a Q_kl d_b Q_kl = m L3 h_a h_b N_ab over raw differences r, with
+                    // N_ab = s_a s_b + w_a w_b + sum over xy, xz, yz of r_a r_b, s = r_xx + r_yy / 2,
+                    // w = (sqrt(3) / 2) r_yy: the xx, yy, zz = -(xx + yy) terms are 2 s s + (3/2) r_yy r_yy
+                    auto r = [&](int c, int a) { return a == 0 ? rx[c] : (a == 1 ? ry1[c] : rz1[c]); };
+                    float sv[3], wv[3];
 #pragma unroll
                     for (int a = 0; a < 3; ++a) {
-                        dyp[a] = Dc[a * P1 + ld + W1];
-                        dym[a] = Dc[a * P1 + ld - W1];
-                        dzp[a] = Dc[a * P1 + ld + 1];
-                        dzm[a] = Dc[a * P1 + ld - 1];
+                        sv[a] = __builtin_fmaf(0.5f, r(3, a), r(0, a));
+                        wv[a] = 0.8660254037844386f * r(3, a);
                     }
-                    const float dDx_dx = (D2[0] - D0[0]) * B.hx, dDy_dx = (D2[1] - D0[1]) * B.hx, dDz_dx = (D2[2] - D0[2]) * B.hx;
-                    const float dDx_dy = (dyp[0] - dym[0]) * B.hy, dDy_dy = (dyp[1] - dym[1]) * B.hy, dDz_dy = (dyp[2] - dym[2]) * B.hy;
-                    const float dDx_dz = (dzp[0] - dzm[0]) * B.hz, dDy_dz = (dzp[1] - dzm[1]) * B.hz, dDz_dz = (dzp[2] - dzm[2]) * B.hz;
-                    const float gxx = B.mL2 * dDx_dx, gyy = B.mL2 * dDy_dy, gzz = B.mL2 * dDz_dz;
-                    const float tr = (gxx + gyy + gzz) * (1.0f / 3.0f);
-                    h[0] -= gxx - tr;
-                    h[3] -= gyy - tr;
-                    h[1] -= B.mL2 * (dDy_dx + dDx_dy);
-                    h[2] -= B.mL2 * (dDz_dx + dDx_dz);
-                    h[4] -= B.mL2 * (dDz_dy + dDy_dz);
+                    auto N = [&](int a, int b) {
+                        float v = r(4, a) * r(4, b);
+                        v = __builtin_fmaf(r(2, a), r(2, b), v);
+                        v = __builtin_fmaf(r(1, a), r(1, b), v);
+                        v = __builtin_fmaf(wv[a], wv[b], v);
+                        return __builtin_fmaf(sv[a], sv[b], v);
+                    };
+                    px = __builtin_fmaf(kxx, N(0, 0), px);
+                    py = __builtin_fmaf(kyy, N(1, 1), py);
+                    pz = __builtin_fmaf(kzz, N(2, 2), pz);
+                    h1 = __builtin_fmaf(kxy, N(0, 1), h1);
+                    h2 = __builtin_fmaf(kxz, N(0, 2), h2);
+                    h4 = __builtin_fmaf(kyz, N(1, 2), h4);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 5; ++c)
+                        e[c] = __builtin_fmaf(lxp, sxx[c], __builtin_fmaf(lyp, syy[c], lzp * szz[c]));
                 }
-                if constexpr (hasL3) {
-                    const float D1[3] = {Dc[0 * P1 + ld], Dc[1 * P1 + ld], Dc[2 * P1 + ld]};
-                    // d[c][axis] at plane t (interior: central differences)
-                    float d[6][3];
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) {
-                        d[c][0] = (q2[c] - q0[c]) * B.hx;
-                        d[c][1] = dy1[c];
-                        d[c][2] = dz1[c];
-                    }
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) d[5][a] = -(d[0][a] + d[3][a]);
-                    float M[3][3];
-#pragma unroll
-                    for (int a = 0; a < 3; ++a)
-#pragma unroll
-                        for (int b = a; b < 3; ++b)
-                            M[a][b] = d[0][a] * d[0][b] + d[3][a] * d[3][b] + d[5][a] * d[5][b] +
-                                      2.0f * (d[1][a] * d[1][b] + d[2][a] * d[2][b] + d[4][a] * d[4][b]);
-                    const float xx = q1[0], xy = q1[1], xz = q1[2], yy = q1[3], yz = q1[4], zz = -(xx + yy);
-                    // Q_ij d_i d_j with the grid scales folded into the coefficients
-                    const float cxx = xx * B.ix2, cyy = yy * B.iy2, czz = zz * B.iz2;
-                    const float cyz = 2.0f * yz;
-                    float Bc[5];
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) {
-                        const float contract = cxx * sx_[c] + cyy * syy1[c] + czz * szz1[c] +
-                                               (pxp[c] - pxm[c]) + cyz * dyz1[c];
-                        Bc[c] = D1[0] * d[c][0] + D1[1] * d[c][1] + D1[2] * d[c][2] + contract;
-                    }
-                    const float hl = 0.5f * B.mL3;
-                    const float tr = hl * (M[0][0] + M[1][1] + M[2][2]) * (1.0f / 3.0f);
-                    h[0] += hl * M[0][0] - B.mL3 * Bc[0] - tr;
-                    h[3] += hl * M[1][1] - B.mL3 * Bc[3] - tr;
-                    h[1] += hl * M[0][1] - B.mL3 * Bc[1];
-                    h[2] += hl * M[0][2] - B.mL3 * Bc[2];
-                    h[4] += hl * M[1][2] - B.mL3 * Bc[4];
+                if constexpr (hasL2) {
+                    // m L2 sym d(Dcol): central differences of Dcol(t-1, t+1) and of plane t's D slot
+                    const float* Dy0 = Dc + ld;
+                    px = __builtin_fmaf(-a2x, D2[0] - D0[0], px);
+                    py = __builtin_fmaf(-a2y, Dy0[1 * P1 + W1] - Dy0[1 * P1 - W1], py);
+                    pz = __builtin_fmaf(-a2z, Dy0[2 * P1 + 1] - Dy0[2 * P1 - 1], pz);
+                    h1 = __builtin_fmaf(-a2x, D2[1] - D0[1], __builtin_fmaf(-a2y, Dy0[W1] - Dy0[-W1], h1));
+                    h2 = __builtin_fmaf(-a2x, D2[2] - D0[2], __builtin_fmaf(-a2z, Dy0[1] - Dy0[-1], h2));
+                    h4 = __builtin_fmaf(-a2y, Dy0[2 * P1 + W1] - Dy0[2 * P1 - W1],
+                                        __builtin_fmaf(-a2z, Dy0[1 * P1 + 1] - Dy0[1 * P1 - 1], h4));
                 }
-                if (imex.w != 0.0f) {
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) h[c] += imex.x * sx_[c] + imex.y * syy1[c] + imex.z * szz1[c];
-                }
+                const float tr = (px + py + pz) * (1.0f / 3.0f);
+                float h[5];
+                h[0] = __builtin_fmaf(g, xx, px - tr) - e[0];
+                h[3] = __builtin_fmaf(g, yy, py - tr) - e[3];
+                h[1] = h1 - e[1];
+                h[2] = h2 - e[2];
+                h[4] = h4 - e[4];
                 float o[5];
                 const long long inpl = (long long)j * Nz + k, pb = (long long)t * plane;
                 if (finish_voxel<NOISE>(B, D, t, j, k, pb + inpl, evolve, shell, q1, h, o, imex.w != 0.0f)) {
@@ -767,12 +790,8 @@ __global__ __launch_bounds__(TK* TJv, (!NOISE && BUF) ? 4 : 3) void step_aniso_k
                 }
             }
         }
-        // the t-1 half of plane t+1's mixed terms: Q(t+1) and d_yQ/d_zQ(t), both resident
-        if constexpr (hasL3) {
-            const float cy = 2.0f * Qn[1 * P2 + lq] * B.hx, cz = 2.0f * Qn[2 * P2 + lq] * B.hx;
-#pragma unroll
-            for (int c = 0; c < 5; ++c) pxm[c] = __builtin_fmaf(cz, dz1[c], cy * dy1[c]);
-        }
+        // the t-1 half of plane t+1's mixed terms: Q(t+1) and the differences of plane t, both resident
+        if constexpr (hasL3) fold_x(Qn[1 * P2 + lq], Qn[2 * P2 + lq], ry1, rz1, pxm);
         // loads for the next iteration: row mask of plane t+1, own and ring Q(t+3), halo of
         // lead plane t+3
         int4 mnext = make_int4(0, 0, 0, 0);
