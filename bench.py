@@ -103,8 +103,8 @@ def workload_params(q, n=256, R=120.0, workload="droplet_l1l2l3_256"):
 
 def profiled(tag, key, workload, variant, planes):
     """A figure recorded in the committed counter summaries under profiles/ as a line
-    `<tag> <workload> variant=V planes=P <key>=X`, for the same workload and tuning;
-    (None, None) if that combination was not profiled."""
+    `<tag> <workload> variant=V planes=P <key>=X`, for the same workload and tuning (variant
+    x2: the two-step kernel); (None, None) if that combination was not profiled."""
     pdir = os.path.join(ROOT, "profiles")
     if not os.path.isdir(pdir):
         return None, None
@@ -116,7 +116,7 @@ def profiled(tag, key, workload, variant, planes):
                 continue
             f = line.split()
             kv = dict(x.split("=", 1) for x in f[2:])
-            if f[1] == workload and int(kv["variant"]) == variant and int(kv["planes"]) == planes:
+            if f[1] == workload and kv["variant"] == str(variant) and int(kv["planes"]) == planes:
                 return float(kv[key]), "profiles/" + fn
     return None, None
 
@@ -223,6 +223,8 @@ def run_ladder(q, args):
             tune = {"variant": v, "planes_per_tile": ch, "fixed": True}
         else:
             tune = s.autotune(10)
+        if args.no_fusion:
+            s.set_step_fusion(False)
         log(f"tuning: {tune}")
         advance = (lambda n: s.step_graph(n, args.graph)) if args.graph else s.step
         advance(args.warmup)
@@ -231,7 +233,10 @@ def run_ladder(q, args):
         wall = time.perf_counter() - t0
         main_ms, face_ms = s.profile_steps(min(100, args.steps))
         work = s.work()
+        fusion = s.step_fusion()
     log(f"batched: {1e3 * wall / args.steps:.4f} ms/step")
+    spl = 2 if fusion["active"] else 1  # steps per stencil launch
+    tile_vox = fusion["tile_voxels"] if fusion["active"] else work["tile_voxels"]
     solvers = []
     try:
         for p, Q in zip(params, fields):
@@ -249,7 +254,7 @@ def run_ladder(q, args):
             sv.close()
     log(f"one context per droplet: {1e3 * wall_plain / args.steps:.4f} ms/step")
     value = total * args.steps / wall
-    traffic_bytes = BYTES_PER_VOXEL * work["tile_voxels"]
+    traffic_bytes = BYTES_PER_VOXEL * tile_vox * spl
     achieved = traffic_bytes / (main_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC, "value": value, "unit": "voxel-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -257,8 +262,9 @@ def run_ladder(q, args):
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded isotropic start, reference init_mode 2)",
         "config": {"workload": "quench-rate ladder: 64 droplets R=16..48 in (2R+8)^3 boxes, ramp 310->300 K over "
                                "1e3..1e5 steps, strong/weak anchoring, L1=6e-12, one launch per step",
-                   "box_voxels": total, "tile_voxels": work["tile_voxels"], "active_voxels": work["active_voxels"],
-                   "graph_chunk": args.graph, "autotune": tune, "fused_stencil_ms": main_ms, "face_ms": face_ms,
+                   "box_voxels": total, "tile_voxels": tile_vox, "active_voxels": work["active_voxels"],
+                   "graph_chunk": args.graph, "autotune": tune, "steps_per_launch": spl, "step_fusion": fusion,
+                   "fused_stencil_ms": main_ms, "stencil_ms_per_step": main_ms / spl, "face_ms": face_ms,
                    "plain_voxel_updates_per_s": total * args.steps / wall_plain,
                    "batched_over_plain": wall_plain / wall},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -604,6 +610,8 @@ def main():
                          "network's first jumps from radial starts reshape the droplet cores)")
     ap.add_argument("--surr-iters", type=int, default=1000,
                     help="surrogate_192: Adam iterations per group of 16 trajectories")
+    ap.add_argument("--no-fusion", action="store_true",
+                    help="run every step on its own (no two-step launches for L1-only contexts)")
     ap.add_argument("--workload", default=WORKLOADS[0], choices=WORKLOADS,
                     help="droplet_l1l2l3_256 is the headline line; the others are the periodic noisy quench, "
                          "the 64^3 L1 droplet, the 64-droplet quench ladder and the learned warm start")
@@ -642,6 +650,8 @@ def main():
             tune = {"variant": v, "planes_per_tile": ch, "fixed": True}
         else:
             tune = s.autotune(10)
+        if args.no_fusion:
+            s.set_step_fusion(False)
         log(f"tuning: {tune}")
         L_stab = p.L1 if p.L1 else p.kappa
 
@@ -662,8 +672,14 @@ def main():
         log(f"timed {args.steps} steps: wall {wall:.3f} s, event {ev_ms:.4f} ms/step")
         main_ms, face_ms = s.profile_steps(min(100, args.steps))
         work = s.work()
+        fusion = s.step_fusion()
         obs = s.observables()
-        log(f"fused stencil {main_ms:.4f} ms, faces {face_ms:.4f} ms; F={obs['total']:.6e}")
+        log(f"stencil launch {main_ms:.4f} ms ({'two steps' if fusion['active'] else 'one step'}), faces "
+            f"{face_ms:.4f} ms; F={obs['total']:.6e}")
+    # a context that pairs its steps runs the two-step kernel: one launch = two steps over its tiles
+    spl = 2 if fusion["active"] else 1
+    kern = ("x2", fusion["planes_per_tile"]) if fusion["active"] else (int(tune["variant"]), int(tune["planes_per_tile"]))
+    tile_vox = fusion["tile_voxels"] if fusion["active"] else work["tile_voxels"]
     relax = None
     if args.workload == WORKLOADS[0] and args.relax_max_steps > 0 and not args.imex:
         relax = time_to_converge(q, p, args)
@@ -673,14 +689,13 @@ def main():
     ms_per_step = 1e3 * wall / args.steps
     # one voxel update = one droplet voxel stepped (what the reference's update kernel changes per step)
     value = work["active_voxels"] * args.steps / wall
-    algo_bytes = BYTES_PER_VOXEL * work["active_voxels"]
-    tile_bytes = BYTES_PER_VOXEL * work["tile_voxels"]
+    # algorithmic bytes per launch: 40 B per droplet voxel and step, times the steps of a launch
+    algo_bytes = BYTES_PER_VOXEL * work["active_voxels"] * spl
+    tile_bytes = BYTES_PER_VOXEL * tile_vox * spl
     achieved = algo_bytes / (main_ms * 1e-3) / 1e9
-    hbm_bytes, hbm_src = measured_traffic(args.workload, int(tune["variant"]), int(tune["planes_per_tile"]))
-    valu_busy, valu_src = profiled("bench-valu:", "busy", args.workload, int(tune["variant"]),
-                                   int(tune["planes_per_tile"]))
-    flop_vox, flop_src = profiled("bench-flop:", "per_tile_voxel", args.workload, int(tune["variant"]),
-                                  int(tune["planes_per_tile"]))
+    hbm_bytes, hbm_src = measured_traffic(args.workload, *kern)
+    valu_busy, valu_src = profiled("bench-valu:", "busy", args.workload, *kern)
+    flop_vox, flop_src = profiled("bench-flop:", "per_tile_voxel", args.workload, *kern)
     compute = None
     if flop_vox is not None:
         tflops = flop_vox * work["tile_voxels"] / (main_ms * 1e-3) / 1e12
@@ -699,10 +714,12 @@ def main():
         "data": "synthetic (seeded random-director droplet, reference init_mode 0)",
         "config": {"workload": describe(p, args.workload) + (f", semi-implicit J={args.imex}" if args.imex else ""),
                    "box_voxels": work["box_voxels"], "active_voxels": work["active_voxels"],
-                   "tile_voxels": work["tile_voxels"], "graph_chunk": args.graph, "autotune": tune,
+                   "tile_voxels": tile_vox, "graph_chunk": args.graph, "autotune": tune,
+                   "steps_per_launch": spl, "step_fusion": fusion,
                    "box_voxel_updates_per_s": work["box_voxels"] * args.steps / wall,
-                   "tile_voxel_updates_per_s": work["tile_voxels"] * args.steps / wall,
-                   "fused_stencil_ms": main_ms, "face_ms": face_ms, "event_ms_per_step": ev_ms,
+                   "tile_voxel_updates_per_s": tile_vox * args.steps / wall,
+                   "fused_stencil_ms": main_ms, "stencil_ms_per_step": main_ms / spl, "face_ms": face_ms,
+                   "event_ms_per_step": ev_ms,
                    "final_total_F": obs["total"], "algorithmic_bytes_per_launch": algo_bytes,
                    "tile_bytes_per_launch": tile_bytes,
                    "valu_busy": None if valu_busy is None else {"frac": valu_busy, "source": valu_src},

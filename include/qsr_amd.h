@@ -178,9 +178,23 @@ int qsr_set_box_active(void* ctx, int box, int active);
 /* Set the stencil variant (tile shape: 0 = 64 k x 4 j rows, 1 = 32 k x 8 j rows)
  * and planes per tile explicitly. */
 int qsr_set_tuning(void* ctx, int variant, int planes_per_tile);
+/* Two explicit steps per launch (temporal blocking).  With on = 1 (the default)
+ * a context whose boxes use L1 / kappa elasticity only, without noise and with
+ * no box-face voxel that can change, runs its steps in pairs through a kernel
+ * that reads the field once and writes it once per pair (the intermediate
+ * field stays on chip); results equal single steps bit for bit.  on = 0 runs
+ * every step on its own.  planes_per_tile > 0 sets the pair kernel's planes per
+ * tile (qsr_autotune also tunes it).  No reference counterpart: it replaces
+ * two passes of the reference's per-iteration kernel chain (QSR.cu:3223-3238). */
+int qsr_set_step_fusion(void* ctx, int on, int planes_per_tile);
+/* Whether qsr_step pairs steps for this context, the pair kernel's planes per
+ * tile and the voxels its tiles cover (0 when it does not pair). */
+int qsr_step_fusion_info(void* ctx, int* active, int* planes_per_tile, uint64_t* tile_voxels);
 /* Instrumentation: run nsteps (<= 512) with hipEvents recorded on the
  * context stream around every launch; returns the average duration (ms) of the
- * fused stencil launch and of the box-face launch. */
+ * stencil launch and of the box-face launch.  A context that pairs its steps
+ * runs nsteps / 2 two-step launches, and main_ms is the duration of one of them
+ * (two steps). */
 int qsr_profile_steps(void* ctx, int nsteps, double* main_ms, double* face_ms);
 /* Work description of one step: bytes moved per step if each active voxel is
  * read and written once (FP32 SoA, 40 B/voxel), voxels covered by launched

@@ -112,6 +112,12 @@ struct Ctx {
     std::vector<int4> h_tiles;
     int4* d_tiles = nullptr;
     int ntiles = 0;
+    // two-step L1 kernel: its own tile table (X2_OUT_K x X2_OUT_J outputs, ch2 planes)
+    bool fuse = true;      // pair steps when the context allows it (qsr_set_step_fusion)
+    int ch2 = 16;
+    int4* d_tiles2 = nullptr;
+    int ntiles2 = 0;
+    long long tile_voxels2 = 0;
     long long tile_voxels = 0, active_voxels = 0, box_voxels = 0;
     qsrk::RowMask* d_step_rows = nullptr;
     qsrk::RowMask* d_obs_rows = nullptr;
@@ -150,7 +156,7 @@ struct Ctx {
         if (gexec) discard(hipGraphExecDestroy(gexec));
         if (graph) discard(hipGraphDestroy(graph));
         for (float* p : q) discard(hipFree(p));
-        for (void* p : {(void*)d_boxes, (void*)d_tiles, (void*)d_step_rows, (void*)d_obs_rows, (void*)d_face_list,
+        for (void* p : {(void*)d_boxes, (void*)d_tiles, (void*)d_tiles2, (void*)d_step_rows, (void*)d_obs_rows, (void*)d_face_list,
                         (void*)d_dyn, (void*)d_part, (void*)d_out, (void*)d_nf_slice[0], (void*)d_nf_slice[1],
                         (void*)d_nf3, (void*)q_rhs, (void*)d_imex, (void*)d_jcoef, (void*)d_vox_prefix,
                         (void*)d_jflags, (void*)d_rhs0, (void*)d_mpart, (void*)d_mout})
@@ -259,22 +265,22 @@ bool live_k_bounds(const Box& b, int ib, int ie, int jb, int je, int& lo, int& h
     return lo <= hi;
 }
 
-int build_tiles(Ctx& c) {
-    // tile shape of the current variant (k extent x j rows)
-    const int tk = qsrk::tile_k(c.variant), tj = qsrk::tile_j(c.variant);
+// Tiles of tk x tj outputs (k x j) and ch planes over the active boxes, in XCD-aware launch
+// order, uploaded to *d (ntiles entries, padding included); voxels = outputs covered.
+int make_tiles(Ctx& c, int tk, int tj, int ch, int4** d, int* ntiles, long long* voxels, std::vector<int4>* host) {
     std::vector<int4> nat;
-    c.tile_voxels = 0;
+    *voxels = 0;
     for (size_t bi = 0; bi < c.boxes.size(); ++bi) {
         const Box& b = c.boxes[bi];
         if (!b.active) continue;
         const int Nx = b.p.Nx, Ny = b.p.Ny, Nz = b.p.Nz;
-        for (int i0 = 0; i0 < Nx; i0 += c.ch)
+        for (int i0 = 0; i0 < Nx; i0 += ch)
             for (int j0 = 0; j0 < Ny; j0 += tj) {
-                const long long ij = (long long)std::min(c.ch, Nx - i0) * std::min(tj, Ny - j0);
+                const long long ij = (long long)std::min(ch, Nx - i0) * std::min(tj, Ny - j0);
                 if (b.p.periodic) {  // every voxel steps: tiles on the k grid
                     for (int k0 = 0; k0 < Nz; k0 += tk) {
                         nat.push_back(make_int4(int(bi), i0, j0, k0));
-                        c.tile_voxels += ij * std::min(tk, Nz - k0);
+                        *voxels += ij * std::min(tk, Nz - k0);
                     }
                     continue;
                 }
@@ -282,7 +288,7 @@ int build_tiles(Ctx& c) {
                 // with the leftover of the last tile split over both ends (the kernels clamp
                 // loads at the box faces and write interior voxels only)
                 int lo, hi;
-                if (!live_k_bounds(b, std::max(i0, 1), std::min(i0 + c.ch, Nx - 1), std::max(j0, 1),
+                if (!live_k_bounds(b, std::max(i0, 1), std::min(i0 + ch, Nx - 1), std::max(j0, 1),
                                    std::min(j0 + tj, Ny - 1), lo, hi))
                     continue;
                 const int span = hi - lo + 1, n = (span + tk - 1) / tk;
@@ -292,7 +298,7 @@ int build_tiles(Ctx& c) {
                     const int k0 = start + t * tk;
                     if (k0 >= Nz) break;
                     nat.push_back(make_int4(int(bi), i0, j0, k0));
-                    c.tile_voxels += ij * std::min(tk, Nz - k0);
+                    *voxels += ij * std::min(tk, Nz - k0);
                 }
             }
     }
@@ -301,19 +307,40 @@ int build_tiles(Ctx& c) {
     // sweeps one contiguous run of neighbouring tiles and shares their halos.
     const int T = int(nat.size());
     const int q = (T + 7) / 8;
-    c.h_tiles.assign(size_t(8) * q, make_int4(-1, 0, 0, 0));
+    std::vector<int4> order(size_t(8) * q, make_int4(-1, 0, 0, 0));
     for (int x = 0; x < 8; ++x)
         for (int r = 0; r < q; ++r) {
             const int n = x * q + r;
-            if (n < T) c.h_tiles[size_t(8) * r + x] = nat[n];
+            if (n < T) order[size_t(8) * r + x] = nat[n];
         }
-    c.ntiles = int(c.h_tiles.size());
-    discard(hipFree(c.d_tiles));
-    c.d_tiles = nullptr;
-    if (c.ntiles == 0) return QSR_OK;
-    HIP_TRY(hipMalloc(&c.d_tiles, sizeof(int4) * c.ntiles));
-    HIP_TRY(hipMemcpy(c.d_tiles, c.h_tiles.data(), sizeof(int4) * c.ntiles, hipMemcpyHostToDevice));
+    *ntiles = int(order.size());
+    discard(hipFree(*d));
+    *d = nullptr;
+    if (*ntiles > 0) {
+        HIP_TRY(hipMalloc(d, sizeof(int4) * size_t(*ntiles)));
+        HIP_TRY(hipMemcpy(*d, order.data(), sizeof(int4) * size_t(*ntiles), hipMemcpyHostToDevice));
+    }
+    if (host) *host = std::move(order);
     return QSR_OK;
+}
+
+// Can this context pair its explicit steps in the two-step kernel?  L1 / kappa only, no
+// noise, no box face that can change, 32-bit buffer offsets.
+bool fusable(const Ctx& c) { return c.fuse && c.elastic == 0 && !c.noise && c.nface == 0 && c.cs < (1LL << 32) / 20; }
+
+int build_tiles(Ctx& c) {
+    // tile shape of the current variant (k extent x j rows)
+    if (int rc = make_tiles(c, qsrk::tile_k(c.variant), qsrk::tile_j(c.variant), c.ch, &c.d_tiles, &c.ntiles,
+                            &c.tile_voxels, &c.h_tiles))
+        return rc;
+    if (!fusable(c)) {
+        discard(hipFree(c.d_tiles2));
+        c.d_tiles2 = nullptr;
+        c.ntiles2 = 0;
+        c.tile_voxels2 = 0;
+        return QSR_OK;
+    }
+    return make_tiles(c, qsrk::X2_OUT_K, qsrk::X2_OUT_J, c.ch2, &c.d_tiles2, &c.ntiles2, &c.tile_voxels2, nullptr);
 }
 
 // Box-face voxels that can change in a step (the face kernel's live set: evolving
@@ -461,10 +488,24 @@ int ensure_partials(Ctx& c, long long doubles) {
     return QSR_OK;
 }
 
-// Enqueue steps [0, nsteps) of the current chunk (dyn rows already uploaded).
-int enqueue_steps(Ctx& c, int nsteps) {
+// Enqueue steps [0, nsteps) of the current chunk (dyn rows already uploaded).  Pairs of
+// steps go to the two-step kernel where the context allows it (one buffer swap per
+// pair); even_swaps keeps the number of swaps of the same parity as nsteps single steps
+// (an even number of pairs), as a replayed graph needs.
+int enqueue_steps(Ctx& c, int nsteps, bool even_swaps = false) {
     const int nb = int(c.boxes.size());
-    for (int s = 0; s < nsteps; ++s) {
+    int s = 0;
+    if (fusable(c) && c.ntiles2 > 0) {
+        int pairs = nsteps / 2;
+        if (even_swaps) pairs &= ~1;
+        for (int p = 0; p < pairs; ++p, s += 2) {
+            const qsrk::StepDyn* dyn = c.d_dyn + (size_t)s * nb;
+            HIP_TRY(qsrk::launch_step2(c.ntiles2, c.st, c.q[c.cur], c.q[1 - c.cur], c.cs, c.d_boxes, c.d_tiles2, dyn,
+                                       dyn + nb, c.d_step_rows, c.ch2));
+            c.cur = 1 - c.cur;
+        }
+    }
+    for (; s < nsteps; ++s) {
         const float* qin = c.q[c.cur];
         float* qout = c.q[1 - c.cur];
         const qsrk::StepDyn* dyn = c.d_dyn + (size_t)s * nb;
@@ -614,7 +655,7 @@ int qsr_step_graph(void* ctx, int nsteps, int chunk) {
         HIP_TRY(hipStreamSynchronize(c->st));
         const int save = c->cur;
         HIP_TRY(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_steps(*c, chunk);
+        const int rc = enqueue_steps(*c, chunk, true);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(c->st, &g);
         c->cur = save;
@@ -660,33 +701,39 @@ int qsr_profile_steps(void* ctx, int nsteps, double* main_ms, double* face_ms) {
     std::vector<hipEvent_t> ev(3 * size_t(nsteps));
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
     const int nb = int(c->boxes.size());
-    for (int s = 0; s < nsteps; ++s) {
+    // a context that pairs its steps is timed per two-step launch (nsteps / 2 of them)
+    const bool pairs = fusable(*c) && c->ntiles2 > 0 && nsteps >= 2;
+    const int nl = pairs ? nsteps / 2 : nsteps;
+    for (int l = 0; l < nl; ++l) {
         const float* qin = c->q[c->cur];
         float* qout = c->q[1 - c->cur];
-        const qsrk::StepDyn* dyn = c->d_dyn + (size_t)s * nb;
-        HIP_TRY(hipEventRecord(ev[3 * s], c->st));
-        if (c->ntiles > 0)
+        const qsrk::StepDyn* dyn = c->d_dyn + (size_t)(pairs ? 2 * l : l) * nb;
+        HIP_TRY(hipEventRecord(ev[3 * l], c->st));
+        if (pairs)
+            HIP_TRY(qsrk::launch_step2(c->ntiles2, c->st, qin, qout, c->cs, c->d_boxes, c->d_tiles2, dyn, dyn + nb,
+                                       c->d_step_rows, c->ch2));
+        else if (c->ntiles > 0)
             HIP_TRY(qsrk::launch_step(c->elastic, c->variant, c->noise, c->ntiles, c->st, qin, qout, c->cs, c->d_boxes, c->d_tiles,
                                       dyn, nb, c->d_step_rows, c->ch, nullptr));
-        HIP_TRY(hipEventRecord(ev[3 * s + 1], c->st));
+        HIP_TRY(hipEventRecord(ev[3 * l + 1], c->st));
         HIP_TRY(qsrk::launch_faces(c->nface, c->st, qin, qout, c->cs, c->d_boxes, dyn, c->d_face_list,
                                    c->d_step_rows));
-        HIP_TRY(hipEventRecord(ev[3 * s + 2], c->st));
+        HIP_TRY(hipEventRecord(ev[3 * l + 2], c->st));
         c->cur = 1 - c->cur;
     }
     HIP_TRY(hipStreamSynchronize(c->st));
-    advance_iters(*c, nsteps);
+    advance_iters(*c, pairs ? 2 * nl : nl);
     double tm = 0, tf = 0;
-    for (int s = 0; s < nsteps; ++s) {
+    for (int l = 0; l < nl; ++l) {
         float a = 0, b = 0;
-        HIP_TRY(hipEventElapsedTime(&a, ev[3 * s], ev[3 * s + 1]));
-        HIP_TRY(hipEventElapsedTime(&b, ev[3 * s + 1], ev[3 * s + 2]));
+        HIP_TRY(hipEventElapsedTime(&a, ev[3 * l], ev[3 * l + 1]));
+        HIP_TRY(hipEventElapsedTime(&b, ev[3 * l + 1], ev[3 * l + 2]));
         tm += a;
         tf += b;
     }
     for (auto& e : ev) discard(hipEventDestroy(e));
-    *main_ms = tm / nsteps;
-    *face_ms = tf / nsteps;
+    *main_ms = tm / nl;
+    *face_ms = tf / nl;
     return QSR_OK;
 }
 
@@ -745,6 +792,32 @@ int qsr_autotune(void* ctx, int trial_steps, int* best_variant, int* best_ch, do
     c->variant = bv;
     c->ch = bch;
     if (int rc = build_tiles(*c)) return rc;
+    if (fusable(*c)) {
+        // planes per tile of the two-step kernel, scored the same way (per step: launch / 2)
+        if (int rc = fill_dyn(*c, 2)) return rc;
+        std::vector<double> m2(chs.size(), 1e300);
+        for (int sweep = 0; sweep < kSweeps; ++sweep)
+            for (size_t ic = 0; ic < chs.size(); ++ic) {
+                c->ch2 = chs[ic];
+                if (int rc = build_tiles(*c)) return rc;
+                for (int s = -1; s < trial_steps; ++s) {
+                    if (s == 0) HIP_TRY(hipEventRecord(c->ev[0], c->st));
+                    HIP_TRY(qsrk::launch_step2(c->ntiles2, c->st, c->q[c->cur], c->q[1 - c->cur], c->cs, c->d_boxes,
+                                               c->d_tiles2, c->d_dyn, c->d_dyn + nb, c->d_step_rows, c->ch2));
+                }
+                HIP_TRY(hipEventRecord(c->ev[1], c->st));
+                HIP_TRY(hipEventSynchronize(c->ev[1]));
+                float ms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+                m2[ic] = std::min(m2[ic], double(ms) / (2.0 * trial_steps));
+            }
+        size_t bi = 0;
+        for (size_t ic = 1; ic < chs.size(); ++ic)
+            if (m2[ic] < m2[bi]) bi = ic;
+        c->ch2 = chs[bi];
+        if (int rc = build_tiles(*c)) return rc;
+        best = std::min(best, m2[bi]);
+    }
     if (c->gexec) discard(hipGraphExecDestroy(c->gexec));
     if (c->graph) discard(hipGraphDestroy(c->graph));
     c->gexec = nullptr;
@@ -781,6 +854,31 @@ int qsr_set_box_active(void* ctx, int box, int active) {
     if (c->graph) discard(hipGraphDestroy(c->graph));
     c->gexec = nullptr;
     c->graph = nullptr;
+    return QSR_OK;
+}
+
+int qsr_set_step_fusion(void* ctx, int on, int planes_per_tile) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || planes_per_tile > 4096) return fail(QSR_E_ARG, "planes_per_tile must be <= 4096");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    c->fuse = on != 0;
+    if (planes_per_tile > 0) c->ch2 = planes_per_tile;
+    if (int rc = build_tiles(*c)) return rc;
+    if (c->gexec) discard(hipGraphExecDestroy(c->gexec));
+    if (c->graph) discard(hipGraphDestroy(c->graph));
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    return QSR_OK;
+}
+
+int qsr_step_fusion_info(void* ctx, int* active, int* planes_per_tile, uint64_t* tile_voxels) {
+    Ctx* c = as_ctx(ctx);
+    if (!c) return fail(QSR_E_ARG, "bad argument");
+    const bool on = fusable(*c) && c->ntiles2 > 0;
+    if (active) *active = on ? 1 : 0;
+    if (planes_per_tile) *planes_per_tile = on ? c->ch2 : 0;
+    if (tile_voxels) *tile_voxels = on ? uint64_t(c->tile_voxels2) : 0;
     return QSR_OK;
 }
 

@@ -63,6 +63,11 @@ constexpr int TILE_THREADS = 256;
 constexpr int tile_k(int variant) { return variant == 1 ? 32 : 64; }
 constexpr int tile_j(int variant) { return variant == 1 ? 8 : 4; }
 
+// Two-step L1 kernel: 512 threads over a region of X2_TK k x X2_TJ j points, whose
+// inner (X2_TK - 2) x (X2_TJ - 2) points are the tile's outputs.
+constexpr int X2_TK = 32, X2_TJ = 16;
+constexpr int X2_OUT_K = X2_TK - 2, X2_OUT_J = X2_TJ - 2;
+
 }  // namespace qsrk
 
 namespace qsro {
@@ -97,6 +102,11 @@ namespace qsrk {
 hipError_t launch_step(int elastic, int variant, bool noise, int ntiles, hipStream_t st, const float* qin, float* qout,
                        long long cs, const BoxDev* boxes, const int4* tiles, const StepDyn* dyn, int nboxes,
                        const RowMask* rows, int ch, const float4* imex);
+// Two explicit L1 / kappa steps per launch (dyn1, dyn2: the two steps' per-box rows) over
+// tiles of X2_OUT_K x X2_OUT_J outputs; only for contexts without noise, live box faces or
+// 64-bit offsets (5 x cs floats below 4 GiB)
+hipError_t launch_step2(int ntiles, hipStream_t st, const float* qin, float* qout, long long cs, const BoxDev* boxes,
+                        const int4* tiles, const StepDyn* dyn1, const StepDyn* dyn2, const RowMask* rows, int ch);
 // face_list: (box, i, j, k) of every box-face voxel that can change (built on the host)
 hipError_t launch_faces(long long nlive, hipStream_t st, const float* qin, float* qout, long long cs,
                         const BoxDev* boxes, const StepDyn* dyn, const int4* face_list, const RowMask* rows);

@@ -89,22 +89,21 @@ struct Field<true> {
     }
 };
 
-// h = m * [A Q - B (Q^2 - tr(Q^2)/3 I) + C tr(Q^2) Q]   (QSR.cu:399-415)
+// h = m * [A Q - B (Q^2 - tr(Q^2)/3 I) + C tr(Q^2) Q]   (QSR.cu:399-415); the off-diagonal
+// entries of Q^2 are reduced with tr Q = 0 (e.g. (Q^2)_xy = Q_xz Q_yz - Q_xy Q_zz)
 __device__ __forceinline__ void bulk_field(const float q[5], float cA, float cB, float cC, float h[5]) {
     const float xx = q[0], xy = q[1], xz = q[2], yy = q[3], yz = q[4], zz = -(xx + yy);
-    const float q2xx = xx * xx + xy * xy + xz * xz;
-    const float q2xy = xx * xy + xy * yy + xz * yz;
-    const float q2xz = xx * xz + xy * yz + xz * zz;
-    const float q2yy = xy * xy + yy * yy + yz * yz;
-    const float q2yz = xy * xz + yy * yz + yz * zz;
-    const float tr2 = xx * xx + yy * yy + zz * zz + 2.0f * (xy * xy + xz * xz + yz * yz);
+    const float q2xx = __builtin_fmaf(xx, xx, __builtin_fmaf(xy, xy, xz * xz));
+    const float q2yy = __builtin_fmaf(xy, xy, __builtin_fmaf(yy, yy, yz * yz));
+    const float q2zz = __builtin_fmaf(xz, xz, __builtin_fmaf(yz, yz, zz * zz));
+    const float tr2 = q2xx + q2yy + q2zz;
     const float t3 = tr2 * (1.0f / 3.0f);
-    const float g = cA + cC * tr2;
-    h[0] = g * xx - cB * (q2xx - t3);
-    h[1] = g * xy - cB * q2xy;
-    h[2] = g * xz - cB * q2xz;
-    h[3] = g * yy - cB * (q2yy - t3);
-    h[4] = g * yz - cB * q2yz;
+    const float g = __builtin_fmaf(cC, tr2, cA);
+    h[0] = __builtin_fmaf(g, xx, -cB * (q2xx - t3));
+    h[1] = __builtin_fmaf(g, xy, -cB * __builtin_fmaf(xz, yz, -(xy * zz)));
+    h[2] = __builtin_fmaf(g, xz, -cB * __builtin_fmaf(xy, yz, -(xz * yy)));
+    h[3] = __builtin_fmaf(g, yy, -cB * (q2yy - t3));
+    h[4] = __builtin_fmaf(g, yz, -cB * __builtin_fmaf(xy, xz, -(yz * xx)));
 }
 
 // Q0 = S (nu nu - I/3), nu = normalised (x/rx^2, y/ry^2, z/rz^2)  (QSR.cu:60-68, 130-142)
@@ -273,6 +272,19 @@ __device__ __forceinline__ void decode_row4(M m, const BoxDev& B, int k, bool& e
 
 }  // namespace
 
+// L1 / kappa molecular field of one voxel, m-scaled: bulk minus L1 lap Q (QSR.cu:279-323,
+// 399-415), from the neighbour sums xs = Q(i-1) + Q(i+1), ys, zs along y and z:
+// L1 lap Q = lx xs + ly ys + lz zs - l2 Q with l2 = 2 (lx + ly + lz).  The single-step and
+// two-step kernels both call this, so they round every voxel alike.
+__device__ __forceinline__ void l1_field(const BoxDev& B, const StepDyn& D, float l2, const float qc[5],
+                                         const float xs[5], const float ys[5], const float zs[5], float h[5]) {
+    bulk_field(qc, D.cA, B.cB, B.cC, h);
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+        h[c] = __builtin_fmaf(l2, qc[c],
+                              __builtin_fmaf(-B.lz, zs[c], __builtin_fmaf(-B.ly, ys[c], __builtin_fmaf(-B.lx, xs[c], h[c]))));
+}
+
 // ---------------------------------------------------------------------------
 // L1 / kappa kernel: x-register queue + one LDS plane (halo 1), double-buffered.
 // ---------------------------------------------------------------------------
@@ -293,6 +305,7 @@ __global__ __launch_bounds__(TK* TJ, (!NOISE && BUF) ? 7 : 1) void step_l1_kerne
     const BoxDev B = boxes[t.x];
     const StepDyn D = dyn[t.x];
     const float4 imex = imex_tab ? imex_tab[t.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float l2 = uni(2.0f * (B.lx + B.ly + B.lz));
     const int tk = threadIdx.x & (TK - 1), tj = threadIdx.x / TK;
     const int Nx = B.Nx, Ny = B.Ny, Nz = B.Nz;
     const bool per = (B.flags & F_PERIODIC) != 0;
@@ -361,16 +374,21 @@ __global__ __launch_bounds__(TK* TJ, (!NOISE && BUF) ? 7 : 1) void step_l1_kerne
             decode_row4(mcur, B, k, evolve, shell);
             const bool live = (evolve && !((B.flags & F_SKIPSHELL) && shell)) || ((B.flags & F_BC) && shell);
             if (live) {
-                float h[5];
-                bulk_field(qc, D.cA, B.cB, B.cC, h);
+                float h[5], xs[5], ys[5], zs[5];
 #pragma unroll
                 for (int c = 0; c < 5; ++c) {
                     const float* Lc = L + c * PL;
-                    const float q2 = 2.0f * qc[c];
-                    const float sxx = qn[c] + qm[c] - q2, syy = Lc[lrow + W] + Lc[lrow - W] - q2,
-                                szz = Lc[lrow + 1] + Lc[lrow - 1] - q2;
-                    h[c] -= B.lx * sxx + B.ly * syy + B.lz * szz;
-                    if (imex.w != 0.0f) h[c] += imex.x * sxx + imex.y * syy + imex.z * szz;
+                    xs[c] = qn[c] + qm[c];
+                    ys[c] = Lc[lrow + W] + Lc[lrow - W];
+                    zs[c] = Lc[lrow + 1] + Lc[lrow - 1];
+                }
+                l1_field(B, D, l2, qc, xs, ys, zs, h);
+                if (imex.w != 0.0f) {
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) {
+                        const float q2 = 2.0f * qc[c];
+                        h[c] += imex.x * (xs[c] - q2) + imex.y * (ys[c] - q2) + imex.z * (zs[c] - q2);
+                    }
                 }
                 float o[5];
                 const long long inpl = (long long)j * Nz + k, pb = (long long)p * plane;
@@ -388,6 +406,154 @@ __global__ __launch_bounds__(TK* TJ, (!NOISE && BUF) ? 7 : 1) void step_l1_kerne
             hc[c] = hn[c];
         }
         mcur = mnext;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Two L1 / kappa steps in one pass (temporal blocking): the launch reads the field
+// once and writes it once for two explicit steps.  A block of 512 threads covers a
+// region of 32 k x 16 j points and streams along i; step 1 updates every region
+// point, step 2 the inner 30 x 14 points, which are the tile's outputs.  For each
+// plane s of the stream:
+//   * plane s of the input (own column + a one-point ring around the region) goes
+//     into an LDS slot; the own column also runs through a register queue along i;
+//   * step 1 gives the intermediate field at plane s on the whole region (its own
+//     register queue along i and LDS slot);
+//   * step 2 gives output plane s-1 on the inner points.
+// Both LDS planes are double-buffered, so one barrier per plane orders every
+// hand-over.  Usable when no box face can change (the face pass would have to run
+// between the two steps), without noise and without the semi-implicit right-hand
+// side; 32-bit buffer offsets (5 x cs floats below 4 GiB).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(X2_TK* X2_TJ, 5) void step_l1x2_kernel(
+    const float* __restrict__ qin, float* __restrict__ qout, long long cs, const BoxDev* __restrict__ boxes,
+    const int4* __restrict__ tiles, const StepDyn* __restrict__ dyn1, const StepDyn* __restrict__ dyn2,
+    const RowMask* __restrict__ rows, int ch) {
+    constexpr int TK = X2_TK, TJ = X2_TJ, NT = TK * TJ;
+    constexpr int WA = TK + 2, PA = WA * (TJ + 2), SA = 5 * PA;  // input plane with its ring
+    constexpr int PB = NT, SB = 5 * PB;                          // intermediate plane (region)
+    __shared__ float la[2 * SA];
+    __shared__ float lb[2 * SB];
+    const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;  // padding tile of the XCD-ordered list (uniform over the block)
+    const BoxDev B = boxes[t.x];
+    const StepDyn D1 = dyn1[t.x], D2 = dyn2[t.x];
+    const float l2 = uni(2.0f * (B.lx + B.ly + B.lz));
+    const int tid = threadIdx.x, tk = tid & (TK - 1), tj = tid / TK;
+    const int Nx = B.Nx, Ny = B.Ny, Nz = B.Nz;
+    const bool per = (B.flags & F_PERIODIC) != 0;
+    int ib = t.y, ie = min(t.y + ch, Nx);
+    if (!per) {
+        ib = max(ib, 1);
+        ie = min(ie, Nx - 1);
+    }
+    if (ib >= ie) return;  // uniform over the block
+    // region point of this thread (tile outputs start at (t.z, t.w), the region one point before)
+    const int j = t.z - 1 + tj, k = t.w - 1 + tk;
+    const bool inner = tj >= 1 && tj <= TJ - 2 && tk >= 1 && tk <= TK - 2;
+    // points a step may change: the interior of a clamped box, all of a periodic one
+    const bool jk_in = per ? true : (j >= 1 && j <= Ny - 2 && k >= 1 && k <= Nz - 2);
+    const bool out_ok = inner && (per ? (j < Ny && k < Nz) : jk_in);
+    const int jc = wrap_or_clamp(j, Ny, per), kc = wrap_or_clamp(k, Nz, per);
+    // ring point of the region (rows t.z-2 and t.z-1+TJ, columns t.w-2 and t.w-1+TK), one per thread
+    const bool ring = tid < 2 * TK + 2 * TJ;
+    int hj = 0, hk = 0;  // region coordinates, -1 .. TJ and -1 .. TK
+    if (tid < TK) { hj = -1; hk = tid; }
+    else if (tid < 2 * TK) { hj = TJ; hk = tid - TK; }
+    else if (tid < 2 * TK + TJ) { hj = tid - 2 * TK; hk = -1; }
+    else if (ring) { hj = tid - 2 * TK - TJ; hk = TK; }
+    const long long plane = (long long)Ny * Nz;
+    const long long o_own = (long long)jc * Nz + kc;
+    const long long o_ring = (long long)wrap_or_clamp(t.z - 1 + hj, Ny, per) * Nz + wrap_or_clamp(t.w - 1 + hk, Nz, per);
+    const Field<true> F(qin, qout, cs, B.off);
+    const int pa = (tj + 1) * WA + tk + 1, pr = (hj + 1) * WA + hk + 1, pb = tj * TK + tk;
+    const int jrow = min(max(j, 0), Ny - 1);
+    auto pl = [&](int p) { return (long long)wrap_or_clamp(p, Nx, per) * plane; };
+    auto ld5 = [&](int p, long long o, float v[5]) {
+        const long long b = pl(p);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) v[c] = F.ld(c, b, o);
+    };
+    auto row = [&](int p) { return load_row3(rows, B, min(max(p, 0), Nx - 1), jrow); };
+    // one explicit step at this thread's point of plane p: qm, qc, qn along i, in-plane
+    // neighbours from slot S (row stride W, own position pos); out = qc where nothing changes
+    auto step_at = [&](const StepDyn& D, int p, Row3 m, const float qm[5], const float qc[5], const float qn[5],
+                       const float* S, int PS, int W, int pos, float out[5]) -> bool {
+        bool evolve, shell;
+        decode_row4(m, B, k, evolve, shell);
+        const bool live = (evolve && !((B.flags & F_SKIPSHELL) && shell)) || ((B.flags & F_BC) && shell);
+        if (!live) return false;
+        float h[5], xs[5], ys[5], zs[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            const float* Sc = S + c * PS;
+            xs[c] = qn[c] + qm[c];
+            ys[c] = Sc[pos + W] + Sc[pos - W];
+            zs[c] = Sc[pos + 1] + Sc[pos - 1];
+        }
+        l1_field(B, D, l2, qc, xs, ys, zs, h);
+        return finish_voxel<false>(B, D, p, j, k, 0, evolve, shell, qc, h, out);
+    };
+
+    // queues: input Q(s-1), Q(s), Q(s+1) (+ Q(s+2) in flight), intermediate Q'(s-2), Q'(s-1)
+    const int s0 = ib - 1;
+    float qm[5], qc[5], qn[5], hc[5] = {}, pm[5] = {}, pc[5] = {};
+    ld5(s0 - 1, o_own, qm);
+    ld5(s0, o_own, qc);
+    ld5(s0 + 1, o_own, qn);
+    if (ring) ld5(s0, o_ring, hc);
+    Row3 ms = row(s0), mu{0, 0, 0};  // row masks of planes s (step 1) and s-1 (step 2)
+    for (int s = s0; s <= ie; ++s) {
+        const bool more = s < ie;
+        float qf[5], hn[5] = {};
+        Row3 mn{0, 0, 0};
+        if (more) {
+            ld5(s + 2, o_own, qf);
+            if (ring) ld5(s + 1, o_ring, hn);
+            mn = row(s + 1);
+        }
+        float* A = la + (s & 1) * SA;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            A[c * PA + pa] = qc[c];
+            if (ring) A[c * PA + pr] = hc[c];
+        }
+        lds_barrier();
+        // step 1 at plane s
+        float p1[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) p1[c] = qc[c];
+        if (jk_in && (per || (s >= 1 && s <= Nx - 2))) {
+            float o[5];
+            if (step_at(D1, s, ms, qm, qc, qn, A, PA, WA, pa, o)) {
+#pragma unroll
+                for (int c = 0; c < 5; ++c) p1[c] = o[c];
+            }
+        }
+        float* Bs = lb + (s & 1) * SB;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) Bs[c * PB + pb] = p1[c];
+        // step 2 at plane s-1 from Q'(s-2), Q'(s-1) (slot written last plane) and Q'(s)
+        const int u = s - 1;
+        if (u >= ib && out_ok) {
+            float o[5];
+            if (step_at(D2, u, mu, pm, pc, p1, lb + (u & 1) * SB, PB, TK, pb, o)) {
+                const long long inpl = (long long)j * Nz + k, ub = (long long)u * plane;
+#pragma unroll
+                for (int c = 0; c < 5; ++c) F.st(c, ub, inpl, o[c]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            pm[c] = pc[c];
+            pc[c] = p1[c];
+            qm[c] = qc[c];
+            qc[c] = qn[c];
+            qn[c] = more ? qf[c] : qn[c];
+            hc[c] = hn[c];
+        }
+        mu = ms;
+        ms = mn;
     }
 }
 
@@ -1203,6 +1369,14 @@ hipError_t launch_step(int elastic, int variant, bool noise, int ntiles, hipStre
         go_step<1>(elastic, noise, buf, grid, block, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex);
     else
         go_step<0>(elastic, noise, buf, grid, block, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex);
+    return hipGetLastError();
+}
+
+hipError_t launch_step2(int ntiles, hipStream_t st, const float* qin, float* qout, long long cs, const BoxDev* boxes,
+                        const int4* tiles, const StepDyn* dyn1, const StepDyn* dyn2, const RowMask* rows, int ch) {
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(step_l1x2_kernel, dim3(ntiles), dim3(X2_TK * X2_TJ), 0, st, qin, qout, cs, boxes, tiles, dyn1,
+                       dyn2, rows, ch);
     return hipGetLastError();
 }
 

@@ -72,7 +72,8 @@ EXPORTS = (
     "qsr_create", "qsr_create_batch", "qsr_destroy", "qsr_last_error", "qsr_device_count",
     "qsr_upload_q", "qsr_download_q", "qsr_download_nematic", "qsr_step", "qsr_step_graph",
     "qsr_step_imex", "qsr_observables", "qsr_box_info_get", "qsr_set_iteration", "qsr_last_step_ms",
-    "qsr_step_work", "qsr_profile_steps", "qsr_autotune", "qsr_set_tuning", "qsr_S_eq", "qsr_dt_estimate", "qsr_init_field",
+    "qsr_step_work", "qsr_profile_steps", "qsr_autotune", "qsr_set_tuning", "qsr_set_step_fusion",
+    "qsr_step_fusion_info", "qsr_S_eq", "qsr_dt_estimate", "qsr_init_field",
     "qsr_load_qtensor_file", "qsr_save_qtensor_file", "qsr_set_box_active",
     "qsr_surr_num_params", "qsr_surr_create", "qsr_surr_destroy", "qsr_surr_init_params", "qsr_surr_set_params",
     "qsr_surr_get_params", "qsr_surr_save", "qsr_surr_load", "qsr_surr_eval", "qsr_surr_adam_step",
@@ -117,6 +118,8 @@ def load():
     L.qsr_profile_steps.argtypes = [vp, C.c_int, dp, dp]
     L.qsr_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), dp]
     L.qsr_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    L.qsr_set_step_fusion.argtypes = [vp, C.c_int, C.c_int]
+    L.qsr_step_fusion_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_uint64)]
     L.qsr_set_box_active.argtypes = [vp, C.c_int, C.c_int]
     L.qsr_step_work.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.qsr_S_eq.restype = C.c_double
@@ -325,7 +328,8 @@ class Solver:
         return v.value
 
     def profile_steps(self, n: int):
-        """(main-kernel ms, face-kernel ms) averaged over n steps, HIP events."""
+        """(main-kernel ms, face-kernel ms) per launch, HIP events: n single-step launches,
+        or n / 2 two-step launches when the context pairs its steps (see step_fusion)."""
         a, b = C.c_double(), C.c_double()
         _check(load().qsr_profile_steps(self._h, int(n), C.byref(a), C.byref(b)))
         return a.value, b.value
@@ -338,6 +342,16 @@ class Solver:
 
     def set_tuning(self, variant: int, planes_per_tile: int):
         _check(load().qsr_set_tuning(self._h, int(variant), int(planes_per_tile)))
+
+    def set_step_fusion(self, on: bool, planes_per_tile: int = 0):
+        """Pair explicit steps in the two-step kernel where the context allows it (default on)."""
+        _check(load().qsr_set_step_fusion(self._h, int(bool(on)), int(planes_per_tile)))
+
+    def step_fusion(self) -> dict:
+        """{"active", "planes_per_tile", "tile_voxels"} of the two-step kernel for this context."""
+        a, ch, tv = C.c_int(), C.c_int(), C.c_uint64()
+        _check(load().qsr_step_fusion_info(self._h, C.byref(a), C.byref(ch), C.byref(tv)))
+        return {"active": bool(a.value), "planes_per_tile": ch.value, "tile_voxels": tv.value}
 
     def set_active(self, box: int, active: bool):
         """Hold (False) or resume (True) one box of a batch: held boxes are not stepped."""

@@ -250,3 +250,100 @@ def test_tile_shapes_agree(qsr, orc, name):
     ref = got[(0, 8)]
     for key, Q in got.items():
         assert np.abs(Q - ref).max() <= 1e-6, key
+
+
+# --- two steps per launch (temporal blocking of the L1 / kappa step) ---------------------
+
+PAIR_CASES = ["l1_strong_64", "kappa_strong_odd", "l1_weak_custom", "offcentre_radius", "cap",
+              "isotropic_init_quench_T"]
+
+
+def _pair_vs_single(qsr, params, fields, n, graph_chunk=0):
+    """Advance a context that pairs its steps and one that does not by n steps; return both
+    downloads per box and whether the first one paired."""
+    with qsr.Solver(params) as a, qsr.Solver(params) as b:
+        b.set_step_fusion(False)
+        for t, Q0 in enumerate(fields):
+            a.upload(Q0, box=t)
+            b.upload(Q0, box=t)
+        active = a.step_fusion()["active"]
+        assert not b.step_fusion()["active"]
+        if graph_chunk:
+            a.step_graph(n, chunk=graph_chunk)
+        else:
+            a.step(n)
+        b.step(n)
+        nb = len(fields)
+        return [a.download(box=t) for t in range(nb)], [b.download(box=t) for t in range(nb)], active
+
+
+@pytest.mark.parametrize("name", PAIR_CASES)
+def test_step_pairs_equal_single_steps(qsr, orc, name):
+    """Pairs of steps in one launch give the single-step field bit for bit (odd count: the last
+    step runs alone)."""
+    p, od, Q0, S_shell = _case(qsr, orc, **CASES[name])
+    print(f"[case] pairs {name}", flush=True)
+    (A,), (Bs,), active = _pair_vs_single(qsr, p, [Q0], 7)
+    print(f"  paired={active} max|diff|={np.abs(A - Bs).max():.3e}")
+    assert np.array_equal(A, Bs)
+
+
+def test_step_pairs_are_used_and_match_oracle(qsr, orc):
+    """A droplet clear of its box faces pairs its steps; six paired steps match the oracle."""
+    p, od, Q0, S_shell = _case(qsr, orc, **CASES["offcentre_radius"])
+    with qsr.Solver(p) as s:
+        info = s.step_fusion()
+        print(f"[case] pairs oracle offcentre_radius {info}", flush=True)
+        assert info["active"] and info["tile_voxels"] > 0
+        s.upload(Q0)
+        s.step(6)
+        Qo = od.step(Q0, S_shell, p.dt, steps=6)
+        err = np.abs(s.download() - Qo).max()
+        print(f"  6 paired steps: err {err:.3e}")
+        assert err <= 3 * STEP_TOL
+
+
+def test_step_pairs_follow_the_temperature_ramp(qsr, orc):
+    """Each step of a pair takes its own row of the quench ramp (T, S_shell change every step)."""
+    c = dict(CASES["offcentre_radius"])
+    p, od, Q0, S_shell = _case(qsr, orc, **c)
+    p.protocol = qsr.PROTOCOL_RAMP
+    p.T_high, p.T_low, p.pre_equil_iters, p.ramp_iters = 300.0, 290.0, 1, 6
+    print("[case] pairs ramp", flush=True)
+    (A,), (Bs,), active = _pair_vs_single(qsr, p, [Q0], 9)
+    assert active
+    assert np.array_equal(A, Bs)
+
+
+@pytest.mark.parametrize("dims", [(16, 16, 16), (20, 13, 70)])
+def test_step_pairs_periodic(qsr, orc, dims):
+    """Periodic box without noise: pairs equal single steps and the oracle's periodic step."""
+    T, Ts, kappa, gamma = 301.0, 307.0, 6e-12, 0.1
+    p = qsr.Params(Nx=dims[0], Ny=dims[1], Nz=dims[2], periodic=1, T_star=Ts, T_high=T, kappa=kappa,
+                   gamma=gamma, dt=1.5e-10, seed=4242)
+    g = orc.Grid(*dims)
+    Q0 = _f32(np.random.default_rng(7).normal(0, 0.05, dims + (5,)))
+    print(f"[case] pairs periodic dims={dims}", flush=True)
+    (A,), (Bs,), active = _pair_vs_single(qsr, p, [Q0], 4)
+    assert active
+    assert np.array_equal(A, Bs)
+    A_, B_, C_ = orc.bulk_ABC(p.a, p.b, p.c, T, Ts, 1)
+    Qo = Q0
+    for it in range(4):
+        Qo = orc.periodic_step(Qo, g, A_, B_, C_, kappa, gamma, p.dt, 0.0, 4242, it)
+    assert np.abs(A - Qo).max() <= 2 * STEP_TOL
+
+
+def test_step_pairs_batch_and_graph(qsr, orc):
+    """A batch of droplets with different protocols: paired steps, plain or replayed from graphs
+    of 2, 4 and 6 steps, equal single steps."""
+    built = [_case(qsr, orc, **CASES[n], seed=300 + t) for t, n in enumerate(["offcentre_radius", "cap"])]
+    params = [b[0] for b in built]
+    params[1].protocol = qsr.PROTOCOL_RAMP
+    params[1].T_high, params[1].T_low, params[1].ramp_iters = 300.0, 292.0, 10
+    fields = [b[2] for b in built]
+    for chunk in (0, 2, 4, 6):
+        print(f"[case] pairs batch graph chunk={chunk}", flush=True)
+        A, Bs, active = _pair_vs_single(qsr, params, fields, 12, graph_chunk=chunk)
+        for a, b in zip(A, Bs):
+            assert np.array_equal(a, b)
