@@ -95,6 +95,13 @@ struct ConvTile {
     static constexpr int UPT = (SLOT + NT - 1) / NT;
 };
 
+// LDS of a convolution launch in 16-byte units: weight fragments, the three plane slots, and
+// for the apply epilogue a per-wave 5 x 16S float transpose area.
+template <int CP, int NB, int EPI, int S, int TJ>
+constexpr int conv_lds_units() {
+    return NB * ksteps(CP) * 64 + 3 * ConvTile<CP, S, TJ>::SLOT + (EPI == EPI_APPLY ? TJ * CH_Q * 16 * S / 4 : 0);
+}
+
 template <int CP, int NB, int EPI, int S, int TJ>
 __global__ __launch_bounds__(64 * TJ) void conv_kernel(ConvArgs a, int ch, int ntj, int ntk, int nti) {
     using T = ConvTile<CP, S, TJ>;
@@ -103,6 +110,7 @@ __global__ __launch_bounds__(64 * TJ) void conv_kernel(ConvArgs a, int ch, int n
     extern __shared__ u32x4 smem[];
     u32x4* wl = smem;                    // NB * KS * 64 weight fragments
     u32x4* ring = smem + NB * KS * 64;   // 3 plane slots
+    float* ytr = reinterpret_cast<float*>(ring + 3 * SLOT);  // apply epilogue: 5 x 16S floats per wave
     const u32x4* wf = reinterpret_cast<const u32x4*>(a.frag);
     for (int f = threadIdx.x; f < NB * KS * 64; f += NT) wl[f] = wf[f];
 
@@ -153,9 +161,29 @@ __global__ __launch_bounds__(64 * TJ) void conv_kernel(ConvArgs a, int ch, int n
     store_plane(2, st);
     __syncthreads();
 
+    // apply epilogue: lane l serves voxel k0 + (l mod 32) of the wave's row and components 0-2
+    // (l < 32) or 3-4; its inputs are loaded before the matrix work of the plane
+    const int vx = lane & 31, half = lane >> 5, kx = k0 + vx;
     for (int i = i0; i < i1; ++i) {
         const bool more = i + 1 < i1;
         if (more) load_plane(i + 2, st);  // in flight during the matrix work of plane i
+        bool alive = false;
+        float qo[3] = {}, ro[3] = {};
+        if constexpr (EPI == EPI_APPLY) {
+            alive = j < a.Y && kx < a.Z && changeable(a.box, a.rows, i, j, kx);
+            if (alive) {
+                const long long v = bbase + (long long)i * YZ + (long long)j * a.Z + kx;
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    const int chn = half ? 3 + m : m;
+                    if (chn < CH_Q) {
+                        const long long idx = (long long)chn * a.cs + a.box.off + v;
+                        qo[m] = a.q_old[idx];
+                        if (a.rhs) ro[m] = a.rhs[idx];
+                    }
+                }
+            }
+        }
         const int sl0 = (i - i0 + 1) % 3;  // slot of plane i (plane p sits in slot (p - i0 + 1) % 3)
         f32x4 acc[S][NB];
 #pragma unroll
@@ -185,7 +213,38 @@ __global__ __launch_bounds__(64 * TJ) void conv_kernel(ConvArgs a, int ch, int n
         lds_barrier();  // every wave is done with the slot of plane i-1
         if (more) store_plane((i + 3 - i0) % 3, st);  // plane i+2 -> slot (i+2 - i0 + 1) % 3
         // lane holds output channels nb*16 + g*4 + r (r = 0..3) of voxel (i, j, k0 + s*16 + col)
-        if (j < a.Y) {
+        if constexpr (EPI == EPI_APPLY) {
+            // The five output channels of the wave's 16S voxels go through its LDS area, so that
+            // lane l then serves voxel l mod 32 and components 0-2 (l < 32) or 3-4: every access
+            // covers 32 consecutive voxels of one component plane (the values are unchanged).
+            static_assert(S == 2, "the apply epilogue maps 32 voxels onto the two half-waves");
+            float* yw = ytr + w * (CH_Q * 16 * S);
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int chn = g * 4 + r;
+                    if (chn < CH_Q) yw[chn * 16 * S + s * 16 + col] = acc[s][0][r] + a.bias[chn];
+                }
+            if (alive) {
+                const long long v = bbase + (long long)i * YZ + (long long)j * a.Z + kx;
+                const bool face = i == 0 || j == 0 || kx == 0 || i == a.box.Nx - 1 || j == a.box.Ny - 1 ||
+                                  kx == a.box.Nz - 1;
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    const int chn = half ? 3 + m : m;
+                    if (chn < CH_Q) {
+                        const long long idx = (long long)chn * a.cs + a.box.off + v;
+                        // one gain per voxel (u (1 + y_0) in every component) or one per component
+                        const float y = yw[(a.scalar_gain ? 0 : chn) * 16 * S + vx];
+                        float d = y;
+                        if (a.rhs)  // increment input: the step's own increment, modulated
+                            d = face ? 0.0f : (ro[m] - qo[m]) * (1.0f + y);
+                        a.q_new[idx] = __builtin_fmaf(a.out_scale, d, qo[m]);
+                    }
+                }
+            }
+        } else if (j < a.Y) {
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 const int k = k0 + s * 16 + col;
@@ -214,37 +273,7 @@ __global__ __launch_bounds__(64 * TJ) void conv_kernel(ConvArgs a, int ch, int n
                             const f32x4 bz = *reinterpret_cast<const f32x4*>(a.bias + co);
                             *reinterpret_cast<f32x4*>(a.pred + v * 8 + co) = c + bz;
                         }
-                    } else if constexpr (EPI == EPI_APPLY) {
-                        if (a.scalar_gain) {  // one gain per voxel: u (1 + y_0) in every component
-                            if (co == 0 && changeable(a.box, a.rows, i, j, k)) {
-                                const bool face = i == 0 || j == 0 || k == 0 || i == a.box.Nx - 1 ||
-                                                  j == a.box.Ny - 1 || k == a.box.Nz - 1;
-                                const float g = 1.0f + (c[0] + a.bias[0]);
-#pragma unroll
-                                for (int chn = 0; chn < CH_Q; ++chn) {
-                                    const long long idx = (long long)chn * a.cs + a.box.off + v;
-                                    const float d = face ? 0.0f : (a.rhs[idx] - a.q_old[idx]) * g;
-                                    a.q_new[idx] = __builtin_fmaf(a.out_scale, d, a.q_old[idx]);
-                                }
-                            }
-                        } else if (co < 8 && changeable(a.box, a.rows, i, j, k)) {
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) {
-                                const int chn = co + r;
-                                if (chn < CH_Q) {
-                                    const long long idx = (long long)chn * a.cs + a.box.off + v;
-                                    const float y = c[r] + a.bias[chn];
-                                    float d = y;
-                                    if (a.rhs) {  // increment input: the step's own increment, modulated
-                                        const bool face = i == 0 || j == 0 || k == 0 || i == a.box.Nx - 1 ||
-                                                          j == a.box.Ny - 1 || k == a.box.Nz - 1;
-                                        d = face ? 0.0f : (a.rhs[idx] - a.q_old[idx]) * (1.0f + y);
-                                    }
-                                    a.q_new[idx] = __builtin_fmaf(a.out_scale, d, a.q_old[idx]);
-                                }
-                            }
-                        }
-                    } else {  // EPI_BWD
+                    } else if constexpr (EPI == EPI_BWD) {
                         const f32x4 z = *reinterpret_cast<const f32x4*>(a.z_in + v * 32 + co);
                         uint16_t h[4];
 #pragma unroll
@@ -752,7 +781,7 @@ unsigned reduce_blocks(long long n) { return std::max(1u, std::min(blocks_for(n)
 template <int CP, int NB, int EPI, int S, int TJ>
 hipError_t go_conv_s(hipStream_t st, const ConvArgs& a) {
     using T = ConvTile<CP, S, TJ>;
-    const size_t lds = (size_t(NB) * ksteps(CP) * 64 + 3 * size_t(T::SLOT)) * 16;
+    const size_t lds = size_t(conv_lds_units<CP, NB, EPI, S, TJ>()) * 16;
     const int ntj = (a.Y + T::TJ - 1) / T::TJ, ntk = (a.Z + T::TK - 1) / T::TK;
     // planes per block: aim at about four blocks per CU over the whole launch
     const long long tiles = (long long)a.n * ntj * ntk;
