@@ -223,8 +223,8 @@ def run_ladder(q, args):
             tune = {"variant": v, "planes_per_tile": ch, "fixed": True}
         else:
             tune = s.autotune(10)
-        if args.no_fusion:
-            s.set_step_fusion(False)
+        if args.no_fusion or args.pair_planes:
+            s.set_step_fusion(not args.no_fusion, args.pair_planes)
         log(f"tuning: {tune}")
         advance = (lambda n: s.step_graph(n, args.graph)) if args.graph else s.step
         advance(args.warmup)
@@ -612,6 +612,8 @@ def main():
                     help="surrogate_192: Adam iterations per group of 16 trajectories")
     ap.add_argument("--no-fusion", action="store_true",
                     help="run every step on its own (no two-step launches for L1-only contexts)")
+    ap.add_argument("--pair-planes", type=int, default=0,
+                    help="planes per tile of the two-step kernel (default: autotuned, or 16 with --tuning)")
     ap.add_argument("--workload", default=WORKLOADS[0], choices=WORKLOADS,
                     help="droplet_l1l2l3_256 is the headline line; the others are the periodic noisy quench, "
                          "the 64^3 L1 droplet, the 64-droplet quench ladder and the learned warm start")
@@ -650,8 +652,8 @@ def main():
             tune = {"variant": v, "planes_per_tile": ch, "fixed": True}
         else:
             tune = s.autotune(10)
-        if args.no_fusion:
-            s.set_step_fusion(False)
+        if args.no_fusion or args.pair_planes:
+            s.set_step_fusion(not args.no_fusion, args.pair_planes)
         log(f"tuning: {tune}")
         L_stab = p.L1 if p.L1 else p.kappa
 
