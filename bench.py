@@ -321,7 +321,7 @@ def ladder_relax(q, args, surr_traj=None, trained=True):
         if trained and surr_traj > 0:
             ws.zero_output_layer(net)
             net.set_input(q.SURR_INPUT_INCREMENT, 500.0)
-            tr = ws.train(net, ws.training_set(surr_traj), gap=500, n_snap=13, stride=250, group=16,
+            tr = ws.train(net, ws.training_set(surr_traj), gap=args.surr_gap, n_snap=13, stride=250, group=16,
                           iters_per_group=args.surr_iters, lr=5e-4, increment=True, first=args.surr_first, log=log)
             nets.append(("trained", net))
         zero.params = np.zeros(zero.nparams, dtype=np.float32)
@@ -524,7 +524,7 @@ def run_surrogate(q, args):
     with q.Surrogate(seed=3) as net, q.Surrogate(seed=None) as zero:
         ws.zero_output_layer(net)
         net.set_input(q.SURR_INPUT_INCREMENT, float(horizon))
-        tr = ws.train(net, trajs, gap=horizon, n_snap=13, stride=250, group=16, iters_per_group=args.surr_iters,
+        tr = ws.train(net, trajs, gap=args.surr_gap, n_snap=13, stride=250, group=16, iters_per_group=args.surr_iters,
                       lr=5e-4, increment=True, first=args.surr_first, log=log)
         zero.params = np.zeros(zero.nparams, dtype=np.float32)
         zero.set_input(q.SURR_INPUT_INCREMENT, float(horizon))
@@ -608,6 +608,9 @@ def main():
     ap.add_argument("--surr-first", type=int, default=500,
                     help="training snapshots start this many plain steps into each trajectory (from step 0, the "
                          "network's first jumps from radial starts reshape the droplet cores)")
+    ap.add_argument("--surr-gap", type=int, default=500,
+                    help="training target Q(k + gap) in plain steps (a multiple of 250); the proposal stays "
+                         "500 u (1 + y), so a longer gap teaches the network to jump further than 500 steps go")
     ap.add_argument("--surr-iters", type=int, default=1000,
                     help="surrogate_192: Adam iterations per group of 16 trajectories")
     ap.add_argument("--no-fusion", action="store_true",
