@@ -323,6 +323,8 @@ def ladder_relax(q, args, surr_traj=None, trained=True):
             net.set_input(q.SURR_INPUT_INCREMENT, 500.0)
             tr = ws.train(net, ws.training_set(surr_traj), gap=args.surr_gap, n_snap=13, stride=250, group=16,
                           iters_per_group=args.surr_iters, lr=5e-4, increment=True, first=args.surr_first, log=log)
+            if args.surr_shrink != 1.0:
+                ws.scale_output_layer(net, args.surr_shrink)
             nets.append(("trained", net))
         zero.params = np.zeros(zero.nparams, dtype=np.float32)
         zero.set_input(q.SURR_INPUT_INCREMENT, 500.0)
@@ -526,6 +528,8 @@ def run_surrogate(q, args):
         net.set_input(q.SURR_INPUT_INCREMENT, float(horizon))
         tr = ws.train(net, trajs, gap=args.surr_gap, n_snap=13, stride=250, group=16, iters_per_group=args.surr_iters,
                       lr=5e-4, increment=True, first=args.surr_first, log=log)
+        if args.surr_shrink != 1.0:
+            ws.scale_output_layer(net, args.surr_shrink)
         zero.params = np.zeros(zero.nparams, dtype=np.float32)
         zero.set_input(q.SURR_INPUT_INCREMENT, float(horizon))
         p = ws.droplet_params(88.0, 300.0, 1e-4, N=192)
@@ -611,6 +615,8 @@ def main():
     ap.add_argument("--surr-gap", type=int, default=500,
                     help="training target Q(k + gap) in plain steps (a multiple of 250); the proposal stays "
                          "500 u (1 + y), so a longer gap teaches the network to jump further than 500 steps go")
+    ap.add_argument("--surr-shrink", type=float, default=1.0,
+                    help="scale the trained network's output layer by this factor before use (0: the zero network)")
     ap.add_argument("--surr-iters", type=int, default=1000,
                     help="surrogate_192: Adam iterations per group of 16 trajectories")
     ap.add_argument("--no-fusion", action="store_true",

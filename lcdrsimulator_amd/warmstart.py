@@ -255,6 +255,14 @@ def zero_output_layer(net: q.Surrogate):
     net.params = p
 
 
+def scale_output_layer(net: q.Surrogate, alpha: float):
+    """Scale the last layer's weights and bias by alpha, so the modulation y becomes alpha * y:
+    alpha = 0 is the zero network (the plain step extrapolated), alpha = 1 the trained one."""
+    p = net.params
+    p[-(27 * 32 * 5 + 5):] *= float(alpha)
+    net.params = p
+
+
 def compare(p: q.Params, net: q.Surrogate, init_mode: int, seed: int = 12345, tol: float = 1e-6, every: int = 100,
             max_steps: int = 200000, scales=(1.0, 0.5, 0.25), tail_tol: float | None = None, hold: int = 0,
             log=None) -> dict:
