@@ -260,6 +260,7 @@ typedef struct qsr_surr_result {
     double ms_network;            /* device time of the network pass */
     double scale;                 /* the scale of the step kept (or of the last one tried) */
     int32_t tried;                /* proposals evaluated (0: nothing to propose, zero increment) */
+    int64_t net_voxels;           /* voxels the network pass computed (those of its tiles) */
 } qsr_surr_result;
 
 int qsr_surr_num_params(size_t* n);
@@ -295,6 +296,12 @@ int qsr_surr_dataset_add_increment(void* net, void* ctx, int box, int* snapshot)
 /* What the network reads and, for increment input, the horizon N its output is
  * scaled by (N > 0).  Training and application use the setting of the call. */
 int qsr_surr_set_input(void* net, int kind, double horizon);
+/* Which tiles of a box the application's network pass computes (a tile: 12 rows
+ * x 32 voxels of k x up to 16 planes): QSR_SURR_TILES_DROPLET (the default)
+ * those within the network's reach, 2 voxels, of a voxel a step can change;
+ * QSR_SURR_TILES_ALL every tile.  The proposals are the same either way. */
+enum { QSR_SURR_TILES_DROPLET = 0, QSR_SURR_TILES_ALL = 1 };
+int qsr_surr_set_tiles(void* net, int mode);
 int qsr_surr_dataset_size(void* net, int* n);
 int qsr_surr_dataset_clear(void* net);
 /* iters Adam steps on batches of `batch` random crop^3 windows (crop % 8 == 0)
@@ -318,10 +325,11 @@ int qsr_surr_apply(void* net, void* ctx, int mode, qsr_surr_result* out);
  * later proposals rescale the first), FORCE and PROPOSE use scales[0]. */
 int qsr_surr_apply_ladder(void* net, void* ctx, int mode, const double* scales, int nscales, qsr_surr_result* out);
 /* Batched contexts: the same for each listed box (uploaded, not held, listed
- * once), one network pass per box; each box is kept or restored on its own by
+ * once), one network pass over all listed boxes (each layer one launch over
+ * their tiles); each box is kept or restored on its own by
  * its own energy and residual, and unlisted boxes are untouched.  out[i] is
  * the result of boxes[i]; its ms_network is the device time of all listed
- * boxes' network passes. */
+ * boxes' network passes and its net_voxels their voxel count. */
 int qsr_surr_apply_boxes(void* net, void* ctx, int mode, const double* scales, int nscales, const int* boxes,
                          int nboxes, qsr_surr_result* out);
 

@@ -1340,6 +1340,7 @@ int field_view(void* ctx, int box, FieldView* out) {
     v.nvox = c->boxes[box].nvox;
     v.box = c->h_boxes[box];
     v.rows = c->d_step_rows;
+    v.host_rows = c->boxes[box].masks.step.empty() ? nullptr : c->boxes[box].masks.step.data();
     v.nboxes = int(c->boxes.size());
     v.uploaded = c->boxes[box].uploaded;
     v.active = c->boxes[box].active;

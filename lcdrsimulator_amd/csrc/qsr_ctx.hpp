@@ -20,6 +20,9 @@ struct FieldView {
     long long cs = 0, nvox = 0;
     qsrk::BoxDev box{};
     const qsrk::RowMask* rows = nullptr;  // step row masks (packed over all boxes)
+    // the box's step row masks on the host, 8 int16 per row (i, j) at (i * Ny + j) * 8: the
+    // interval [e0, e1] of k that evolves first (empty when e0 > e1)
+    const int16_t* host_rows = nullptr;
     int nboxes = 0;
     bool uploaded = false;
     bool active = true;  // held boxes are not stepped (their step buffer is not written)

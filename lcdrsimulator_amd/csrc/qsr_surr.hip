@@ -114,18 +114,39 @@ __global__ __launch_bounds__(64 * TJ) void conv_kernel(ConvArgs a, int ch, int n
     const u32x4* wf = reinterpret_cast<const u32x4*>(a.frag);
     for (int f = threadIdx.x; f < NB * KS * 64; f += NT) wl[f] = wf[f];
 
-    int bid = blockIdx.x;
-    const int kt = bid % ntk;
-    bid /= ntk;
-    const int jt = bid % ntj;
-    bid /= ntj;
-    const int it = bid % nti;
-    const int b = bid / nti;
-    const int k0 = kt * T::TK, j0 = jt * T::TJ, i0 = it * ch, i1 = min(i0 + ch, a.X);
+    // the block's tile: an entry of the tile list, or decoded from the block index over the
+    // uniform batch; bbase is the first voxel of its box (batch element) in the activations
+    int X = a.X, Y = a.Y, Z = a.Z, i0, j0, k0;
+    long long bbase;
+    BoxDev B = a.box;
+    if (a.tiles) {
+        const int4 t4 = a.tiles[blockIdx.x];
+        const ConvBox* cb = a.boxes + t4.x;
+        X = cb->X;
+        Y = cb->Y;
+        Z = cb->Z;
+        bbase = cb->act_off;
+        if constexpr (EPI == EPI_APPLY) B = cb->box;
+        i0 = t4.y;
+        j0 = t4.z;
+        k0 = t4.w;
+    } else {
+        int bid = blockIdx.x;
+        const int kt = bid % ntk;
+        bid /= ntk;
+        const int jt = bid % ntj;
+        bid /= ntj;
+        const int it = bid % nti;
+        const int b = bid / nti;
+        k0 = kt * T::TK;
+        j0 = jt * T::TJ;
+        i0 = it * ch;
+        bbase = (long long)b * X * Y * Z;
+    }
+    const int i1 = min(i0 + ch, X);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, col = lane & 15;
     const int j = j0 + w;
-    const long long YZ = (long long)a.Y * a.Z;
-    const long long bbase = (long long)b * a.X * YZ;  // first voxel of batch element b
+    const long long YZ = (long long)Y * Z;
 
     // staging: unit u = (r * RW + kk) * G + gg, consecutive threads read consecutive 16 B
     auto load_plane = [&](int ip, u32x4 (&st)[UPT]) {
@@ -136,9 +157,9 @@ __global__ __launch_bounds__(64 * TJ) void conv_kernel(ConvArgs a, int ch, int n
             if (u < SLOT) {
                 const int gg = u % G, kk = (u / G) % RW, r = u / (G * RW);
                 const int jj = j0 - 1 + r, kq = k0 - 1 + kk;
-                if (unsigned(ip) < unsigned(a.X) && unsigned(jj) < unsigned(a.Y) && unsigned(kq) < unsigned(a.Z))
+                if (unsigned(ip) < unsigned(X) && unsigned(jj) < unsigned(Y) && unsigned(kq) < unsigned(Z))
                     st[s] = *reinterpret_cast<const u32x4*>(
-                        a.in + (bbase + (long long)ip * YZ + (long long)jj * a.Z + kq) * CP + gg * 8);
+                        a.in + (bbase + (long long)ip * YZ + (long long)jj * Z + kq) * CP + gg * 8);
             }
         }
     };
@@ -170,14 +191,15 @@ __global__ __launch_bounds__(64 * TJ) void conv_kernel(ConvArgs a, int ch, int n
         bool alive = false;
         float qo[3] = {}, ro[3] = {};
         if constexpr (EPI == EPI_APPLY) {
-            alive = j < a.Y && kx < a.Z && changeable(a.box, a.rows, i, j, kx);
+            alive = j < Y && kx < Z && changeable(B, a.rows, i, j, kx);
             if (alive) {
-                const long long v = bbase + (long long)i * YZ + (long long)j * a.Z + kx;
+                // the field is indexed by the box-local voxel (the activations by bbase + it)
+                const long long vl = (long long)i * YZ + (long long)j * Z + kx;
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {
                     const int chn = half ? 3 + m : m;
                     if (chn < CH_Q) {
-                        const long long idx = (long long)chn * a.cs + a.box.off + v;
+                        const long long idx = (long long)chn * a.cs + B.off + vl;
                         qo[m] = a.q_old[idx];
                         if (a.rhs) ro[m] = a.rhs[idx];
                     }
@@ -227,14 +249,13 @@ __global__ __launch_bounds__(64 * TJ) void conv_kernel(ConvArgs a, int ch, int n
                     if (chn < CH_Q) yw[chn * 16 * S + s * 16 + col] = acc[s][0][r] + a.bias[chn];
                 }
             if (alive) {
-                const long long v = bbase + (long long)i * YZ + (long long)j * a.Z + kx;
-                const bool face = i == 0 || j == 0 || kx == 0 || i == a.box.Nx - 1 || j == a.box.Ny - 1 ||
-                                  kx == a.box.Nz - 1;
+                const long long vl = (long long)i * YZ + (long long)j * Z + kx;
+                const bool face = i == 0 || j == 0 || kx == 0 || i == B.Nx - 1 || j == B.Ny - 1 || kx == B.Nz - 1;
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {
                     const int chn = half ? 3 + m : m;
                     if (chn < CH_Q) {
-                        const long long idx = (long long)chn * a.cs + a.box.off + v;
+                        const long long idx = (long long)chn * a.cs + B.off + vl;
                         // one gain per voxel (u (1 + y_0) in every component) or one per component
                         const float y = yw[(a.scalar_gain ? 0 : chn) * 16 * S + vx];
                         float d = y;
@@ -244,12 +265,12 @@ __global__ __launch_bounds__(64 * TJ) void conv_kernel(ConvArgs a, int ch, int n
                     }
                 }
             }
-        } else if (j < a.Y) {
+        } else if (j < Y) {
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 const int k = k0 + s * 16 + col;
-                if (k >= a.Z) continue;
-                const long long v = bbase + (long long)i * YZ + (long long)j * a.Z + k;
+                if (k >= Z) continue;
+                const long long v = bbase + (long long)i * YZ + (long long)j * Z + k;
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb) {
                     const int co = nb * 16 + g * 4;
@@ -782,13 +803,24 @@ template <int CP, int NB, int EPI, int S, int TJ>
 hipError_t go_conv_s(hipStream_t st, const ConvArgs& a) {
     using T = ConvTile<CP, S, TJ>;
     const size_t lds = size_t(conv_lds_units<CP, NB, EPI, S, TJ>()) * 16;
-    const int ntj = (a.Y + T::TJ - 1) / T::TJ, ntk = (a.Z + T::TK - 1) / T::TK;
-    // planes per block: aim at about four blocks per CU over the whole launch
-    const long long tiles = (long long)a.n * ntj * ntk;
-    const long long want_nti = std::max(1LL, (1024 + tiles - 1) / tiles);
-    const int ch = int(std::max(2LL, (a.X + want_nti - 1) / want_nti));
-    const int nti = (a.X + ch - 1) / ch;
-    const long long grid = tiles * nti;
+    int ntj = 0, ntk = 0, ch = 0, nti = 0;
+    long long grid = 0;
+    if (a.tiles) {
+        // a tile list: one block per listed tile
+        if (T::TJ != TILE_TJ || T::TK != TILE_TK || a.tile_ch < 1) return hipErrorInvalidValue;
+        ch = a.tile_ch;
+        grid = a.ntiles;
+    } else {
+        ntj = (a.Y + T::TJ - 1) / T::TJ;
+        ntk = (a.Z + T::TK - 1) / T::TK;
+        // planes per block: aim at about four blocks per CU over the whole launch
+        const long long tiles = (long long)a.n * ntj * ntk;
+        const long long want_nti = std::max(1LL, (1024 + tiles - 1) / tiles);
+        ch = int(std::max(2LL, (a.X + want_nti - 1) / want_nti));
+        nti = (a.X + ch - 1) / ch;
+        grid = tiles * nti;
+    }
+    if (grid <= 0) return hipSuccess;
     if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
     // allow the kernel more than the default dynamic LDS; a device that cannot
     // give it makes the launch below fail with an error
@@ -803,7 +835,8 @@ template <int CP, int NB, int EPI>
 hipError_t go_conv(hipStream_t st, const ConvArgs& a) {
     // 32 voxels of k per block and one wave per row: 12 rows (768 threads, three waves per SIMD,
     // 146 KB of LDS for 32 -> 32) on large boxes, 8 rows on short ones (training crops)
-    return a.Y > 32 ? go_conv_s<CP, NB, EPI, 2, 12>(st, a) : go_conv_s<CP, NB, EPI, 2, 8>(st, a);
+    // (tile lists use the 12-row shape)
+    return a.tiles || a.Y > 32 ? go_conv_s<CP, NB, EPI, 2, TILE_TJ>(st, a) : go_conv_s<CP, NB, EPI, 2, 8>(st, a);
 }
 
 }  // namespace

@@ -71,6 +71,22 @@ enum : int {
     EPI_BWD = 3,     // dz = bf16(acc * gelu'(z_in)) (NDHWC and cm)
 };
 
+// One box of a tile-listed convolution launch: its dimensions, the index of its
+// first voxel in the launch's packed activation tensors, and (apply epilogue)
+// its description in the context.
+struct ConvBox {
+    int X, Y, Z, pad;
+    long long act_off;
+    qsrk::BoxDev box;
+};
+
+// Tiles of a tile-listed launch: TILE_TJ rows (one wave each) x TILE_TK voxels of
+// k x the launch's planes per tile.  A box's list holds every tile with a voxel
+// within TILE_REACH of one a step can change: the three layers' outputs there
+// only read inputs the list computes, so the network's output on the voxels a
+// step can change is the same as over the whole box.
+constexpr int TILE_TJ = 12, TILE_TK = 32, TILE_REACH = 2;
+
 struct ConvArgs {
     const uint16_t* in;     // NDHWC BF16, CP channels
     const uint16_t* frag;   // weight fragments
@@ -91,6 +107,13 @@ struct ConvArgs {
     long long cs;
     qsrk::BoxDev box;
     const qsrk::RowMask* rows;
+    // tile-listed launch (null for a uniform batch): block b computes tiles[b] = (index
+    // into boxes, i0, j0, k0) over tile_ch planes; in and the activation outputs are
+    // packed box after box (ConvBox::act_off), and the apply epilogue takes each tile's
+    // box from boxes instead of `box`
+    const int4* tiles;
+    const ConvBox* boxes;
+    int ntiles, tile_ch;
 };
 
 // One dataset snapshot: five FP32 planes of nvox voxels plus a byte per voxel

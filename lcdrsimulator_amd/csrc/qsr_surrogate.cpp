@@ -78,9 +78,16 @@ struct Net {
     long long stage_cap = 0;
     double* d_out = nullptr;  // 2 doubles per training iteration (loss partial sums)
     long long out_cap = 0;
-    // application workspace
+    // application workspace: the listed boxes' activations packed box after box, and the tile
+    // table of their network pass (device, and pinned host memory it is staged in)
     uint16_t *ax = nullptr, *aa1 = nullptr, *aa2 = nullptr;
     long long a_cap = 0;
+    int4 *d_tiles = nullptr, *h_tiles = nullptr;
+    ConvBox *d_cboxes = nullptr, *h_cboxes = nullptr;
+    long long tiles_cap = 0, cboxes_cap = 0;
+    hipEvent_t ev_stage = nullptr;  // the last staging copies; waited for before the staging is reused
+    bool stage_pending = false;
+    bool tiles_all = false;  // list every tile of a box (QSR_SURR_TILES_ALL)
     // dataset
     std::vector<SnapH> snaps;
     Snap* d_snaps = nullptr;  // device copy of the snapshot table
@@ -92,10 +99,14 @@ struct Net {
     ~Net() {
         for (SnapH& s : snaps) free_snap(s);
         for (void* p : {(void*)P, (void*)G, (void*)M, (void*)Vm, frag_mem, w_mem, (void*)stage, (void*)d_out,
-                        (void*)ax, (void*)aa1, (void*)aa2, (void*)d_snaps, (void*)d_jobs})
+                        (void*)ax, (void*)aa1, (void*)aa2, (void*)d_snaps, (void*)d_jobs, (void*)d_tiles,
+                        (void*)d_cboxes})
             discard(hipFree(p));
+        for (void* p : {(void*)h_tiles, (void*)h_cboxes})
+            if (p) discard(hipHostFree(p));
         for (hipEvent_t e : ev)
             if (e) discard(hipEventDestroy(e));
+        if (ev_stage) discard(hipEventDestroy(ev_stage));
         if (st) discard(hipStreamDestroy(st));
     }
 };
@@ -254,13 +265,71 @@ int check_net(Net* n) {
     return QSR_OK;
 }
 
-// Network pass over one box of a context, on the context's stream: the proposal
-// q_old + out_scale * (y, or u (1 + y) under increment and gain input) into
-// v.other on the voxels a step can change, from v.cur (and v.rhs = E(v.cur)).
-// With ms_out the pass is timed (and waited for).
-int propose_box(Net& n, const qsrc::FieldView& v, double s_inc, float out_scale, double* ms_out) {
+// One box of a network pass: its view of the context and, for increment and
+// step-size input, the RMS s of its increment.
+struct Prop {
+    qsrc::FieldView v;
+    double s_inc;
+};
+
+// The tiles of one box for a tile-listed network pass, appended to out as
+// (lbox, i0, j0, k0) with i0 a multiple of ch, j0 of TILE_TJ and k0 of TILE_TK:
+// every tile holding a voxel within TILE_REACH (in each direction) of a voxel
+// inside its row's evolve interval, which holds every voxel a step can change.
+// The interval hull of the rows within reach of a tile's rows bounds the tile's
+// k range.  Periodic boxes, boxes without host row masks and `all` list every tile.
+void box_tiles(const qsrc::FieldView& v, int ch, bool all, int lbox, std::vector<int4>& out) {
+    const int X = v.box.Nx, Y = v.box.Ny, Z = v.box.Nz, R = TILE_REACH;
+    const bool every = all || (v.box.flags & qsrk::F_PERIODIC) || !v.host_rows;
+    const int ntk = (Z + TILE_TK - 1) / TILE_TK;
+    for (int i0 = 0; i0 < X; i0 += ch)
+        for (int j0 = 0; j0 < Y; j0 += TILE_TJ) {
+            int lo = 0, hi = Z - 1;
+            if (!every) {
+                lo = Z;
+                hi = -1;
+                const int ia = std::max(0, i0 - R), ib = std::min(X - 1, i0 + ch - 1 + R);
+                const int ja = std::max(0, j0 - R), jb = std::min(Y - 1, j0 + TILE_TJ - 1 + R);
+                for (int i = ia; i <= ib; ++i)
+                    for (int j = ja; j <= jb; ++j) {
+                        const int16_t* m = v.host_rows + (size_t(i) * Y + j) * 8;
+                        if (m[0] <= m[1]) {
+                            lo = std::min(lo, int(m[0]));
+                            hi = std::max(hi, int(m[1]));
+                        }
+                    }
+                if (lo > hi) continue;
+                lo -= R;
+                hi += R;
+            }
+            for (int kt = 0; kt < ntk; ++kt) {
+                const int k0 = kt * TILE_TK;
+                if (k0 + TILE_TK - 1 >= lo && k0 <= hi) out.push_back(make_int4(lbox, i0, j0, k0));
+            }
+        }
+}
+
+// Network pass over the listed boxes of one context, on the context's stream: for
+// each box the proposal q_old + out_scale * (y, or u (1 + y) under increment and
+// gain input) into v.other on the voxels a step can change, from v.cur (and
+// v.rhs = E(v.cur)).  The boxes' inputs are packed box after box and each layer is
+// one launch over the tiles of every listed box.  With ms_out the pass is timed
+// (and waited for).
+int propose_list(Net& n, const std::vector<Prop>& props, float out_scale, double* ms_out,
+                 long long* voxels_out) {
+    if (ms_out) *ms_out = 0.0;
+    if (voxels_out) *voxels_out = 0;
+    if (props.empty()) return QSR_OK;
     const bool incr = n.input_kind != QSR_SURR_INPUT_FIELD;
-    const long long V = v.nvox;
+    const qsrc::FieldView& v0 = props[0].v;
+    const hipStream_t st = v0.st;
+    const int nb = int(props.size());
+    std::vector<long long> off(static_cast<size_t>(nb));
+    long long V = 0;
+    for (int b = 0; b < nb; ++b) {
+        off[b] = V;
+        V += props[b].v.nvox;
+    }
     if (V > n.a_cap) {
         for (uint16_t** p : {&n.ax, &n.aa1, &n.aa2}) {
             discard(hipFree(*p));
@@ -272,13 +341,71 @@ int propose_box(Net& n, const qsrc::FieldView& v, double s_inc, float out_scale,
         HIP_TRY(hipMalloc(&n.aa2, sizeof(uint16_t) * 32 * V));
         n.a_cap = V;
     }
-    const hipStream_t st = v.st;
-    ConvArgs a = conv_dims(1, v.box.Nx, v.box.Ny, v.box.Nz);
+    // planes per tile: the largest of 16, 8 and 4 that lists at least 1024 tiles
+    std::vector<int4> tiles;
+    int ch = 16;
+    for (;;) {
+        tiles.clear();
+        for (int b = 0; b < nb; ++b) box_tiles(props[b].v, ch, n.tiles_all, b, tiles);
+        if (tiles.size() >= 1024 || ch <= 4) break;
+        ch /= 2;
+    }
+    if (tiles.empty()) return QSR_OK;  // no voxel of any listed box can change
+    if (voxels_out)
+        for (const int4& t : tiles) {
+            const qsrk::BoxDev& bx = props[t.x].v.box;
+            *voxels_out += (long long)std::min(ch, bx.Nx - t.y) * std::min(TILE_TJ, bx.Ny - t.z) *
+                           std::min(TILE_TK, bx.Nz - t.w);
+        }
+    // stage the tile table and the boxes in pinned memory (once the previous copies are done)
+    if (n.stage_pending) {
+        HIP_TRY(hipEventSynchronize(n.ev_stage));
+        n.stage_pending = false;
+    }
+    const long long nt = (long long)tiles.size();
+    if (nt > n.tiles_cap) {
+        discard(hipFree(n.d_tiles));
+        if (n.h_tiles) discard(hipHostFree(n.h_tiles));
+        n.d_tiles = n.h_tiles = nullptr;
+        n.tiles_cap = 0;
+        HIP_TRY(hipMalloc(&n.d_tiles, sizeof(int4) * nt));
+        HIP_TRY(hipHostMalloc(&n.h_tiles, sizeof(int4) * nt, hipHostMallocDefault));
+        n.tiles_cap = nt;
+    }
+    if (nb > n.cboxes_cap) {
+        discard(hipFree(n.d_cboxes));
+        if (n.h_cboxes) discard(hipHostFree(n.h_cboxes));
+        n.d_cboxes = n.h_cboxes = nullptr;
+        n.cboxes_cap = 0;
+        HIP_TRY(hipMalloc(&n.d_cboxes, sizeof(ConvBox) * nb));
+        HIP_TRY(hipHostMalloc(&n.h_cboxes, sizeof(ConvBox) * nb, hipHostMallocDefault));
+        n.cboxes_cap = nb;
+    }
+    std::copy(tiles.begin(), tiles.end(), n.h_tiles);
+    for (int b = 0; b < nb; ++b) {
+        const qsrk::BoxDev& bx = props[b].v.box;
+        n.h_cboxes[b] = ConvBox{bx.Nx, bx.Ny, bx.Nz, 0, off[b], bx};
+    }
+    HIP_TRY(hipMemcpyAsync(n.d_tiles, n.h_tiles, sizeof(int4) * nt, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(n.d_cboxes, n.h_cboxes, sizeof(ConvBox) * nb, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(n.ev_stage, st));
+    n.stage_pending = true;
+
     if (ms_out) HIP_TRY(hipEventRecord(n.ev[0], st));
-    if (n.input_kind == QSR_SURR_INPUT_INCREMENT || n.input_kind == QSR_SURR_INPUT_STEP)
-        HIP_TRY(launch_pack_increment(st, v.cur, v.rhs, v.cs, v.box, v.rows, float(1.0 / s_inc), n.ax));
-    else
-        HIP_TRY(launch_pack_field(st, v.cur, v.cs, v.box.off, V, n.ax));
+    for (int b = 0; b < nb; ++b) {
+        const qsrc::FieldView& v = props[b].v;
+        uint16_t* x = n.ax + off[b] * 8;
+        if (n.input_kind == QSR_SURR_INPUT_INCREMENT || n.input_kind == QSR_SURR_INPUT_STEP)
+            HIP_TRY(launch_pack_increment(st, v.cur, v.rhs, v.cs, v.box, v.rows, float(1.0 / props[b].s_inc), x));
+        else
+            HIP_TRY(launch_pack_field(st, v.cur, v.cs, v.box.off, v.nvox, x));
+    }
+    ConvArgs a{};
+    a.n = 1;
+    a.tiles = n.d_tiles;
+    a.boxes = n.d_cboxes;
+    a.ntiles = int(nt);
+    a.tile_ch = ch;
     a.in = n.ax;
     a.frag = n.fr.f1;
     a.bias = n.fr.b1;
@@ -293,14 +420,13 @@ int propose_box(Net& n, const qsrc::FieldView& v, double s_inc, float out_scale,
     a.frag = n.fr.f3;
     a.bias = n.fr.b3;
     a.a_out = nullptr;
-    a.q_old = v.cur;
-    a.q_new = v.other;
+    a.q_old = v0.cur;
+    a.q_new = v0.other;
     a.out_scale = out_scale;
-    a.rhs = incr ? v.rhs : nullptr;
+    a.rhs = incr ? v0.rhs : nullptr;
     a.scalar_gain = n.input_kind == QSR_SURR_INPUT_STEP ? 1 : 0;
-    a.cs = v.cs;
-    a.box = v.box;
-    a.rows = v.rows;
+    a.cs = v0.cs;
+    a.rows = v0.rows;
     HIP_TRY(launch_conv(32, 1, EPI_APPLY, st, a));
     if (ms_out) {
         HIP_TRY(hipEventRecord(n.ev[1], st));
@@ -344,6 +470,7 @@ int qsr_surr_create(int device, void** out) {
     n->device = device;
     HIP_TRY(hipStreamCreateWithFlags(&n->st, hipStreamNonBlocking));
     for (hipEvent_t& e : n->ev) HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipEventCreateWithFlags(&n->ev_stage, hipEventDisableTiming));
     for (float** p : {&n->P, &n->G, &n->M, &n->Vm}) HIP_TRY(hipMalloc(p, sizeof(float) * NPARAM));
     HIP_TRY(hipMemsetAsync(n->G, 0, sizeof(float) * NPARAM, n->st));
     const long long fe[5] = {frag_elems(8, 2), frag_elems(32, 2), frag_elems(32, 1), frag_elems(8, 2),
@@ -591,6 +718,14 @@ int qsr_surr_set_input(void* net, int kind, double horizon) {
     return QSR_OK;
 }
 
+int qsr_surr_set_tiles(void* net, int mode) {
+    Net* n = as_net(net);
+    if (!n) return fail(QSR_E_ARG, "null network");
+    if (mode != QSR_SURR_TILES_DROPLET && mode != QSR_SURR_TILES_ALL) return fail(QSR_E_ARG, "bad tile mode");
+    n->tiles_all = mode == QSR_SURR_TILES_ALL;
+    return QSR_OK;
+}
+
 int qsr_surr_dataset_size(void* net, int* count) {
     Net* n = as_net(net);
     if (!n || !count) return fail(QSR_E_ARG, "null argument");
@@ -747,7 +882,11 @@ int qsr_surr_apply_ladder(void* net, void* ctx, int mode, const double* scales, 
         return QSR_OK;
     }
 
-    if (int rc = propose_box(*n, v, s_inc, float(scales[0] * (incr ? n->horizon : 1.0)), &r.ms_network)) return rc;
+    long long nv = 0;
+    if (int rc = propose_list(*n, {Prop{v, s_inc}}, float(scales[0] * (incr ? n->horizon : 1.0)), &r.ms_network,
+                              &nv))
+        return rc;
+    r.net_voxels = nv;
     const hipStream_t st = v.st;
 
     // proposal i lies in the other buffer; the previous field stays in the current one
@@ -823,18 +962,20 @@ int qsr_surr_apply_boxes(void* net, void* ctx, int mode, const double* scales, i
     // proposals into the other buffer, which holds a copy of the current field elsewhere
     if (int rc = qsrc::copy_current_to_other(ctx)) return rc;
     std::vector<int> pending;
-    HIP_TRY(hipEventRecord(n->ev[0], v.st));
+    std::vector<Prop> props;
     for (int i = 0; i < nsel; ++i) {
         if (incr && !(s_inc[i] > 0.0 && std::isfinite(s_inc[i]))) continue;  // nothing to propose
         if (int rc = qsrc::field_view(ctx, boxes[i], &v)) return rc;
-        if (int rc = propose_box(*n, v, s_inc[i], float(scales[0] * (incr ? n->horizon : 1.0)), nullptr)) return rc;
+        props.push_back(Prop{v, s_inc[i]});
         pending.push_back(i);
     }
-    HIP_TRY(hipEventRecord(n->ev[1], v.st));
-    HIP_TRY(hipEventSynchronize(n->ev[1]));
-    float phase_ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&phase_ms, n->ev[0], n->ev[1]));
-    for (int i : pending) R[i].ms_network = phase_ms;  // the device time of all listed boxes' passes
+    double phase_ms = 0.0;  // the device time of all listed boxes' passes
+    long long nv = 0;
+    if (int rc = propose_list(*n, props, float(scales[0] * (incr ? n->horizon : 1.0)), &phase_ms, &nv)) return rc;
+    for (int i : pending) {
+        R[i].ms_network = phase_ms;
+        R[i].net_voxels = nv;
+    }
     if (!pending.empty()) {
         // current: the proposals (every other box unchanged); other: the previous field
         if (int rc = qsrc::swap_buffers(ctx)) return rc;

@@ -79,17 +79,18 @@ EXPORTS = (
     "qsr_surr_get_params", "qsr_surr_save", "qsr_surr_load", "qsr_surr_eval", "qsr_surr_adam_step",
     "qsr_surr_dataset_add", "qsr_surr_dataset_size", "qsr_surr_dataset_clear", "qsr_surr_train", "qsr_surr_apply",
     "qsr_surr_dataset_add_increment", "qsr_surr_set_input", "qsr_surr_apply_ladder", "qsr_surr_eval_gain",
-    "qsr_surr_apply_boxes", "qsr_energies", "qsr_energies_noalloc",
+    "qsr_surr_apply_boxes", "qsr_energies", "qsr_energies_noalloc", "qsr_surr_set_tiles",
 )
 
 SURR_CHECKED, SURR_FORCE, SURR_PROPOSE = 0, 1, 2
 SURR_INPUT_FIELD, SURR_INPUT_INCREMENT, SURR_INPUT_GAIN, SURR_INPUT_STEP = 0, 1, 2, 3
+SURR_TILES_DROPLET, SURR_TILES_ALL = 0, 1
 
 
 class _SurrResult(C.Structure):
     _fields_ = [("F_before", C.c_double), ("F_after", C.c_double), ("res_before", C.c_double),
                 ("res_after", C.c_double), ("accepted", C.c_int32), ("ms_network", C.c_double),
-                ("scale", C.c_double), ("tried", C.c_int32)]
+                ("scale", C.c_double), ("tried", C.c_int32), ("net_voxels", C.c_int64)]
 
 
 def load():
@@ -149,6 +150,7 @@ def load():
     L.qsr_surr_apply.argtypes = [vp, vp, C.c_int, C.POINTER(_SurrResult)]
     L.qsr_surr_dataset_add_increment.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
     L.qsr_surr_set_input.argtypes = [vp, C.c_int, C.c_double]
+    L.qsr_surr_set_tiles.argtypes = [vp, C.c_int]
     L.qsr_surr_apply_ladder.argtypes = [vp, vp, C.c_int, dp, C.c_int, C.POINTER(_SurrResult)]
     L.qsr_surr_eval_gain.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, fp, fp, fp, fp, dp, fp, fp]
     L.qsr_energies.argtypes = [vp, C.POINTER(C.c_int), C.c_int, dp]
@@ -502,6 +504,12 @@ class Surrogate:
         and SURR_INPUT_GAIN: it modulates u, one explicit step's increment of Q:
         dQ = horizon * u * (1 + y), reading u / s (s the RMS of u per component) or Q."""
         _check(load().qsr_surr_set_input(self._h, int(kind), float(horizon)))
+
+    def set_tiles(self, mode: int):
+        """SURR_TILES_DROPLET (the default): the application's network pass computes the tiles
+        within its reach of a voxel a step can change; SURR_TILES_ALL: every tile of the box.
+        The proposals are the same."""
+        _check(load().qsr_surr_set_tiles(self._h, int(mode)))
 
     def dataset_add(self, solver: "Solver", box: int = 0, increment: bool = False) -> int:
         """Snapshot a box's field (and, with ``increment``, one explicit step's increment
