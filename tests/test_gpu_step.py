@@ -239,7 +239,7 @@ def test_tile_shapes_agree(qsr, orc, name):
     p, od, Q0, S_shell = _case(qsr, orc, **CASES[name])
     Qo = od.step(Q0, S_shell, p.dt, steps=3)
     got = {}
-    for variant, ch in ((0, 8), (0, 32), (1, 8), (1, 16)):
+    for variant, ch in ((0, 8), (0, 32), (1, 8), (1, 16), (1, 18), (0, 14)):
         with qsr.Solver(p) as s:
             s.set_tuning(variant, ch)
             s.upload(Q0)
