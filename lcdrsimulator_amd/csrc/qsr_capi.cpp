@@ -151,6 +151,11 @@ struct Ctx {
     long long mpart_cap = 0;
     double* d_mout = nullptr;
     long long mout_cap = 0;
+    // their per-call tables: the boxes' observable parameters, block prefix and box list
+    qsro::ObsParams* d_mparams = nullptr;
+    long long* d_mprefix = nullptr;
+    int* d_mlist = nullptr;
+    int mtab_cap = 0;
 
     ~Ctx() {
         if (gexec) discard(hipGraphExecDestroy(gexec));
@@ -159,7 +164,8 @@ struct Ctx {
         for (void* p : {(void*)d_boxes, (void*)d_tiles, (void*)d_tiles2, (void*)d_step_rows, (void*)d_obs_rows, (void*)d_face_list,
                         (void*)d_dyn, (void*)d_part, (void*)d_out, (void*)d_nf_slice[0], (void*)d_nf_slice[1],
                         (void*)d_nf3, (void*)q_rhs, (void*)d_imex, (void*)d_jcoef, (void*)d_vox_prefix,
-                        (void*)d_jflags, (void*)d_rhs0, (void*)d_mpart, (void*)d_mout})
+                        (void*)d_jflags, (void*)d_rhs0, (void*)d_mpart, (void*)d_mout, (void*)d_mparams,
+                        (void*)d_mprefix, (void*)d_mlist})
             discard(hipFree(p));
         if (h_dyn) discard(hipHostFree(h_dyn));
         for (hipEvent_t e : ev)
@@ -1061,6 +1067,21 @@ int ensure_many(Ctx& c, long long part_doubles, long long out_doubles) {
     return QSR_OK;
 }
 
+// Room for the tables of a many-box reduction over n boxes.
+int ensure_mtab(Ctx& c, int n) {
+    if (n <= c.mtab_cap) return QSR_OK;
+    for (void* p : {(void*)c.d_mparams, (void*)c.d_mprefix, (void*)c.d_mlist}) discard(hipFree(p));
+    c.d_mparams = nullptr;
+    c.d_mprefix = nullptr;
+    c.d_mlist = nullptr;
+    c.mtab_cap = 0;
+    HIP_TRY(hipMalloc(&c.d_mparams, sizeof(qsro::ObsParams) * n));
+    HIP_TRY(hipMalloc(&c.d_mprefix, sizeof(long long) * (n + 1)));
+    HIP_TRY(hipMalloc(&c.d_mlist, sizeof(int) * n));
+    c.mtab_cap = n;
+    return QSR_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1073,31 +1094,37 @@ int qsr_energies(void* ctx, const int* boxes, int n, double* totals) {
     if (n == 0) return QSR_OK;
     if (int rc = check_uploaded(*c)) return rc;
     HIP_TRY(hipSetDevice(c->device));
-    long long part = 0;
-    for (int i = 0; i < n; ++i) part += (c->boxes[boxes[i]].nvox + 255) / 256 * qsro::NPART;
-    if (int rc = ensure_many(*c, part, (long long)n * qsro::NPART)) return rc;
-    const float* q = c->q[c->cur];
-    long long off = 0;
-    std::vector<int> periodic;
+    // the droplet boxes in one volume launch and one finalize launch (box m owns the same blocks and
+    // partial order as a single-box pass, so the sums are the same bit for bit)
+    std::vector<int> periodic, drop;
+    std::vector<qsro::ObsParams> P;
+    std::vector<long long> prefix{0};
     for (int i = 0; i < n; ++i) {
         const Box& b = c->boxes[boxes[i]];
         if (b.p.periodic) {  // the periodic pass also fills a nematic field: one box at a time
             periodic.push_back(i);
             continue;
         }
-        const qsro::ObsParams P = obs_params(*c, boxes[i], true);
-        long long nblk = 0;
-        HIP_TRY(qsro::launch_obs_droplet(c->st, q, c->cs, P, c->d_step_rows, c->d_obs_rows, c->d_mpart + off, &nblk));
-        HIP_TRY(qsro::launch_obs_finalize(c->st, c->d_mpart + off, nblk, qsro::NPART, 1u << 7,
-                                          c->d_mout + (long long)i * qsro::NPART));
-        off += (b.nvox + 255) / 256 * qsro::NPART;
+        drop.push_back(i);
+        P.push_back(obs_params(*c, boxes[i], true));
+        prefix.push_back(prefix.back() + qsro::obs_volume_blocks(b.nvox));
     }
-    std::vector<double> v(size_t(n) * qsro::NPART, 0.0);
-    HIP_TRY(hipMemcpyAsync(v.data(), c->d_mout, sizeof(double) * v.size(), hipMemcpyDeviceToHost, c->st));
-    HIP_TRY(hipStreamSynchronize(c->st));
-    for (int i = 0; i < n; ++i) {
-        const double* w = v.data() + (long long)i * qsro::NPART;
-        totals[i] = w[0] + w[1] + w[2];
+    const int nd = int(drop.size());
+    std::vector<double> v(size_t(nd) * qsro::NPART, 0.0);
+    if (nd > 0) {
+        if (int rc = ensure_many(*c, prefix.back() * qsro::NPART, (long long)nd * qsro::NPART)) return rc;
+        if (int rc = ensure_mtab(*c, nd)) return rc;
+        HIP_TRY(hipMemcpyAsync(c->d_mparams, P.data(), sizeof(qsro::ObsParams) * nd, hipMemcpyHostToDevice, c->st));
+        HIP_TRY(hipMemcpyAsync(c->d_mprefix, prefix.data(), sizeof(long long) * (nd + 1), hipMemcpyHostToDevice,
+                               c->st));
+        HIP_TRY(qsro::launch_obs_droplet_multi(c->st, c->q[c->cur], c->cs, c->d_mparams, c->d_mprefix, nd,
+                                               prefix.back(), c->d_step_rows, c->d_obs_rows, c->d_mpart, c->d_mout));
+        HIP_TRY(hipMemcpyAsync(v.data(), c->d_mout, sizeof(double) * v.size(), hipMemcpyDeviceToHost, c->st));
+        HIP_TRY(hipStreamSynchronize(c->st));
+    }
+    for (int m = 0; m < nd; ++m) {
+        const double* w = v.data() + (long long)m * qsro::NPART;
+        totals[drop[m]] = w[0] + w[1] + w[2];
     }
     const qsr_obs_opts oo = {-1, 0.1, 0.25, -1, -1.0, 0, 0, 1};
     for (int i : periodic) {
@@ -1433,16 +1460,19 @@ int box_residuals(void* ctx, const int* boxes, int n, double* res, long long* co
     if (!c->q_rhs) return fail(QSR_E_STATE, "no explicit step computed");
     if (n == 0) return QSR_OK;
     HIP_TRY(hipSetDevice(c->device));
-    constexpr long long kMaxBlk = 2048;  // launch_residual's block cap
-    if (int rc = ensure_many(*c, 2 * kMaxBlk * n, 2LL * n)) return rc;
+    // every box in one launch and one summing launch (box i owns the blocks of a single-box
+    // residual pass, so the sums are the same bit for bit)
+    std::vector<long long> prefix{0};
     for (int i = 0; i < n; ++i) {
         if (boxes[i] < 0 || boxes[i] >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad box");
-        long long nblk = 0;
-        double* part = c->d_mpart + 2 * kMaxBlk * i;
-        HIP_TRY(qsrs::launch_residual(c->st, c->q[c->cur], c->q_rhs, c->cs, c->h_boxes[boxes[i]], c->d_step_rows,
-                                      part, &nblk));
-        HIP_TRY(qsrs::launch_sum_slots(c->st, part, nblk, 2, c->d_mout + 2LL * i));
+        prefix.push_back(prefix.back() + qsrs::residual_blocks(c->boxes[boxes[i]].nvox));
     }
+    if (int rc = ensure_many(*c, 2 * prefix.back(), 2LL * n)) return rc;
+    if (int rc = ensure_mtab(*c, n)) return rc;
+    HIP_TRY(hipMemcpyAsync(c->d_mprefix, prefix.data(), sizeof(long long) * (n + 1), hipMemcpyHostToDevice, c->st));
+    HIP_TRY(hipMemcpyAsync(c->d_mlist, boxes, sizeof(int) * n, hipMemcpyHostToDevice, c->st));
+    HIP_TRY(qsrs::launch_residual_multi(c->st, c->q[c->cur], c->q_rhs, c->cs, c->d_boxes, c->d_mlist, c->d_mprefix, n,
+                                        prefix.back(), c->d_step_rows, c->d_mpart, c->d_mout));
     std::vector<double> v(2 * size_t(n));
     HIP_TRY(hipMemcpyAsync(v.data(), c->d_mout, sizeof(double) * v.size(), hipMemcpyDeviceToHost, c->st));
     HIP_TRY(hipStreamSynchronize(c->st));

@@ -138,5 +138,12 @@ hipError_t launch_obs_slice_stats(hipStream_t st, const double* nf_def, const do
                                   int want_checker, double* partials, long long* nblk);
 hipError_t launch_obs_finalize(hipStream_t st, const double* partials, long long nblk, int nslot, unsigned max_mask,
                                double* out);
+// The droplet volume pass and its finalize for n boxes in two launches: box m has parameters Ps[m]
+// and owns partial records [prefix[m], prefix[m+1]) (prefix[m+1] - prefix[m] = obs_volume_blocks of
+// its voxels, total_blocks = prefix[n]); its NPART sums go to out + m * NPART, bit for bit equal to
+// launch_obs_droplet + launch_obs_finalize on the box alone.
+hipError_t launch_obs_droplet_multi(hipStream_t st, const float* q, long long cs, const ObsParams* Ps,
+                                    const long long* prefix, int n, long long total_blocks, const RowMask* step_rows,
+                                    const RowMask* obs_rows, double* partials, double* out);
 hipError_t launch_nematic_aos(hipStream_t st, const float* q, long long cs, long long off, long long n, double* out);
 }  // namespace qsro

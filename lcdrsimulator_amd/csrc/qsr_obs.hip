@@ -138,10 +138,12 @@ __device__ __forceinline__ double wrap_loop(double d) {
 // (QSR.cu:654-695), avg/max S (QSR.cu:2285-2321), radiality (QSR.cu:994-1015).
 // partial slots: bulk, elastic, anchoring, S_sum, S_cnt, rad_sum, rad_cnt, max_S
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void obs_droplet_kernel(const float* __restrict__ q, long long cs, ObsParams P,
-                                                          const RowMask* __restrict__ step_rows,
-                                                          const RowMask* __restrict__ obs_rows,
-                                                          double* __restrict__ partials) {
+// Block bid of an nblk-block droplet volume pass over one box: its partials (NPART slots) into
+// partials + bid * NPART.
+__device__ __forceinline__ void droplet_block(const float* __restrict__ q, long long cs, const ObsParams& P,
+                                              const RowMask* __restrict__ step_rows,
+                                              const RowMask* __restrict__ obs_rows, double* __restrict__ partials,
+                                              long long bid, long long nblk) {
     double acc[NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
     // rows (i, j) in a grid stride, a thread's k fixed within a row: each thread accumulates its
     // voxels in a fixed order (deterministic sums) and splits a row index once, not per voxel
@@ -149,7 +151,7 @@ __global__ __launch_bounds__(256) void obs_droplet_kernel(const float* __restric
     const int kw = P.Nz < 256 ? P.Nz : 256;  // threads per row
     const int rpi = 256 / kw;                 // rows per block iteration
     const int rsub = threadIdx.x / kw, k0 = threadIdx.x - rsub * kw;
-    for (long long r = (long long)blockIdx.x * rpi + rsub; rsub < rpi && r < rows; r += (long long)gridDim.x * rpi) {
+    for (long long r = bid * rpi + rsub; rsub < rpi && r < rows; r += nblk * rpi) {
       const int i = int(r / P.Ny), j = int(r - (long long)i * P.Ny);
       for (int k = k0; k < P.Nz; k += kw) {
         const long long v = r * P.Nz + k;
@@ -248,7 +250,35 @@ __global__ __launch_bounds__(256) void obs_droplet_kernel(const float* __restric
       }
     }
     const bool is_max[NPART] = {false, false, false, false, false, false, false, true};
-    block_reduce_store<NPART>(acc, is_max, partials + (long long)blockIdx.x * NPART);
+    block_reduce_store<NPART>(acc, is_max, partials + bid * NPART);
+}
+
+__global__ __launch_bounds__(256) void obs_droplet_kernel(const float* __restrict__ q, long long cs, ObsParams P,
+                                                          const RowMask* __restrict__ step_rows,
+                                                          const RowMask* __restrict__ obs_rows,
+                                                          double* __restrict__ partials) {
+    droplet_block(q, cs, P, step_rows, obs_rows, partials, blockIdx.x, gridDim.x);
+}
+
+// The droplet volume pass of n boxes in one launch: global block g belongs to box m with
+// prefix[m] <= g < prefix[m+1] and computes exactly block g - prefix[m] of obs_droplet_kernel's
+// grid for that box (prefix[m+1] - prefix[m] blocks), writing its partials to partials + g * NPART.
+__global__ __launch_bounds__(256) void obs_droplet_multi_kernel(const float* __restrict__ q, long long cs,
+                                                                const ObsParams* __restrict__ Ps,
+                                                                const long long* __restrict__ prefix, int n,
+                                                                const RowMask* __restrict__ step_rows,
+                                                                const RowMask* __restrict__ obs_rows,
+                                                                double* __restrict__ partials) {
+    const long long g = blockIdx.x;
+    int lo = 0, hi = n - 1;  // the box whose block range holds g
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const ObsParams P = Ps[lo];
+    droplet_block(q, cs, P, step_rows, obs_rows, partials + prefix[lo] * NPART, g - prefix[lo],
+                  prefix[lo + 1] - prefix[lo]);
 }
 
 // ---------------------------------------------------------------------------
@@ -432,8 +462,8 @@ __global__ __launch_bounds__(256) void obs_slice_stats_kernel(const double* __re
 
 // Fixed-order reduction of nblk x nslot partials into out[nslot]; max_mask
 // bit s marks max slots.  One block.
-__global__ __launch_bounds__(256) void obs_finalize_kernel(const double* __restrict__ partials, long long nblk,
-                                                           int nslot, unsigned max_mask, double* __restrict__ out) {
+__device__ __forceinline__ void finalize_block(const double* __restrict__ partials, long long nblk, int nslot,
+                                               unsigned max_mask, double* __restrict__ out) {
     __shared__ double red[256];
     for (int s = 0; s < nslot; ++s) {
         const bool mx = (max_mask >> s) & 1u;
@@ -452,6 +482,21 @@ __global__ __launch_bounds__(256) void obs_finalize_kernel(const double* __restr
         if (threadIdx.x == 0) out[s] = red[0];
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(256) void obs_finalize_kernel(const double* __restrict__ partials, long long nblk,
+                                                           int nslot, unsigned max_mask, double* __restrict__ out) {
+    finalize_block(partials, nblk, nslot, max_mask, out);
+}
+
+// finalize_block for n boxes, one block each: box m's partials [prefix[m], prefix[m+1]) into
+// out + m * nslot.
+__global__ __launch_bounds__(256) void obs_finalize_multi_kernel(const double* __restrict__ partials,
+                                                                 const long long* __restrict__ prefix, int nslot,
+                                                                 unsigned max_mask, double* __restrict__ out) {
+    const int m = blockIdx.x;
+    finalize_block(partials + prefix[m] * nslot, prefix[m + 1] - prefix[m], nslot, max_mask,
+                   out + (long long)m * nslot);
 }
 
 // Nematic field of a whole box, AoS double {S, nx, ny, nz} (computeNematicFieldKernel, QSR.cu:1017-1025)
@@ -513,6 +558,18 @@ hipError_t launch_obs_slice_stats(hipStream_t st, const double* nf_def, const do
     *nblk = blocks_for(n);
     hipLaunchKernelGGL(obs_slice_stats_kernel, dim3(blocks_for(n)), dim3(256), 0, st, nf_def, nf_xi, P, obs_rows, z_def,
                        S_thr, cutoff, xi_thr, want_checker, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_obs_droplet_multi(hipStream_t st, const float* q, long long cs, const ObsParams* Ps,
+                                    const long long* prefix, int n, long long total_blocks, const RowMask* step_rows,
+                                    const RowMask* obs_rows, double* partials, double* out) {
+    if (n <= 0 || total_blocks <= 0) return hipSuccess;
+    if (total_blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(obs_droplet_multi_kernel, dim3(unsigned(total_blocks)), dim3(256), 0, st, q, cs, Ps, prefix, n,
+                       step_rows, obs_rows, partials);
+    hipLaunchKernelGGL(obs_finalize_multi_kernel, dim3(unsigned(n)), dim3(256), 0, st, partials, prefix, NPART, 1u << 7,
+                       out);
     return hipGetLastError();
 }
 

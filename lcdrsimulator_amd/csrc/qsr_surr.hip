@@ -669,8 +669,9 @@ __global__ __launch_bounds__(256) void loss_kernel(Work w, float scale) {
     block_sum_store<2>(acc, w.red + 2 * blockIdx.x);
 }
 
-__global__ __launch_bounds__(256) void sum_slots_kernel(const double* __restrict__ partials, long long nblk,
-                                                        int nslot, double* __restrict__ out) {
+// Fixed-order sum of nblk x nslot partials into out[nslot], by one block.
+__device__ __forceinline__ void sum_slots_block(const double* __restrict__ partials, long long nblk, int nslot,
+                                                double* __restrict__ out) {
     __shared__ double red[256];
     for (int s = 0; s < nslot; ++s) {
         double a = 0.0;
@@ -684,6 +685,31 @@ __global__ __launch_bounds__(256) void sum_slots_kernel(const double* __restrict
         if (threadIdx.x == 0) out[s] = red[0];
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(256) void sum_slots_kernel(const double* __restrict__ partials, long long nblk,
+                                                        int nslot, double* __restrict__ out) {
+    sum_slots_block(partials, nblk, nslot, out);
+}
+
+// The same for n lists: block m sums partials [prefix[m], prefix[m+1]) into out + m * nslot.
+__global__ __launch_bounds__(256) void sum_slots_multi_kernel(const double* __restrict__ partials,
+                                                              const long long* __restrict__ prefix, int nslot,
+                                                              double* __restrict__ out) {
+    const int m = blockIdx.x;
+    sum_slots_block(partials + prefix[m] * nslot, prefix[m + 1] - prefix[m], nslot, out + (long long)m * nslot);
+}
+
+// Global block g of a multi-box launch over prefix[0..n] (prefix[m] = first block of box m):
+// its box m and its block index within the box.
+__device__ __forceinline__ int block_box(long long g, const long long* __restrict__ prefix, int n) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 __global__ __launch_bounds__(256) void pair_power_kernel(const float* __restrict__ qa, const float* __restrict__ qb,
@@ -711,13 +737,14 @@ __global__ __launch_bounds__(256) void mask_bytes_kernel(long long nvox, BoxDev 
     out[v] = changeable(B, rows, i, j, k) ? 1 : 0;
 }
 
-__global__ __launch_bounds__(256) void residual_kernel(const float* __restrict__ q, const float* __restrict__ rhs,
-                                                       long long cs, BoxDev B, const RowMask* __restrict__ rows,
-                                                       double* __restrict__ partials) {
+// One block's share (block bid of nblk, grid stride over the box's voxels) of the residual
+// partials: sum of |rhs - q|^2 and the voxel count.
+__device__ __forceinline__ void residual_block(const float* __restrict__ q, const float* __restrict__ rhs,
+                                               long long cs, const BoxDev& B, const RowMask* __restrict__ rows,
+                                               double* __restrict__ partials, long long bid, long long nblk) {
     const long long nvox = (long long)B.Nx * B.Ny * B.Nz;
     double acc[2] = {0.0, 0.0};
-    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
-         v += (long long)gridDim.x * blockDim.x) {
+    for (long long v = bid * blockDim.x + threadIdx.x; v < nvox; v += nblk * blockDim.x) {
         const int k = int(v % B.Nz), j = int((v / B.Nz) % B.Ny), i = int(v / ((long long)B.Nz * B.Ny));
         if (!stepped_interior(B, rows, i, j, k)) continue;
 #pragma unroll
@@ -728,7 +755,29 @@ __global__ __launch_bounds__(256) void residual_kernel(const float* __restrict__
         }
         acc[1] += 1.0;
     }
-    block_sum_store<2>(acc, partials + 2 * blockIdx.x);
+    block_sum_store<2>(acc, partials + 2 * bid);
+}
+
+__global__ __launch_bounds__(256) void residual_kernel(const float* __restrict__ q, const float* __restrict__ rhs,
+                                                       long long cs, BoxDev B, const RowMask* __restrict__ rows,
+                                                       double* __restrict__ partials) {
+    residual_block(q, rhs, cs, B, rows, partials, blockIdx.x, gridDim.x);
+}
+
+// Residual partials of n boxes in one launch: global block g belongs to box m = boxes[...] with
+// prefix[m] <= g < prefix[m+1], and computes exactly block g - prefix[m] of residual_kernel's
+// grid for that box; its partials go to partials + 2 g.
+__global__ __launch_bounds__(256) void residual_multi_kernel(const float* __restrict__ q,
+                                                             const float* __restrict__ rhs, long long cs,
+                                                             const BoxDev* __restrict__ boxes,
+                                                             const int* __restrict__ list,
+                                                             const long long* __restrict__ prefix, int n,
+                                                             const RowMask* __restrict__ rows,
+                                                             double* __restrict__ partials) {
+    const long long g = blockIdx.x;
+    const int m = block_box(g, prefix, n);
+    const BoxDev B = boxes[list[m]];
+    residual_block(q, rhs, cs, B, rows, partials + 2 * prefix[m], g - prefix[m], prefix[m + 1] - prefix[m]);
 }
 
 // The increment of one explicit step, rhs - q, on the voxels the residual sums
@@ -945,6 +994,19 @@ hipError_t launch_residual(hipStream_t st, const float* q, const float* rhs, lon
     const unsigned g = reduce_blocks(nvox);
     *nblk = g;
     hipLaunchKernelGGL(residual_kernel, dim3(g), dim3(256), 0, st, q, rhs, cs, box, rows, partials);
+    return hipGetLastError();
+}
+
+long long residual_blocks(long long nvox) { return reduce_blocks(nvox); }
+
+hipError_t launch_residual_multi(hipStream_t st, const float* q, const float* rhs, long long cs,
+                                 const qsrk::BoxDev* boxes, const int* list, const long long* prefix, int n,
+                                 long long total_blocks, const qsrk::RowMask* rows, double* partials, double* out) {
+    if (n <= 0 || total_blocks <= 0) return hipSuccess;
+    if (total_blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(residual_multi_kernel, dim3(unsigned(total_blocks)), dim3(256), 0, st, q, rhs, cs, boxes, list,
+                       prefix, n, rows, partials);
+    hipLaunchKernelGGL(sum_slots_multi_kernel, dim3(unsigned(n)), dim3(256), 0, st, partials, prefix, 2, out);
     return hipGetLastError();
 }
 

@@ -186,6 +186,15 @@ hipError_t launch_mask_bytes(hipStream_t st, long long nvox, const qsrk::BoxDev&
 // change, box faces excluded.
 hipError_t launch_residual(hipStream_t st, const float* q, const float* rhs, long long cs, const qsrk::BoxDev& box,
                            const qsrk::RowMask* rows, double* partials, long long* nblk);
+// Blocks (= partial records) of launch_residual's grid for a box of nvox voxels.
+long long residual_blocks(long long nvox);
+// The residual partials and their sums (out + 2 m: sum |rhs - q|^2, count) of n boxes in two
+// launches: box m is boxes[list[m]] and owns partial records [prefix[m], prefix[m+1]), prefix[m+1] -
+// prefix[m] = residual_blocks(its voxels), total_blocks = prefix[n]; each box's values equal
+// launch_residual + launch_sum_slots on it alone, bit for bit.
+hipError_t launch_residual_multi(hipStream_t st, const float* q, const float* rhs, long long cs,
+                                 const qsrk::BoxDev* boxes, const int* list, const long long* prefix, int n,
+                                 long long total_blocks, const qsrk::RowMask* rows, double* partials, double* out);
 hipError_t launch_copy_box(hipStream_t st, const float* q, long long cs, long long off, long long nvox, float* dst);
 // One explicit step's increment rhs - q of a box on the voxels the residual sums
 // over (zero elsewhere), into five planes of the box's voxel count.
