@@ -175,6 +175,9 @@ int qsr_autotune(void* ctx, int trial_steps, int* best_variant, int* best_ch, do
  * as they are), so a quench-rate ladder can retire finished droplets early.
  * No reference counterpart (the reference runs one droplet per process). */
 int qsr_set_box_active(void* ctx, int box, int active);
+/* The same for the n listed boxes at once (one synchronisation and one rebuild of
+ * the tile tables, where n calls of qsr_set_box_active make n). */
+int qsr_set_boxes_active(void* ctx, const int* boxes, int n, int active);
 /* Set the stencil variant (tile shape: 0 = 64 k x 4 j rows, 1 = 32 k x 8 j rows)
  * and planes per tile explicitly. */
 int qsr_set_tuning(void* ctx, int variant, int planes_per_tile);

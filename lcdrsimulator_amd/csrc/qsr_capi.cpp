@@ -111,12 +111,12 @@ struct Ctx {
     qsrk::BoxDev* d_boxes = nullptr;
     std::vector<int4> h_tiles;
     int4* d_tiles = nullptr;
-    int ntiles = 0;
+    int ntiles = 0, tiles_cap = 0;
     // two-step L1 kernel: its own tile table (X2_OUT_K x X2_OUT_J outputs, ch2 planes)
     bool fuse = true;      // pair steps when the context allows it (qsr_set_step_fusion)
     int ch2 = 16;
     int4* d_tiles2 = nullptr;
-    int ntiles2 = 0;
+    int ntiles2 = 0, tiles2_cap = 0;
     long long tile_voxels2 = 0;
     long long tile_voxels = 0, active_voxels = 0, box_voxels = 0;
     qsrk::RowMask* d_step_rows = nullptr;
@@ -273,7 +273,8 @@ bool live_k_bounds(const Box& b, int ib, int ie, int jb, int je, int& lo, int& h
 
 // Tiles of tk x tj outputs (k x j) and ch planes over the active boxes, in XCD-aware launch
 // order, uploaded to *d (ntiles entries, padding included); voxels = outputs covered.
-int make_tiles(Ctx& c, int tk, int tj, int ch, int4** d, int* ntiles, long long* voxels, std::vector<int4>* host) {
+int make_tiles(Ctx& c, int tk, int tj, int ch, int4** d, int* cap, int* ntiles, long long* voxels,
+               std::vector<int4>* host) {
     std::vector<int4> nat;
     *voxels = 0;
     for (size_t bi = 0; bi < c.boxes.size(); ++bi) {
@@ -320,12 +321,14 @@ int make_tiles(Ctx& c, int tk, int tj, int ch, int4** d, int* ntiles, long long*
             if (n < T) order[size_t(8) * r + x] = nat[n];
         }
     *ntiles = int(order.size());
-    discard(hipFree(*d));
-    *d = nullptr;
-    if (*ntiles > 0) {
+    if (*ntiles > *cap) {  // the table keeps its allocation while the new one fits
+        discard(hipFree(*d));
+        *d = nullptr;
+        *cap = 0;
         HIP_TRY(hipMalloc(d, sizeof(int4) * size_t(*ntiles)));
-        HIP_TRY(hipMemcpy(*d, order.data(), sizeof(int4) * size_t(*ntiles), hipMemcpyHostToDevice));
+        *cap = *ntiles;
     }
+    if (*ntiles > 0) HIP_TRY(hipMemcpy(*d, order.data(), sizeof(int4) * size_t(*ntiles), hipMemcpyHostToDevice));
     if (host) *host = std::move(order);
     return QSR_OK;
 }
@@ -336,17 +339,16 @@ bool fusable(const Ctx& c) { return c.fuse && c.elastic == 0 && !c.noise && c.nf
 
 int build_tiles(Ctx& c) {
     // tile shape of the current variant (k extent x j rows)
-    if (int rc = make_tiles(c, qsrk::tile_k(c.variant), qsrk::tile_j(c.variant), c.ch, &c.d_tiles, &c.ntiles,
-                            &c.tile_voxels, &c.h_tiles))
+    if (int rc = make_tiles(c, qsrk::tile_k(c.variant), qsrk::tile_j(c.variant), c.ch, &c.d_tiles, &c.tiles_cap,
+                            &c.ntiles, &c.tile_voxels, &c.h_tiles))
         return rc;
     if (!fusable(c)) {
-        discard(hipFree(c.d_tiles2));
-        c.d_tiles2 = nullptr;
         c.ntiles2 = 0;
         c.tile_voxels2 = 0;
         return QSR_OK;
     }
-    return make_tiles(c, qsrk::X2_OUT_K, qsrk::X2_OUT_J, c.ch2, &c.d_tiles2, &c.ntiles2, &c.tile_voxels2, nullptr);
+    return make_tiles(c, qsrk::X2_OUT_K, qsrk::X2_OUT_J, c.ch2, &c.d_tiles2, &c.tiles2_cap, &c.ntiles2,
+                      &c.tile_voxels2, nullptr);
 }
 
 // Box-face voxels that can change in a step (the face kernel's live set: evolving
@@ -837,26 +839,35 @@ int qsr_autotune(void* ctx, int trial_steps, int* best_variant, int* best_ch, do
     return QSR_OK;
 }
 
-int qsr_set_box_active(void* ctx, int box, int active) {
+int qsr_set_box_active(void* ctx, int box, int active) { return qsr_set_boxes_active(ctx, &box, 1, active); }
+
+int qsr_set_boxes_active(void* ctx, const int* boxes, int n, int active) {
     Ctx* c = as_ctx(ctx);
-    if (!c || box < 0 || box >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad argument");
-    Box& b = c->boxes[size_t(box)];
+    if (!c || !boxes || n < 0) return fail(QSR_E_ARG, "bad argument");
+    for (int i = 0; i < n; ++i)
+        if (boxes[i] < 0 || boxes[i] >= int(c->boxes.size())) return fail(QSR_E_ARG, "bad argument");
     const bool on = active != 0;
-    if (b.active == on) return QSR_OK;
+    bool changed = false;
+    for (int i = 0; i < n; ++i) changed |= c->boxes[size_t(boxes[i])].active != on;
+    if (!changed) return QSR_OK;
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->st));
-    if (!on) {
-        // both ping-pong buffers hold the held state, whichever one later steps leave current
-        for (int comp = 0; comp < 5; ++comp) {
-            const size_t o = size_t(comp) * size_t(c->cs) + size_t(b.off);
-            HIP_TRY(hipMemcpyAsync(c->q[1 - c->cur] + o, c->q[c->cur] + o, sizeof(float) * size_t(b.nvox),
-                                   hipMemcpyDeviceToDevice, c->st));
+    for (int i = 0; i < n; ++i) {
+        Box& b = c->boxes[size_t(boxes[i])];
+        if (b.active == on) continue;
+        if (!on) {
+            // both ping-pong buffers hold the held state, whichever one later steps leave current
+            for (int comp = 0; comp < 5; ++comp) {
+                const size_t o = size_t(comp) * size_t(c->cs) + size_t(b.off);
+                HIP_TRY(hipMemcpyAsync(c->q[1 - c->cur] + o, c->q[c->cur] + o, sizeof(float) * size_t(b.nvox),
+                                       hipMemcpyDeviceToDevice, c->st));
+            }
         }
+        b.active = on;
+        qsrk::BoxDev& d = c->h_boxes[size_t(boxes[i])];
+        d.flags = on ? (d.flags & ~qsrk::F_INACTIVE) : (d.flags | qsrk::F_INACTIVE);
+        HIP_TRY(hipMemcpyAsync(c->d_boxes + boxes[i], &d, sizeof(qsrk::BoxDev), hipMemcpyHostToDevice, c->st));
     }
-    b.active = on;
-    qsrk::BoxDev& d = c->h_boxes[size_t(box)];
-    d.flags = on ? (d.flags & ~qsrk::F_INACTIVE) : (d.flags | qsrk::F_INACTIVE);
-    HIP_TRY(hipMemcpyAsync(c->d_boxes + box, &d, sizeof(qsrk::BoxDev), hipMemcpyHostToDevice, c->st));
     HIP_TRY(hipStreamSynchronize(c->st));
     if (int rc = build_tiles(*c)) return rc;
     if (c->gexec) discard(hipGraphExecDestroy(c->gexec));

@@ -74,7 +74,7 @@ EXPORTS = (
     "qsr_step_imex", "qsr_observables", "qsr_box_info_get", "qsr_set_iteration", "qsr_last_step_ms",
     "qsr_step_work", "qsr_profile_steps", "qsr_autotune", "qsr_set_tuning", "qsr_set_step_fusion",
     "qsr_step_fusion_info", "qsr_S_eq", "qsr_dt_estimate", "qsr_init_field",
-    "qsr_load_qtensor_file", "qsr_save_qtensor_file", "qsr_set_box_active",
+    "qsr_load_qtensor_file", "qsr_save_qtensor_file", "qsr_set_box_active", "qsr_set_boxes_active",
     "qsr_surr_num_params", "qsr_surr_create", "qsr_surr_destroy", "qsr_surr_init_params", "qsr_surr_set_params",
     "qsr_surr_get_params", "qsr_surr_save", "qsr_surr_load", "qsr_surr_eval", "qsr_surr_adam_step",
     "qsr_surr_dataset_add", "qsr_surr_dataset_size", "qsr_surr_dataset_clear", "qsr_surr_train", "qsr_surr_apply",
@@ -122,6 +122,7 @@ def load():
     L.qsr_set_step_fusion.argtypes = [vp, C.c_int, C.c_int]
     L.qsr_step_fusion_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_uint64)]
     L.qsr_set_box_active.argtypes = [vp, C.c_int, C.c_int]
+    L.qsr_set_boxes_active.argtypes = [vp, C.POINTER(C.c_int), C.c_int, C.c_int]
     L.qsr_step_work.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.qsr_S_eq.restype = C.c_double
     L.qsr_S_eq.argtypes = [C.c_double] * 5 + [C.c_int]
@@ -358,6 +359,12 @@ class Solver:
     def set_active(self, box: int, active: bool):
         """Hold (False) or resume (True) one box of a batch: held boxes are not stepped."""
         _check(load().qsr_set_box_active(self._h, int(box), int(bool(active))))
+
+    def set_active_many(self, boxes, active: bool):
+        """set_active for every listed box at once (one tile rebuild)."""
+        bx = np.ascontiguousarray(boxes, dtype=np.int32).ravel()
+        _check(load().qsr_set_boxes_active(self._h, bx.ctypes.data_as(C.POINTER(C.c_int)), bx.size,
+                                           int(bool(active))))
 
     def work(self) -> dict:
         a, t, b = C.c_uint64(), C.c_uint64(), C.c_uint64()

@@ -229,17 +229,19 @@ def relax_batch(s: q.Solver, n_boxes: int, net: q.Surrogate | None = None, tol: 
             s.step(every)
             done += 2 * every
             F = energies(active)
-        still = []
+        still, done_now = [], []
         for b in active:
             ref = prev[b] if net is None else Fa[b]
             if net is not None and tail_tol is not None:
                 tail[b] = abs((F[b] - ref) / ref) < tail_tol
             if ref is not None and abs((F[b] - ref) / ref) < tol:
                 steps[b] = done
-                s.set_active(b, False)
+                done_now.append(b)
             else:
                 still.append(b)
             prev[b] = F[b]
+        if done_now:
+            s.set_active_many(done_now, False)
         active = still
     return {"seconds": time.perf_counter() - t0, "steps": steps, "total_steps": done,
             "converged": n_boxes - len(active), "jumps_accepted": acc, "jumps_rejected": att - acc,

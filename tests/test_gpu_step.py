@@ -205,6 +205,35 @@ def test_imex_converged_sweeps_are_exact(qsr, orc):
     assert np.abs(runs[25] - Qo).max() <= 4 * STEP_TOL
 
 
+def test_holding_boxes_together_equals_holding_them_one_by_one(qsr, orc):
+    """qsr_set_boxes_active on a list gives the fields and iteration counts of one qsr_set_box_active
+    call per box, bit for bit, in an L1 / kappa batch (held and resumed mid-run)."""
+    cases = [CASES["l1_weak_custom"], CASES["kappa_strong_odd"], CASES["l1_strong_64"], CASES["offcentre_radius"]]
+    built = [_case(qsr, orc, **c, seed=400 + t) for t, c in enumerate(cases)]
+    out = []
+    for together in (True, False):
+        with qsr.Solver([b[0] for b in built]) as s:
+            for t, b in enumerate(built):
+                s.upload(b[2], box=t)
+            s.step(3)
+            if together:
+                s.set_active_many([2, 0], False)
+            else:
+                s.set_active(2, False)
+                s.set_active(0, False)
+            s.step(5)
+            if together:
+                s.set_active_many([0, 2, 3], True)  # box 3 is active already
+            else:
+                s.set_active(0, True)
+                s.set_active(2, True)
+            s.step(4)
+            out.append(([s.download(box=t) for t in range(len(built))], [s.info(t)["iteration"] for t in range(4)]))
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
+    assert out[0][1] == out[1][1] == [7, 12, 7, 12]
+
+
 def test_held_box_is_frozen_and_others_continue(qsr, orc):
     """A held box keeps its field and iteration count while the rest of the batch steps on; resumed, it
     continues from where it stopped (quench-rate ladders retire finished droplets this way)."""

@@ -104,6 +104,11 @@ class _FakeSolver:
     def set_active(self, b, on):
         self.active[b] = bool(on)
 
+    def set_active_many(self, boxes, on):
+        self.log.append(("retire" if not on else "resume", tuple(boxes)))
+        for b in boxes:
+            self.active[b] = bool(on)
+
 
 class _FakeNet:
     """Keeps a box's proposal when decide(box, call) says so, shrinking its amplitude."""
