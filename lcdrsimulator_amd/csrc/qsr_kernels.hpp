@@ -63,9 +63,11 @@ constexpr int TILE_THREADS = 256;
 constexpr int tile_k(int variant) { return variant == 1 ? 32 : 64; }
 constexpr int tile_j(int variant) { return variant == 1 ? 8 : 4; }
 
-// Two-step L1 kernel: 512 threads over a region of X2_TK k x X2_TJ j points, whose
-// inner (X2_TK - 2) x (X2_TJ - 2) points are the tile's outputs.
-constexpr int X2_TK = 32, X2_TJ = 16;
+// Two-step L1 kernel: 256 threads over a region of X2_TK k x X2_TJ j points, whose
+// inner (X2_TK - 2) x (X2_TJ - 2) points are the tile's outputs.  Eight rows rather than
+// 16: at 93 VGPRs five 256-thread blocks fit a CU (20 waves) where two 512-thread blocks
+// did (16), and the 30 x 6 output tiles follow small droplets more closely.
+constexpr int X2_TK = 32, X2_TJ = 8;
 constexpr int X2_OUT_K = X2_TK - 2, X2_OUT_J = X2_TJ - 2;
 
 }  // namespace qsrk

@@ -411,9 +411,9 @@ __global__ __launch_bounds__(TK* TJ, (!NOISE && BUF) ? 7 : 1) void step_l1_kerne
 
 // ---------------------------------------------------------------------------
 // Two L1 / kappa steps in one pass (temporal blocking): the launch reads the field
-// once and writes it once for two explicit steps.  A block of 512 threads covers a
-// region of 32 k x 16 j points and streams along i; step 1 updates every region
-// point, step 2 the inner 30 x 14 points, which are the tile's outputs.  For each
+// once and writes it once for two explicit steps.  A block of 256 threads covers a
+// region of 32 k x 8 j points and streams along i; step 1 updates every region
+// point, step 2 the inner 30 x 6 points, which are the tile's outputs.  For each
 // plane s of the stream:
 //   * plane s of the input (own column + a one-point ring around the region) goes
 //     into an LDS slot; the own column also runs through a register queue along i;
