@@ -230,7 +230,7 @@ int qsr_init_field(const qsr_params* p, int mode, double S0, double noise_amplit
 /* ---------------------------------------------------------------------------
  * Learned warm-start step.  No counterpart in the reference: QSR.cu only has
  * the plain relaxation loop (QSR.cu:3126-3476), which stays the fallback and
- * the judge of every state.  A three-layer 3x3x3 convolution network (5 -> 32
+ * the arbiter of every state.  A three-layer 3x3x3 convolution network (5 -> 32
  * -> 32 -> 5 channels, GELU, 36,357 parameters) predicts dQ = Q(k+N) - Q(k)
  * from Q(k) (field input), or modulates the increment u(k) = E(Q(k)) - Q(k)
  * of one explicit Euler step E: dQ = N u (1 + y) per voxel and component, the

@@ -2,7 +2,7 @@
 
 No counterpart in the reference: QSR.cu relaxes with the plain loop only
 (sim_mode 1, QSR.cu:3126-3476; early stop on |dF/F| between log points,
-QSR.cu:3322-3335).  Here that loop stays both the fallback and the judge, and
+QSR.cu:3322-3335).  Here that loop stays both the fallback and the arbiter, and
 a learned step (:class:`lcdrsimulator_amd.Surrogate`) may be interleaved:
 
 * training data: trajectories of the plain solver itself, droplets of radius

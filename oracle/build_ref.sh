@@ -4,7 +4,7 @@
 # QSR.cpp's interactive main() is renamed so its free functions can be called.
 # Flags follow the reference's documented build line (QSR.h:183:
 # g++ -O3 -std=c++23 -fopenmp).  Exits 0 without building when the reference
-# tree is absent (the GPU box uses the prebuilt library).
+# tree is absent (a machine without it uses the prebuilt library).
 set -euo pipefail
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 REF="${QSR_REFERENCE_DIR:-/root/reference}"
