@@ -118,6 +118,8 @@ typedef struct qsr_box_info {
 int qsr_create(const qsr_params* p, int device, void** ctx);
 int qsr_create_batch(const qsr_params* p, int n_boxes, int device, void** ctx);
 int qsr_destroy(void* ctx);
+/* No counterpart (the reference exits on errors, QSR.cu:13-24): the calling
+ * thread's last error message, and the number of visible HIP devices. */
 const char* qsr_last_error(void);
 int qsr_device_count(int* n);
 
@@ -157,8 +159,11 @@ int qsr_energies(void* ctx, const int* boxes, int n, double* totals);
  * calls that overlap on one GPU (several host threads, several contexts on
  * one device) race; serialise them.  totals is a host array of n doubles. */
 int qsr_energies_noalloc(void* ctx, const int* boxes, int n, double* totals);
+/* Shell diagnostics of a box, the block QSR.cu prints before a run
+ * (QSR.cu:2179-2187; geometry QSR.cu:164-185), and its step count. */
 int qsr_box_info_get(void* ctx, int box, qsr_box_info* out);
-/* Iteration counter of a box (T(iter) follows the protocol). */
+/* Iteration counter of a box, the loop's `iter` that T(iter) follows
+ * (QSR.cu:2679-2687). */
 int qsr_set_iteration(void* ctx, int box, int64_t iteration);
 
 /* Instrumentation for measurement: average duration (ms) of the last `qsr_step`
