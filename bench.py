@@ -335,7 +335,7 @@ def ladder_relax(q, args, surr_traj=None, trained=True):
                     s.upload(Q, box=b)
                 s.set_tuning(tune["variant"], tune["planes_per_tile"])
                 tail = 1e-5 if args.warm_tail_tol is None else (args.warm_tail_tol or None)
-                r = ws.relax_batch(s, n, nt, tol, every, nmax, tail_tol=tail)
+                r = ws.relax_batch(s, n, nt, tol, every, nmax, tail_tol=tail, short=not args.warm_long_cycles)
                 r["final"] = [s.observables(box=b, want_xi=False) for b in range(n)]
             warm[name] = r
             log(f"ladder relaxation batched + {name} network: {r['seconds']:.3f} s, {r['converged']} of {n} converged, "
@@ -392,7 +392,12 @@ def ladder_relax(q, args, surr_traj=None, trained=True):
            "same_steps": int(sum(a == c for a, c in zip(steps_b, steps_s))),
            "same_defects": same_defects, "max_rel_dF_final": max_rel_dF,
            "defects_per_droplet_batched": [int(o["defects"]) for o in final_b],
-           "batched_plain_speedup": t_seq / t_batched}
+           "batched_plain_speedup": t_seq / t_batched,
+           "warm_cycle": ("a checked learned step and 2 x %d plain steps, the test between the two energies" % every
+                          if args.warm_long_cycles else
+                          "a checked learned step and %d plain steps, the test from the energy right after the "
+                          "step (or before it, when refused) to the energy %d plain steps later; no convergence "
+                          "in a cycle whose step was kept" % (every, every))}
     for name, r in warm.items():
         out["warm_trained" if name == "trained" else "warm_zero_network"] = vs_plain(r)
     if tr is not None:
@@ -613,6 +618,11 @@ def main():
                          "below this, so the plain solver makes the last approach (0: off; default 1e-5 for the "
                          "ladder's radial starts, off for surrogate_192, whose random-director start coarsens through "
                          "long stretches of small |dF/F| where the learned steps are what speeds it up)")
+    ap.add_argument("--warm-long-cycles", action="store_true",
+                    help="ladder_relax_64: warm-started cycles of a learned step and 2 x relax_every plain steps "
+                         "with the test between the two energies (the default for the ladder's radial starts: a "
+                         "learned step and relax_every plain steps, tested from the energy right after the step, "
+                         "never converging in a cycle whose step was kept)")
     ap.add_argument("--surr-first", type=int, default=500,
                     help="training snapshots start this many plain steps into each trajectory (from step 0, the "
                          "network's first jumps from radial starts reshape the droplet cores)")

@@ -146,14 +146,22 @@ def relax_plain(s: q.Solver, tol: float = 1e-6, every: int = 100, max_steps: int
 
 
 def relax_warm(s: q.Solver, net: q.Surrogate, tol: float = 1e-6, every: int = 100, max_steps: int = 200000,
-               scales=(1.0, 0.5, 0.25), tail_tol: float | None = None, hold: int = 0) -> dict:
+               scales=(1.0, 0.5, 0.25), tail_tol: float | None = None, hold: int = 0, short: bool = False) -> dict:
     """Cycles of one checked learned step (the ladder ``scales``: the first proposal that
     raises neither F nor the residual is kept, else the field is left as it was) and 2 x ``every``
     plain steps with the energy checked after each half; converged when |dF/F| < tol between
     the two checks of a cycle, which no jump separates (the energy part of QSR.cu:3322-3335).
     With ``tail_tol``, a cycle whose predecessor's |dF/F| was below it takes no learned step (the
     plain solver alone makes the last approach; a plateau of a coarsening run only pauses them).
-    No learned step is tried in the first ``hold`` plain steps.  ``steps`` counts plain steps only."""
+    No learned step is tried in the first ``hold`` plain steps.  With ``short`` a cycle is the
+    learned step and ``every`` plain steps: the test interval runs from the energy after the learned
+    step (the check's F_after, or F_before when it was refused) to the energy ``every`` steps later,
+    and the run converges only in a cycle whose learned step was not kept.  That halves the plain
+    steps per learned step; on coarsening (random-director) starts it also speeds up the coarsening
+    itself, so there the defect count moves away from the plain run's.  ``steps`` counts plain
+    steps only."""
+    if short:
+        return _relax_warm_short(s, net, tol, every, max_steps, scales, tail_tol, hold)
     n, att, acc, net_ms = 0, 0, 0, 0.0
     kept = []
     conv, rel = False, math.nan
@@ -181,17 +189,52 @@ def relax_warm(s: q.Solver, net: q.Surrogate, tol: float = 1e-6, every: int = 10
             "jumps_accepted": acc, "jumps_rejected": att - acc, "scales_kept": kept, "network_ms": net_ms}
 
 
+def _relax_warm_short(s, net, tol, every, max_steps, scales, tail_tol, hold) -> dict:
+    n, att, acc, net_ms = 0, 0, 0, 0.0
+    kept_scales = []
+    conv, rel = False, math.nan
+    t0 = time.perf_counter()
+    tail = False
+    prev = s.energies([0])[0]
+    while n < max_steps:
+        kept = False
+        F0 = prev
+        if not tail and n >= hold:
+            r = net.apply(s, q.SURR_CHECKED, scales=list(scales))
+            att += 1
+            net_ms += r["ms_network"]
+            if r["accepted"]:
+                acc += 1
+                kept = True
+                kept_scales.append(r["scale"])
+                F0 = r["F_after"]
+        s.step(every)
+        n += every
+        F = s.energies([0])[0]
+        rel = abs((F - F0) / F0)
+        prev = F
+        if rel < tol and not kept:
+            conv = True
+            break
+        tail = tail_tol is not None and rel < tail_tol
+    return {"steps": n, "seconds": time.perf_counter() - t0, "converged": conv, "last_rel_dF": rel,
+            "jumps_accepted": acc, "jumps_rejected": att - acc, "scales_kept": kept_scales, "network_ms": net_ms}
+
+
 def relax_batch(s: q.Solver, n_boxes: int, net: q.Surrogate | None = None, tol: float = 1e-6, every: int = 100,
                 max_steps: int = 200000, scales=(1.0, 0.5, 0.25), backoff: bool = True,
-                tail_tol: float | None = None, hold: int = 0) -> dict:
+                tail_tol: float | None = None, hold: int = 0, short: bool = False) -> dict:
     """Relax every box of a batched solver, retiring each (qsr_set_box_active) once its energy
     test passes.  Plain (no ``net``): the test of :func:`relax_plain` per box, checks every
     ``every`` steps.  Warm-started: the cycles of :func:`relax_warm`, with one checked learned step
     per active box per cycle (:meth:`Surrogate.apply_boxes`); with ``backoff`` a box whose step was
     refused sits out the next 1, 2, 4, ... cycles (reset when one is kept).  With ``tail_tol`` a box
-    whose |dF/F| over the last cycle's second half was below it takes no learned step in the next
+    whose |dF/F| over the last cycle's test interval was below it takes no learned step in the next
     cycle: the plain solver alone makes the last approach to the test.  No learned step is tried in
-    the first ``hold`` plain steps.  ``steps`` per box counts plain steps."""
+    the first ``hold`` plain steps.  With ``short`` a cycle is the learned step and ``every`` plain
+    steps (see :func:`relax_warm`).  ``steps`` per box counts plain steps."""
+    if net is not None and short:
+        return _relax_batch_short(s, n_boxes, net, tol, every, max_steps, scales, backoff, tail_tol, hold)
     active = list(range(n_boxes))
     steps = [0] * n_boxes
     prev = [None] * n_boxes
@@ -248,6 +291,62 @@ def relax_batch(s: q.Solver, n_boxes: int, net: q.Surrogate | None = None, tol: 
             "network_ms": net_ms, "apply_s": apply_s}
 
 
+def _relax_batch_short(s, n_boxes, net, tol, every, max_steps, scales, backoff, tail_tol, hold) -> dict:
+    """relax_batch's short cycles: one checked learned step per selected box, then ``every`` plain
+    steps; a box's test interval runs from its energy after the learned step (the check's F_after,
+    or F_before when the step was refused or not tried) to its energy ``every`` plain steps later,
+    and a box converges only in a cycle whose learned step was not kept."""
+    active = list(range(n_boxes))
+    steps = [0] * n_boxes
+    wait, gap = [0] * n_boxes, [0] * n_boxes
+    tail = [False] * n_boxes
+    done, acc, att, net_ms, apply_s = 0, 0, 0, 0.0, 0.0
+    t0 = time.perf_counter()
+    prev = dict(zip(active, s.energies(active)))
+    while active and done < max_steps:
+        sel = [b for b in active if wait[b] == 0 and not tail[b]] if done >= hold else []
+        for b in active:
+            wait[b] = max(0, wait[b] - 1)
+        ta = time.perf_counter()
+        res = net.apply_boxes(s, sel, q.SURR_CHECKED, scales) if sel else []
+        apply_s += time.perf_counter() - ta
+        F0 = {b: prev[b] for b in active}
+        kept = set()
+        for b, r in zip(sel, res):
+            if r["accepted"]:
+                gap[b] = 0
+                F0[b] = r["F_after"]
+                kept.add(b)
+            else:
+                F0[b] = r["F_before"]
+                if backoff:
+                    gap[b] = max(1, 2 * gap[b])
+                    wait[b] = gap[b]
+        att += len(res)
+        acc += len(kept)
+        net_ms += res[0]["ms_network"] if res else 0.0
+        s.step(every)
+        done += every
+        F = dict(zip(active, s.energies(active)))
+        still, done_now = [], []
+        for b in active:
+            rel = abs((F[b] - F0[b]) / F0[b])
+            if tail_tol is not None:
+                tail[b] = rel < tail_tol
+            if rel < tol and b not in kept:
+                steps[b] = done
+                done_now.append(b)
+            else:
+                still.append(b)
+            prev[b] = F[b]
+        if done_now:
+            s.set_active_many(done_now, False)
+        active = still
+    return {"seconds": time.perf_counter() - t0, "steps": steps, "total_steps": done,
+            "converged": n_boxes - len(active), "jumps_accepted": acc, "jumps_rejected": att - acc,
+            "network_ms": net_ms, "apply_s": apply_s}
+
+
 def zero_output_layer(net: q.Surrogate):
     """Zero the last layer's weights and bias (W3, b3: the tail of the flat parameter array), so
     that a modulating network starts as the plain step extrapolated and training moves it from
@@ -267,7 +366,7 @@ def scale_output_layer(net: q.Surrogate, alpha: float):
 
 def compare(p: q.Params, net: q.Surrogate, init_mode: int, seed: int = 12345, tol: float = 1e-6, every: int = 100,
             max_steps: int = 200000, scales=(1.0, 0.5, 0.25), tail_tol: float | None = None, hold: int = 0,
-            log=None) -> dict:
+            log=None, short: bool = False) -> dict:
     """Plain relaxation and warm-started relaxation of the same start on one context."""
     Q0 = q.init_field(p, init_mode, 0.5, 0.0, seed)
     out = {}
@@ -280,7 +379,7 @@ def compare(p: q.Params, net: q.Surrogate, init_mode: int, seed: int = 12345, to
             if name == "plain":
                 r = relax_plain(s, tol, every, max_steps)
             else:
-                r = relax_warm(s, net, tol, every, max_steps, scales, tail_tol, hold)
+                r = relax_warm(s, net, tol, every, max_steps, scales, tail_tol, hold, short)
             o = s.observables()
             r.update({"final_F": o["total"], "defects": int(o["defects"]), "plaquettes": int(o["plaquettes"])})
             out[name] = r

@@ -119,9 +119,11 @@ class _FakeNet:
     def _one(self, s, b):
         ok = self.decide(b, len([c for c in self.calls if c[0] == b]))
         self.calls.append((b, ok))
+        F_before = s.F(b)
         if ok:
             s.a[b] *= self.jump
-        return {"accepted": int(ok), "scale": 1.0, "ms_network": 1.0}
+        return {"accepted": int(ok), "scale": 1.0, "ms_network": 1.0, "F_before": F_before,
+                "F_after": s.F(b) if ok else F_before * (1.0 - 1e-3)}
 
     def apply(self, s, mode, scales=None):
         s.log.append(("apply", 0))
@@ -199,3 +201,33 @@ def test_plain_tail_stops_learned_steps(ws):
     rb0 = ws.relax_batch(sb0, 2, _FakeNet(lambda b, k: False), tol=1e-6, every=100, max_steps=100000,
                          backoff=False)
     assert rb["steps"] == rb0["steps"] and rb["jumps_rejected"] < rb0["jumps_rejected"]
+
+
+def test_short_cycles_test_from_the_energy_after_the_step(ws):
+    """Short cycles: a learned step, `every` plain steps, one energy.  The test interval starts at the
+    check's F_after of a kept step, and a cycle whose step was kept never ends the run."""
+    every_kept = _FakeNet(lambda b, k: True, jump=1e-9)
+    s = _FakeSolver([100.0])
+    r = ws.relax_warm(s, every_kept, tol=1e-6, every=100, max_steps=3000, short=True)
+    assert not r["converged"] and r["steps"] == 3000 and r["jumps_accepted"] == 30
+    kinds = [e[0] for e in s.log]
+    assert kinds[0] == "energies"
+    assert all(kinds[i:i + 3] == ["apply", "step", "energies"] for i in range(1, len(kinds), 3))
+    # with the tail rule: the first kept step leaves |dF/F| from its F_after far below 1e-5, so the next
+    # cycle proposes nothing and converges (an interval starting before the step would not)
+    s = _FakeSolver([100.0])
+    r = ws.relax_warm(s, _FakeNet(lambda b, k: True, jump=1e-9), tol=1e-6, every=100, max_steps=3000,
+                      short=True, tail_tol=1e-5)
+    assert r["converged"] and r["steps"] == 200 and r["jumps_accepted"] == 1 and r["jumps_rejected"] == 0
+
+
+def test_short_batch_cycles_match_the_plain_test_for_refused_boxes(ws):
+    """Batched short cycles: box 0's steps are kept (it converges two cycles in, by the tail rule),
+    box 1's are refused, so its test is relax_plain's on the same 100-step intervals."""
+    s = _FakeSolver([100.0, 700.0])
+    r = ws.relax_batch(s, 2, _FakeNet(lambda b, k: b == 0, jump=1e-9), tol=1e-6, every=100, max_steps=100000,
+                       tail_tol=1e-5, short=True)
+    assert r["converged"] == 2 and r["steps"][0] == 200
+    alone = ws.relax_plain(_FakeSolver([700.0]), tol=1e-6, every=100, max_steps=100000)
+    assert r["steps"][1] == alone["steps"]
+    assert ("retire", (0,)) in s.log and not any(s.active)
