@@ -602,6 +602,37 @@ def test_batched_warm_start_relaxes_every_box(qsr):
     assert [int(o["defects"]) for o in pl["obs"]] == [int(o["defects"]) for o in wa["obs"]]
 
 
+def test_batched_warm_start_short_cycles_meet_the_bar(qsr):
+    """The same four droplets with short cycles (a learned step and 100 plain steps, tested from
+    the energy right after the step, never converging in a cycle whose step was kept) and the tail
+    rule: every box converges in fewer plain steps than plain and than the two-block cycles' total,
+    at its plain energy to 1e-5 with the same defect count."""
+    from lcdrsimulator_amd import warmstart as ws
+
+    params = _ladder_boxes(qsr, radii=(24.0, 30.0, 36.0, 40.0))
+    fields = [qsr.init_field(p, 1, 0.5, seed=0) for p in params]
+    out = {}
+    with qsr.Surrogate(seed=None) as zero:
+        zero.params = np.zeros(S.NPARAM, dtype=np.float32)
+        zero.set_input(qsr.SURR_INPUT_INCREMENT, HORIZON)
+        for name, net, short in (("plain", None, False), ("long", zero, False), ("short", zero, True)):
+            with qsr.Solver(params) as s:
+                for b, Q in enumerate(fields):
+                    s.upload(Q, box=b)
+                r = ws.relax_batch(s, len(params), net, max_steps=40000, tail_tol=1e-5, short=short)
+                r["obs"] = [s.observables(box=b, want_xi=False) for b in range(len(params))]
+            out[name] = r
+    pl, lo, sh = out["plain"], out["long"], out["short"]
+    rel = [abs(a["total"] - b["total"]) / abs(a["total"]) for a, b in zip(pl["obs"], sh["obs"])]
+    print(f"  steps plain {pl['steps']} two-block {lo['steps']} short {sh['steps']}; short: "
+          f"{sh['jumps_accepted']} kept, {sh['jumps_rejected']} refused; rel dF {', '.join(f'{x:.1e}' for x in rel)}")
+    assert pl["converged"] == lo["converged"] == sh["converged"] == len(params)
+    assert all(w < p for w, p in zip(sh["steps"], pl["steps"]))
+    assert sh["total_steps"] <= lo["total_steps"]
+    assert max(rel) <= 1e-5
+    assert [int(o["defects"]) for o in pl["obs"]] == [int(o["defects"]) for o in sh["obs"]]
+
+
 def test_increment_training_loss_matches_the_host(qsr):
     """The increment snapshots, their RMS s, the target scaling 1 / (N s), the pair weight and the
     modulated prediction, pinned together: with a zero network (y = 0, prediction u / s) and one
