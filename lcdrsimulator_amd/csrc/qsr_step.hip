@@ -964,6 +964,10 @@ __global__ __launch_bounds__(TK* TJv, (!NOISE && BUF) ? 4 : 3) void step_aniso_k
                 }
             }
         }
+        // from here to the barrier the wave runs at raised priority, so a wave done with its output
+        // pass issues the next plane's loads and reaches the barrier ahead of others' arithmetic
+        // (about 2 % per launch at 256^3; the results are the same)
+        __builtin_amdgcn_s_setprio(2);
         // the t-1 half of plane t+1's mixed terms: Q(t+1) and the differences of plane t, both resident
         if constexpr (hasL3) fold_x(Qn[1 * P2 + lq], Qn[2 * P2 + lq], ry1, rz1, pxm);
         // loads for the next iteration: row mask of plane t+1, own and ring Q(t+3), halo of
@@ -987,6 +991,7 @@ __global__ __launch_bounds__(TK* TJv, (!NOISE && BUF) ? 4 : 3) void step_aniso_k
         for (int a = 0; a < 3; ++a) r2[a] = r3[a];
         mcur = mnext;
         lds_barrier();
+        __builtin_amdgcn_s_setprio(0);
     }
 }
 
