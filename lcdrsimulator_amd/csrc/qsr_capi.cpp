@@ -509,7 +509,7 @@ int enqueue_steps(Ctx& c, int nsteps, bool even_swaps = false) {
         for (int p = 0; p < pairs; ++p, s += 2) {
             const qsrk::StepDyn* dyn = c.d_dyn + (size_t)s * nb;
             HIP_TRY(qsrk::launch_step2(c.ntiles2, c.st, c.q[c.cur], c.q[1 - c.cur], c.cs, c.d_boxes, c.d_tiles2, dyn,
-                                       dyn + nb, c.d_step_rows, c.ch2));
+                                       dyn + nb, c.d_step_rows, c.ch2, c.cur));
             c.cur = 1 - c.cur;
         }
     }
@@ -719,7 +719,7 @@ int qsr_profile_steps(void* ctx, int nsteps, double* main_ms, double* face_ms) {
         HIP_TRY(hipEventRecord(ev[3 * l], c->st));
         if (pairs)
             HIP_TRY(qsrk::launch_step2(c->ntiles2, c->st, qin, qout, c->cs, c->d_boxes, c->d_tiles2, dyn, dyn + nb,
-                                       c->d_step_rows, c->ch2));
+                                       c->d_step_rows, c->ch2, c->cur));
         else if (c->ntiles > 0)
             HIP_TRY(qsrk::launch_step(c->elastic, c->variant, c->noise, c->ntiles, c->st, qin, qout, c->cs, c->d_boxes, c->d_tiles,
                                       dyn, nb, c->d_step_rows, c->ch, nullptr));
@@ -814,7 +814,7 @@ int qsr_autotune(void* ctx, int trial_steps, int* best_variant, int* best_ch, do
                 for (int s = -1; s < trial_steps; ++s) {
                     if (s == 0) HIP_TRY(hipEventRecord(c->ev[0], c->st));
                     HIP_TRY(qsrk::launch_step2(c->ntiles2, c->st, c->q[c->cur], c->q[1 - c->cur], c->cs, c->d_boxes,
-                                               c->d_tiles2, c->d_dyn, c->d_dyn + nb, c->d_step_rows, c->ch2));
+                                               c->d_tiles2, c->d_dyn, c->d_dyn + nb, c->d_step_rows, c->ch2, c->cur));
                 }
                 HIP_TRY(hipEventRecord(c->ev[1], c->st));
                 HIP_TRY(hipEventSynchronize(c->ev[1]));

@@ -106,9 +106,11 @@ hipError_t launch_step(int elastic, int variant, bool noise, int ntiles, hipStre
                        const RowMask* rows, int ch, const float4* imex);
 // Two explicit L1 / kappa steps per launch (dyn1, dyn2: the two steps' per-box rows) over
 // tiles of X2_OUT_K x X2_OUT_J outputs; only for contexts without noise, live box faces or
-// 64-bit offsets (5 x cs floats below 4 GiB)
+// 64-bit offsets (5 x cs floats below 4 GiB).  reverse != 0 runs the tile list back to front
+// (callers alternate it with the buffer parity so each launch first reads what the last one wrote last)
 hipError_t launch_step2(int ntiles, hipStream_t st, const float* qin, float* qout, long long cs, const BoxDev* boxes,
-                        const int4* tiles, const StepDyn* dyn1, const StepDyn* dyn2, const RowMask* rows, int ch);
+                        const int4* tiles, const StepDyn* dyn1, const StepDyn* dyn2, const RowMask* rows, int ch,
+                        int reverse);
 // face_list: (box, i, j, k) of every box-face voxel that can change (built on the host)
 hipError_t launch_faces(long long nlive, hipStream_t st, const float* qin, float* qout, long long cs,
                         const BoxDev* boxes, const StepDyn* dyn, const int4* face_list, const RowMask* rows);

@@ -432,17 +432,22 @@ __global__ __launch_bounds__(TK* TJ, (!NOISE && BUF) ? 7 : 1) void step_l1_kerne
 // hand-over.  Usable when no box face can change (the face pass would have to run
 // between the two steps), without noise and without the semi-implicit right-hand
 // side; 32-bit buffer offsets (5 x cs floats below 4 GiB).
+// The stores are ordinary cache-allocating ones, so the memory-side cache still
+// holds much of the field when the next launch reads it.  reverse != 0 takes the
+// tile list back to front: launches alternate it, so the next launch first reads
+// the tiles written last, the ones the cache has kept.  Tiles are independent, so
+// the order changes no value.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(X2_TK* X2_TJ, 5) void step_l1x2_kernel(
     const float* __restrict__ qin, float* __restrict__ qout, long long cs, const BoxDev* __restrict__ boxes,
     const int4* __restrict__ tiles, const StepDyn* __restrict__ dyn1, const StepDyn* __restrict__ dyn2,
-    const RowMask* __restrict__ rows, int ch) {
+    const RowMask* __restrict__ rows, int ch, int reverse) {
     constexpr int TK = X2_TK, TJ = X2_TJ, NT = TK * TJ;
     constexpr int WA = TK + 2, PA = WA * (TJ + 2), SA = 5 * PA;  // input plane with its ring
     constexpr int PB = NT, SB = 5 * PB;                          // intermediate plane (region)
     __shared__ float la[2 * SA];
     __shared__ float lb[2 * SB];
-    const int4 t = tiles[blockIdx.x];
+    const int4 t = tiles[reverse ? gridDim.x - 1 - blockIdx.x : blockIdx.x];
     if (t.x < 0) return;  // padding tile of the XCD-ordered list (uniform over the block)
     const BoxDev B = boxes[t.x];
     const StepDyn D1 = dyn1[t.x], D2 = dyn2[t.x];
@@ -1386,10 +1391,11 @@ hipError_t launch_step(int elastic, int variant, bool noise, int ntiles, hipStre
 }
 
 hipError_t launch_step2(int ntiles, hipStream_t st, const float* qin, float* qout, long long cs, const BoxDev* boxes,
-                        const int4* tiles, const StepDyn* dyn1, const StepDyn* dyn2, const RowMask* rows, int ch) {
+                        const int4* tiles, const StepDyn* dyn1, const StepDyn* dyn2, const RowMask* rows, int ch,
+                        int reverse) {
     if (ntiles <= 0) return hipSuccess;
     hipLaunchKernelGGL(step_l1x2_kernel, dim3(ntiles), dim3(X2_TK * X2_TJ), 0, st, qin, qout, cs, boxes, tiles, dyn1,
-                       dyn2, rows, ch);
+                       dyn2, rows, ch, reverse);
     return hipGetLastError();
 }
 

@@ -22,6 +22,8 @@ def bench():
     ("droplet_l1l2l3_256", 1, 12, 1),
     ("droplet_l1_256", "x2", 10, 2),    # the two-step kernel, two steps per launch
     ("droplet_l1_256", "x2", 8, 2),
+    ("droplet_l1_256", "x2", 12, 2),
+    ("droplet_l1_256", "x2", 16, 2),
 ])
 def test_profile_figures_cover_the_picked_tunings(bench, workload, variant, planes, steps):
     """Each tuning has a traffic figure and a VALU-busy fraction, both from a committed summary
