@@ -519,7 +519,7 @@ int enqueue_steps(Ctx& c, int nsteps, bool even_swaps = false) {
         const qsrk::StepDyn* dyn = c.d_dyn + (size_t)s * nb;
         if (c.ntiles > 0)
             HIP_TRY(qsrk::launch_step(c.elastic, c.variant, c.noise, c.ntiles, c.st, qin, qout, c.cs, c.d_boxes, c.d_tiles, dyn, nb,
-                                      c.d_step_rows, c.ch, nullptr));
+                                      c.d_step_rows, c.ch, nullptr, c.cur));
         HIP_TRY(qsrk::launch_faces(c.nface, c.st, qin, qout, c.cs, c.d_boxes, dyn, c.d_face_list, c.d_step_rows));
         c.cur = 1 - c.cur;
     }
@@ -722,7 +722,7 @@ int qsr_profile_steps(void* ctx, int nsteps, double* main_ms, double* face_ms) {
                                        c->d_step_rows, c->ch2, c->cur));
         else if (c->ntiles > 0)
             HIP_TRY(qsrk::launch_step(c->elastic, c->variant, c->noise, c->ntiles, c->st, qin, qout, c->cs, c->d_boxes, c->d_tiles,
-                                      dyn, nb, c->d_step_rows, c->ch, nullptr));
+                                      dyn, nb, c->d_step_rows, c->ch, nullptr, c->cur));
         HIP_TRY(hipEventRecord(ev[3 * l + 1], c->st));
         HIP_TRY(qsrk::launch_faces(c->nface, c->st, qin, qout, c->cs, c->d_boxes, dyn, c->d_face_list,
                                    c->d_step_rows));

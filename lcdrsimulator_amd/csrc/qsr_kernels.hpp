@@ -100,10 +100,12 @@ constexpr int NPLAQ = 3;     // periodic plaquette slots
 // Host launchers (defined in qsr_step.hip / qsr_obs.hip)
 namespace qsrk {
 // elastic: 0 = L1/kappa stencil; otherwise the anisotropic stencil with the terms
-// of the mask compiled in (1 = L2, 2 = L3, 3 = both)
+// of the mask compiled in (1 = L2, 2 = L3, 3 = both).  reverse != 0 runs the L1/kappa stencil's
+// tile list back to front (callers alternate it with the buffer parity, so a launch first reads
+// the tiles the previous one wrote last; the anisotropic stencil ignores it)
 hipError_t launch_step(int elastic, int variant, bool noise, int ntiles, hipStream_t st, const float* qin, float* qout,
                        long long cs, const BoxDev* boxes, const int4* tiles, const StepDyn* dyn, int nboxes,
-                       const RowMask* rows, int ch, const float4* imex);
+                       const RowMask* rows, int ch, const float4* imex, int reverse = 0);
 // Two explicit L1 / kappa steps per launch (dyn1, dyn2: the two steps' per-box rows) over
 // tiles of X2_OUT_K x X2_OUT_J outputs; only for contexts without noise, live box faces or
 // 64-bit offsets (5 x cs floats below 4 GiB).  reverse != 0 runs the tile list back to front

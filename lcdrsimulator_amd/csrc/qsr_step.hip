@@ -299,16 +299,18 @@ __device__ __forceinline__ void l1_field(const BoxDev& B, const StepDyn& D, floa
 // Without noise the kernel fits 7 waves per SIMD (72 VGPRs, no spills; 6 waves uncapped): the
 // step is latency bound, so the extra wave pays.  The noisy variant (Philox + Box-Muller) keeps
 // its registers (5 waves); so do 64-bit-offset contexts, which would spill under the cap.
+// reverse != 0 takes the tile list back to front; launches alternate it, so a launch first
+// reads the tiles the previous one wrote last (tiles are independent: no value changes).
 template <int TK, int TJ, bool NOISE, bool BUF>
 __global__ __launch_bounds__(TK* TJ, (!NOISE && BUF) ? 7 : 1) void step_l1_kernel(const float* __restrict__ qin, float* __restrict__ qout,
                                                           long long cs, const BoxDev* __restrict__ boxes,
                                                           const int4* __restrict__ tiles,
                                                           const StepDyn* __restrict__ dyn, int nboxes,
                                                           const RowMask* __restrict__ rows, int ch,
-                                                          const float4* __restrict__ imex_tab) {
+                                                          const float4* __restrict__ imex_tab, int reverse) {
     constexpr int W = TK + 2, R = TJ + 2, PL = W * R, SLOT = 5 * PL;
     __shared__ float lds[2 * SLOT];
-    const int4 t = tiles[blockIdx.x];
+    const int4 t = tiles[reverse ? gridDim.x - 1 - blockIdx.x : blockIdx.x];
     if (t.x < 0) return;  // padding tile of the XCD-ordered list (uniform over the block)
     const BoxDev B = boxes[t.x];
     const StepDyn D = dyn[t.x];
@@ -1345,15 +1347,16 @@ void go_aniso(dim3 grid, dim3 block, hipStream_t st, const float* qin, float* qo
 
 template <int V, bool NZ, bool BF>
 void go_l1(dim3 grid, dim3 block, hipStream_t st, const float* qin, float* qout, long long cs, const BoxDev* boxes,
-           const int4* tiles, const StepDyn* dyn, int nboxes, const RowMask* rows, int ch, const float4* imex) {
+           const int4* tiles, const StepDyn* dyn, int nboxes, const RowMask* rows, int ch, const float4* imex,
+           int reverse) {
     hipLaunchKernelGGL(HIP_KERNEL_NAME(step_l1_kernel<tile_k(V), tile_j(V), NZ, BF>), grid, block, 0, st, qin, qout, cs,
-                       boxes, tiles, dyn, nboxes, rows, ch, imex);
+                       boxes, tiles, dyn, nboxes, rows, ch, imex, reverse);
 }
 
 template <int V>
 void go_step(int elastic, bool noise, bool buf, dim3 grid, dim3 block, hipStream_t st, const float* qin, float* qout,
              long long cs, const BoxDev* boxes, const int4* tiles, const StepDyn* dyn, int nboxes, const RowMask* rows,
-             int ch, const float4* imex) {
+             int ch, const float4* imex, int reverse) {
 #define QSR_ARGS grid, block, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex
 #define QSR_A(EL) \
     (buf ? (noise ? go_aniso<V, EL, true, true>(QSR_ARGS) : go_aniso<V, EL, false, true>(QSR_ARGS)) \
@@ -1364,11 +1367,11 @@ void go_step(int elastic, bool noise, bool buf, dim3 grid, dim3 block, hipStream
         case 3: QSR_A(3); break;
         default:
             if (buf) {
-                if (noise) go_l1<V, true, true>(QSR_ARGS);
-                else go_l1<V, false, true>(QSR_ARGS);
+                if (noise) go_l1<V, true, true>(QSR_ARGS, reverse);
+                else go_l1<V, false, true>(QSR_ARGS, reverse);
             } else {
-                if (noise) go_l1<V, true, false>(QSR_ARGS);
-                else go_l1<V, false, false>(QSR_ARGS);
+                if (noise) go_l1<V, true, false>(QSR_ARGS, reverse);
+                else go_l1<V, false, false>(QSR_ARGS, reverse);
             }
     }
 #undef QSR_A
@@ -1379,14 +1382,14 @@ void go_step(int elastic, bool noise, bool buf, dim3 grid, dim3 block, hipStream
 
 hipError_t launch_step(int elastic, int variant, bool noise, int ntiles, hipStream_t st, const float* qin, float* qout,
                        long long cs, const BoxDev* boxes, const int4* tiles, const StepDyn* dyn, int nboxes,
-                       const RowMask* rows, int ch, const float4* imex) {
+                       const RowMask* rows, int ch, const float4* imex, int reverse) {
     const dim3 grid(ntiles), block(TILE_THREADS);
     // 32-bit buffer offsets reach every element when 5 planes x cs floats fit 4 GiB
     const bool buf = cs < (1LL << 32) / 20;
     if (variant == 1)
-        go_step<1>(elastic, noise, buf, grid, block, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex);
+        go_step<1>(elastic, noise, buf, grid, block, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex, reverse);
     else
-        go_step<0>(elastic, noise, buf, grid, block, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex);
+        go_step<0>(elastic, noise, buf, grid, block, st, qin, qout, cs, boxes, tiles, dyn, nboxes, rows, ch, imex, reverse);
     return hipGetLastError();
 }
 
